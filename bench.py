@@ -1,0 +1,174 @@
+"""Throughput of the XFluids per-step hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--form plain|mixed]
+
+Workload: 2D inert shock-bubble interaction, 3 species (H2, O2, N2) with
+NASA-7 (JANAF) thermodynamics, 1024 x 512 cells, WENO5-Z characteristic
+fluxes over Ns+4 = 7 variables, SSP-RK3, CFL 0.4 (cases/shock-bubble.json).
+One step = dt reduction + 3 RK stages (ghost fill, conservative->primitive
+with Newton temperature, x and y characteristic sweeps, RK update).
+Inputs are resident in HBM before the timed region.  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import xfluids_amd as xf  # noqa: E402
+
+PEAK_HBM_GBS = 8000.0       # MI355X HBM3E
+PEAK_FP64_TFLOPS = 78.6     # MI355X FP64 (vector and matrix)
+CASE = "shock-bubble"
+NX, NY = 1024, 512
+# Compulsory HBM bytes per cell-update of one full step for Ns = 3, E = 7:
+# 8 * (8E + 6) (stage traffic of U, U1 and the T guess).
+ALG_BYTES_PER_CELL = 8 * (8 * 7 + 6)
+
+
+def face_flops(E, ncop, scheme):
+    """FP64 operations per characteristic face flux of the plain algorithm
+    (counted per operation class; see DESIGN.md, "Algorithmic flops")."""
+    from xfluids_amd.opcount import face_flop_count
+    return face_flop_count(E, ncop, scheme)
+
+
+def run(args):
+    s = xf.Setup(xf.case_path(CASE), nx=NX, ny=NY)
+    d = s.desc
+    precision = xf.XF_MIXED if args.form == "mixed" else xf.XF_PLAIN_FP64
+    U, T = s.initial(CASE, seed=args.seed)
+    ctx = s.context(precision=precision)
+    ctx.upload(U, T)
+    ctx.initialize()
+    ctx.steps(args.warmup, graph=False)
+    ctx.synchronize()
+    ctx.set_profiling(True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.steps_async(args.steps, graph=False)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    kt = ctx.kernel_times()
+    ctx.set_profiling(False)
+    Uf, Tf = ctx.download(0)
+    ctx.close()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    cells = d.X_inner * d.Y_inner * d.Z_inner
+    value = cells * args.steps / (t1 - t0) / 1e6
+    finite = bool(np.all(np.isfinite(Uf)))
+    return d, value, ms_per_step, kt, finite
+
+
+def roofline(d, kt):
+    # dominant kernel: largest total device time
+    name = max(kt, key=lambda k: kt[k][0] * kt[k][1])
+    avg_ms, cnt = kt[name]
+    E, ncop = d.E, (d.NS - 1 if d.cop else 0)
+    if name.startswith("sweep"):
+        faces = {"sweep_x": (d.X_inner + 1) * d.Y_inner * d.Z_inner,
+                 "sweep_y": d.X_inner * (d.Y_inner + 1) * d.Z_inner,
+                 "sweep_z": d.X_inner * d.Y_inner * (d.Z_inner + 1)}[name]
+        flops = faces * face_flops(E, ncop, d.scheme)
+        achieved = flops / (avg_ms * 1e-3) / 1e12
+        return {"kernel": name, "bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+                "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                "avg_launch_ms": avg_ms, "launches": cnt, "alg_flops_per_launch": flops}
+    bytes_ = d.Xmax * d.Ymax * d.Zmax * 8 * (E + 10)
+    achieved = bytes_ / (avg_ms * 1e-3) / 1e9
+    return {"kernel": name, "bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": achieved / PEAK_HBM_GBS, "traffic": None, "avg_launch_ms": avg_ms, "launches": cnt}
+
+
+def accuracy_check():
+    """Plain form vs the CPU oracle on a reduced grid (L1 relative error)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import Oracle, config_from_setup
+    s = xf.Setup(xf.case_path(CASE), nx=128, ny=64)
+    d = s.desc
+    U, T = s.initial(CASE, seed=3)
+    ctx = s.context()
+    ctx.upload(U, T)
+    ctx.initialize()
+    ctx.steps(10)
+    Ug, Tg = ctx.download(0)
+    ctx.close()
+    o = Oracle(config_from_setup(s))
+    o.set_state(U, T)
+    o.initialize()
+    o.steps(10)
+    Uo, To = o.field("U").reshape(-1, d.E), o.field("T")
+    Ug = Ug.reshape(-1, d.E)
+    l1_rho = float(np.abs(Ug[:, 0] - Uo[:, 0]).sum() / np.abs(Uo[:, 0]).sum())
+    l1_T = float(np.abs(Tg - To).sum() / np.abs(To).sum())
+    bitwise = float(np.mean(Ug.view(np.uint64) == Uo.view(np.uint64)))
+    return {"vs": "cpu oracle", "grid": "128x64", "steps": 10, "l1_rel_rho": l1_rho, "l1_rel_T": l1_T,
+            "bitwise_fraction_U": bitwise}
+
+
+def cpu_baseline(sample_steps):
+    """Oracle (plain C, FP64, OpenMP) on the full 1024x512 grid for a few steps."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import Oracle, config_from_setup
+    s = xf.Setup(xf.case_path(CASE), nx=NX, ny=NY)
+    d = s.desc
+    U, T = s.initial(CASE, seed=1)
+    o = Oracle(config_from_setup(s))
+    o.set_state(U, T)
+    o.initialize()
+    o.steps(1)
+    t0 = time.perf_counter()
+    o.steps(sample_steps)
+    dt = time.perf_counter() - t0
+    cells = d.X_inner * d.Y_inner
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": cells * sample_steps / dt / 1e6, "unit": "Mcell-updates/s", "cores": threads,
+            "kind": "port", "sample": "%d steps of the %dx%d workload after 1 warm-up step (%.1f s)" %
+            (sample_steps, NX, NY, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--form", choices=["plain", "mixed"], default="plain")
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    d, value, ms, kt, finite = run(args)
+    line = {
+        "metric": "Mcell-updates/s per RK3 step, plain FP64 vs mixed/batched, at stated L1 err",
+        "value": value,
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "2D inert shock-bubble, Ns=3 (H2,O2,N2), JANAF, WENO5-Z, SSP-RK3, CFL 0.4",
+                   "grid": "%dx%d" % (NX, NY), "form": args.form, "finite": finite},
+        "roofline": roofline(d, kt),
+        "kernels_ms": {k: v[0] for k, v in kt.items()},
+    }
+    line["accuracy"] = accuracy_check()
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(args.cpu_steps)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

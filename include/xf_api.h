@@ -1,0 +1,180 @@
+/*
+ * xf_api.h -- C-ABI of the MI355X (gfx950) implementation of the XFluids
+ * per-step hot path.
+ *
+ * Plain C types only: pointers, sizes, POD structs.  Every call returns an
+ * int status: 0 = ok, > 0 = physical error code (XF_ERR_*; the offending cell
+ * is available from xf_last_error), < 0 = HIP/runtime error (XF_EHIP...).
+ * No exception crosses the ABI.  One context per host thread; all calls
+ * enqueue on the context's HIP stream and return after the stream drained,
+ * except the *_async variants.
+ *
+ * Layout contract: host arrays exchanged through xf_upload_state /
+ * xf_download_* use the reference's array-of-structures order
+ * U[Emax*id + n], y[NS*id + k], id = Xmax*Ymax*k + Xmax*j + i
+ * (src/solver_UpdateStates/Update_kernels.hpp:14-15).  On the device the
+ * fields are structure-of-arrays.
+ */
+#ifndef XF_API_H
+#define XF_API_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define XF_MAXNS 16
+#define XF_MAXNR 40
+
+/* status codes */
+#define XF_OK 0
+#define XF_ERR_RHO_Y 1     /* EstimateYiKernel (Estimate_kernels.hpp:5-102)            */
+#define XF_ERR_PRIM 2      /* EstimatePrimitiveVarKernel (Estimate_kernels.hpp:104-162) */
+#define XF_ERR_NAN 3       /* EstimateFluidNANKernel (src/Fluids.cpp:47-87)             */
+#define XF_ERR_CHEM 5      /* chemistry substep bound reached                          */
+#define XF_EINVAL (-1)     /* invalid argument / unsupported configuration             */
+#define XF_EHIP (-2)       /* HIP runtime error                                        */
+#define XF_ENOKERNEL (-3)  /* no kernel instance for this (NS, scheme, precision)      */
+
+/* boundary codes: BConditions, src/include/global_setup.h:96-110 */
+#define XF_BC_INFLOW 0
+#define XF_BC_OUTFLOW 1
+#define XF_BC_SYMMETRY 2
+#define XF_BC_PERIODIC 3
+#define XF_BC_NSLIPWALL 4
+#define XF_BC_VISCWALL 5
+#define XF_BC_SLIPWALL 6
+
+/* face-flux schemes */
+#define XF_WENO5JS 0 /* weno5old_GPU, schemes/WENO5s_schemes.hpp:71-95 */
+#define XF_WENO5Z 1  /* weno5Z_P + weno5Z_M, schemes/WENO5s_schemes.hpp:246-316 */
+
+/* precision forms */
+#define XF_PLAIN_FP64 0 /* everything FP64, bitwise reproducible vs the CPU restatement */
+#define XF_MIXED 1      /* FP32 smoothness indicators, weights and projections */
+
+/* chemistry drivers */
+#define XF_CHEM_PER_CELL 0 /* one lane per cell */
+#define XF_CHEM_QUEUE 1    /* persistent lanes draining a device work queue */
+
+/* Case descriptor: the runtime form of the reference's Block/MeshSize
+   (src/include/global_setup.h:162-200, derived as Setup::init,
+   src/read_ini/src/iniset.cpp:290-369) plus the compile-time switches
+   (NUM_SPECIES, NUM_REA, COP, Thermo, SCHEME_ORDER, CHEME_SPLITTING). */
+typedef struct xf_case_desc {
+    int Xmax, Ymax, Zmax;
+    int X_inner, Y_inner, Z_inner;
+    int Bwidth_X, Bwidth_Y, Bwidth_Z;
+    int DimX, DimY, DimZ;
+    double dx, dy, dz, _dx, _dy, _dz;
+    double CFLnumber;
+    int BC[6];          /* XMIN, XMAX, YMIN, YMAX, ZMIN, ZMAX */
+    int NS;             /* species (1 = single-species ideal gas, NO-COP) */
+    int NR;             /* reactions */
+    int cop;            /* multi-component (COP) */
+    int thermo;         /* 1 = NASA-9 (Thermo=1), 0 = JANAF/NASA-7 (Thermo=0) */
+    int scheme;         /* XF_WENO5JS or XF_WENO5Z */
+    int react;          /* Sources_React */
+    int strang;         /* 1 = Strang splitting, 0 = Lie */
+    double ncop_gamma;  /* NCOP_Gamma */
+    int precision;      /* XF_PLAIN_FP64 or XF_MIXED */
+    int chem_driver;    /* XF_CHEM_PER_CELL or XF_CHEM_QUEUE */
+} xf_case_desc;
+
+typedef struct xf_ctx xf_ctx;
+
+/* ---- context ----------------------------------------------------------- */
+/* Allocates all device fields (U, U1, LU, primitives, scratch) for one case.
+   Replaces Fluid::AllocateFluidMemory (src/Fluids.cpp:270-583). */
+int xf_create(const xf_case_desc *desc, xf_ctx **out);
+void xf_destroy(xf_ctx *ctx);
+int xf_num_eqn(const xf_ctx *ctx);
+/* The descriptor the context was created with. */
+int xf_get_desc(const xf_ctx *ctx, xf_case_desc *out);
+size_t xf_num_cells(const xf_ctx *ctx);
+const char *xf_error_string(int status);
+
+/* Thermal tables in the reference's layout: Hia[NS*7*3], Hib[NS*2*3],
+   Wi[NS] in kg/mol.  Ri = Ru/Wi and _Wi = 1/Wi are derived as
+   src/read_ini/src/thermal.cpp:158-161.  Replaces Setup::ReadThermal +
+   CpyToGPU (src/read_ini/src/gpucopy.cpp:4-36). */
+int xf_set_thermo(xf_ctx *ctx, const double *Hia, const double *Hib, const double *Wi);
+
+/* Mechanism tables in the reference's layout: Nu_f[NR*NS], Nu_b[NR*NS],
+   React_ThirdCoef[NR*NS], Rargus[NR*6]; Nu_d and third_ind derived as
+   src/read_ini/src/reaction.cpp:31-40,142-172.  Replaces
+   Setup::ReadReactions (src/read_ini/src/reaction.cpp:6-138). */
+int xf_set_mechanism(xf_ctx *ctx, const int *Nu_f, const int *Nu_b, const double *React_ThirdCoef,
+                     const double *Rargus);
+
+/* ---- state exchange (AoS host <-> SoA device) ---------------------------- */
+/* which: 0 = U, 1 = U1.  T is the per-cell temperature (Newton initial guess).
+   Uploading U (which = 0) also sets U1 = U, as the reference's initial-
+   condition kernels do (e.g. 2D-EulerVortex/ini_sample.hpp:84-88). */
+int xf_upload_state(xf_ctx *ctx, int which, const double *U_aos, const double *T);
+int xf_download_state(xf_ctx *ctx, int which, double *U_aos, double *T);
+/* Primitives of the last xf_update_states call (all padded cells); any
+   pointer may be NULL.  y_aos is NS per cell. */
+int xf_download_prims(xf_ctx *ctx, double *rho, double *p, double *u, double *v, double *w, double *H,
+                      double *c, double *gamma, double *e, double *y_aos);
+/* LU (flux divergence) of the last xf_compute_lu, AoS, interior values. */
+int xf_download_lu(xf_ctx *ctx, double *LU_aos);
+/* GLF maxima eigen_block[3][XF_MAXNS+4] and uvw_c_max[3] of the last dt. */
+int xf_download_maxima(xf_ctx *ctx, double *eigen_block, double *uvw_c_max);
+
+/* ---- the stage seam ---------------------------------------------------- */
+/* FluidBoundaryCondition (src/solver_BCs/BCs_block.cpp:4-218) on U (0) or U1 (1). */
+int xf_boundary(xf_ctx *ctx, int which);
+/* UpdateFluidStateFlux (src/solver_UpdateStates/UpdateStates_block.cpp:7-191):
+   rho, Y, u, v, w, p, H, c, gamma, T (Newton), validation.  Also folds the
+   GLF maxima (GetLU's eigen/GLF passes, ConVenction_block.hpp:42-172) and the
+   CFL maxima (GetDt, GlobalDt_block.hpp:237-277) into the same pass. */
+int xf_update_states(xf_ctx *ctx, int which);
+/* GetLU (src/solver_Reconstruction/FDM_Method/ConVenction_block.hpp:10-617)
+   without positivity limiter and viscous terms: characteristic WENO face
+   fluxes and LU = -div F. */
+int xf_compute_lu(xf_ctx *ctx, int which);
+/* UpdateURK3rd (src/solver_UpdateStates/UpdateStates_block.cpp:193-211). */
+int xf_rk3_stage(xf_ctx *ctx, int stage, double dt);
+/* GetDt (src/solver_GetDt/GlobalDt_block.hpp:6-114) from the maxima of the
+   last xf_update_states. */
+int xf_get_dt(xf_ctx *ctx, double *dt_out);
+/* ChemeODEQ2Solver (src/solver_Reaction/Reaction_block.hpp:4-38): CHEMEQ2
+   over dt_chem on every interior cell, using the primitives of the last
+   xf_update_states(0).  substeps (optional, NULL ok) receives the per-cell
+   predictor-pass counts (interior cells, row-major i fastest). */
+int xf_chemistry(xf_ctx *ctx, double dt_chem, int *substeps);
+/* XFLUIDS::Reaction (src/XFLUIDS.cpp:625-642): UpdateStates(U), CHEMEQ2 over
+   dt/2 (Strang) or dt (Lie), NaN check. */
+int xf_reaction(xf_ctx *ctx, double dt);
+/* EstimateFluidNAN (src/Fluids.cpp:963-1040). */
+int xf_estimate_nan(xf_ctx *ctx, int flag);
+
+/* ---- whole steps --------------------------------------------------------- */
+/* main.cpp:44-47: BoundaryCondition(U) + UpdateStates(U). */
+int xf_initialize(xf_ctx *ctx);
+/* nsteps of XFLUIDS::Evolution's ordering (src/XFLUIDS.cpp:172-225) with a
+   device-resident dt and fused stage kernels; dt values written to dt_out
+   (may be NULL).  use_graph = 1 captures one step as a hipGraph and replays it. */
+int xf_steps(xf_ctx *ctx, int nsteps, int use_graph, double *dt_out);
+/* Enqueue nsteps without waiting (for timing with events on the stream). */
+int xf_steps_async(xf_ctx *ctx, int nsteps, int use_graph);
+int xf_synchronize(xf_ctx *ctx);
+/* The context's HIP stream (hipStream_t as void*). */
+void *xf_stream(xf_ctx *ctx);
+
+/* ---- diagnostics --------------------------------------------------------- */
+int xf_last_error(xf_ctx *ctx, int *code, int *i, int *j, int *k);
+/* Average device time (ms) per launch of each kernel class since the last
+   reset, measured with HIP events on the context stream when profiling is on. */
+int xf_set_profiling(xf_ctx *ctx, int on);
+int xf_kernel_times(xf_ctx *ctx, const char **names, double *avg_ms, int *counts, int cap);
+/* Evaluate the device exp and log (the fixed-sequence FP64 functions every
+   kernel uses) on n host values; for bit-level checks against the CPU. */
+int xf_debug_math(const double *x, double *exp_out, double *log_out, int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

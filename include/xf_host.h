@@ -1,0 +1,52 @@
+/*
+ * xf_host.h -- C-ABI of the host layer above the kernels: case loading
+ * from the reference's settings/*.json + runtime.dat formats, synthetic
+ * initial states, and a driver that sequences the stage seam exactly as
+ * XFLUIDS::Evolution does.
+ */
+#ifndef XF_HOST_H
+#define XF_HOST_H
+
+#include "xf_api.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct xf_setup xf_setup;
+
+/* Setup::Setup (src/read_ini/src/constructor.cpp:12-77): read the settings
+   JSON and the runtime.dat tables; geometry as Setup::init
+   (src/read_ini/src/iniset.cpp:290-369).  err (may be NULL) receives a
+   message of at most errcap bytes. */
+int xf_setup_load(const char *settings_json, const char *runtime_root, xf_setup **out, char *err, size_t errcap);
+void xf_setup_free(xf_setup *s);
+/* Override mesh resolution / CFL / scheme / thermo before use (recomputes
+   the geometry as Setup::init); pass 0 (or < 0 for cfl) to keep a value. */
+int xf_setup_override(xf_setup *s, int nx, int ny, int nz, double cfl, int scheme, int thermo);
+int xf_setup_desc(const xf_setup *s, xf_case_desc *out);
+/* Thermal tables Hia[NS*21], Hib[NS*6], Wi[NS] (kg/mol). */
+int xf_setup_thermo(const xf_setup *s, double *Hia, double *Hib, double *Wi);
+/* Mechanism tables Nu_f[NR*NS], Nu_b[NR*NS], third[NR*NS], Rargus[NR*6]. */
+int xf_setup_mechanism(const xf_setup *s, int *Nu_f, int *Nu_b, double *third, double *Rargus);
+/* Mass fractions outside / inside the bubble (species_list.dat ratios). */
+int xf_setup_mass_fractions(const xf_setup *s, double *Y_out, double *Y_in);
+/* Species name n. */
+const char *xf_setup_species(const xf_setup *s, int n);
+/* Domain origin and size [xmin, ymin, zmin, Lx, Ly, Lz]. */
+int xf_setup_domain(const xf_setup *s, double *dom6);
+/* Synthetic initial state (AoS U over padded cells, T per cell). */
+int xf_setup_initial(const xf_setup *s, const char *workload, unsigned long long seed, double *U_aos, double *T);
+/* Create a device context for the setup and upload thermo + mechanism. */
+int xf_setup_context(const xf_setup *s, int precision, int chem_driver, xf_ctx **out);
+
+/* XFLUIDS::Evolution (src/XFLUIDS.cpp:172-225) through the stage seam, one
+   call per reference stage: ComputeTimeStep, [Reaction], 3 x
+   (BoundaryCondition, UpdateStates, ComputeLU, EstimateNAN, UpdateU),
+   [Reaction].  Call xf_initialize first (main.cpp:44-47). */
+int xf_evolve_staged(xf_ctx *ctx, int nsteps, double *dt_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

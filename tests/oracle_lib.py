@@ -1,0 +1,183 @@
+"""ctypes access to the CPU oracle (oracle/_build/liboracle.so).
+
+Test infrastructure: the oracle is the checker, never the thing measured or
+shipped.  Builds the oracle configuration from an xfluids_amd.Setup (the
+same thermo / mechanism tables the GPU context receives).
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+MAXNS, MAXNR = 16, 40
+MAXE = MAXNS + 4
+
+
+class OraConfig(C.Structure):
+    _fields_ = [("Xmax", C.c_int), ("Ymax", C.c_int), ("Zmax", C.c_int),
+                ("X_inner", C.c_int), ("Y_inner", C.c_int), ("Z_inner", C.c_int),
+                ("Bwidth_X", C.c_int), ("Bwidth_Y", C.c_int), ("Bwidth_Z", C.c_int),
+                ("DimX", C.c_int), ("DimY", C.c_int), ("DimZ", C.c_int),
+                ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
+                ("_dx", C.c_double), ("_dy", C.c_double), ("_dz", C.c_double), ("CFLnumber", C.c_double),
+                ("BC", C.c_int * 6),
+                ("NS", C.c_int), ("NR", C.c_int), ("cop", C.c_int), ("thermo", C.c_int), ("scheme", C.c_int),
+                ("react", C.c_int), ("strang", C.c_int), ("ncop_gamma", C.c_double),
+                ("Hia", C.c_double * (MAXNS * 21)), ("Hib", C.c_double * (MAXNS * 6)),
+                ("Wi", C.c_double * MAXNS), ("_Wi", C.c_double * MAXNS), ("Ri", C.c_double * MAXNS),
+                ("Nu_f", C.c_int * (MAXNR * MAXNS)), ("Nu_b", C.c_int * (MAXNR * MAXNS)),
+                ("Nu_d", C.c_int * (MAXNR * MAXNS)), ("third_ind", C.c_int * MAXNR),
+                ("React_ThirdCoef", C.c_double * (MAXNR * MAXNS)), ("Rargus", C.c_double * (MAXNR * 6)),
+                ("use_libm", C.c_int)]
+
+
+_lib = None
+_DP = C.POINTER(C.c_double)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
+        L = C.CDLL(ORACLE_SO)
+        v = C.c_void_p
+        P = C.POINTER(OraConfig)
+        sig = {
+            "ora_create": (v, [P]), "ora_destroy": (None, [v]), "ora_num_eqn": (C.c_int, [v]),
+            "ora_num_cells": (C.c_size_t, [v]), "ora_field": (_DP, [v, C.c_char_p]),
+            "ora_boundary": (None, [v, C.c_int]), "ora_update_states": (C.c_int, [v, C.c_int]),
+            "ora_compute_lu": (None, [v, C.c_int]), "ora_rk3": (None, [v, C.c_int, C.c_double]),
+            "ora_get_dt": (C.c_double, [v]), "ora_chemistry": (None, [v, C.c_double]),
+            "ora_reaction": (C.c_int, [v, C.c_double]), "ora_estimate_nan": (C.c_int, [v, C.c_int]),
+            "ora_initialize": (C.c_int, [v]), "ora_steps": (C.c_int, [v, C.c_int, _DP]),
+            "ora_exp": (C.c_double, [C.c_double]), "ora_log": (C.c_double, [C.c_double]),
+            "ora_cp_i": (C.c_double, [P, C.c_double, C.c_int]), "ora_h_i": (C.c_double, [P, C.c_double, C.c_int]),
+            "ora_s_i": (C.c_double, [P, C.c_double, C.c_int]), "ora_g_i": (C.c_double, [P, C.c_double, C.c_int]),
+            "ora_e_i": (C.c_double, [P, C.c_double, C.c_int]),
+            "ora_mix_cp": (C.c_double, [P, _DP, C.c_double]), "ora_mix_cv": (C.c_double, [P, _DP, C.c_double]),
+            "ora_mix_h": (C.c_double, [P, _DP, C.c_double]), "ora_mix_e": (C.c_double, [P, _DP, C.c_double]),
+            "ora_mix_R": (C.c_double, [P, _DP]), "ora_get_T": (C.c_double, [P, _DP, C.c_double, C.c_double]),
+            "ora_weno5js": (C.c_double, [_DP, _DP]), "ora_weno5z": (C.c_double, [_DP, _DP]),
+            "ora_eigen_lr": (None, [P, C.c_int, C.c_int, _DP, _DP, _DP, _DP, _DP] + [C.c_double] * 9),
+            "ora_evalcv": (None, [P, _DP, _DP, _DP, C.c_double, C.c_double]),
+            "ora_chemeq2": (C.c_int, [P, _DP, C.c_double, C.c_double, C.c_double]),
+        }
+        for n, (r, a) in sig.items():
+            f = getattr(L, n)
+            f.restype, f.argtypes = r, a
+        _lib = L
+    return _lib
+
+
+RU = 6.02214076e26 * 1.380649e-23 * 1.0E-3
+
+
+def config_from_setup(setup, use_libm=0, **over):
+    """OraConfig carrying the same geometry and tables as setup.context()."""
+    d = setup.desc
+    c = OraConfig()
+    for name, _ in OraConfig._fields_:
+        if hasattr(d, name) and name not in ("BC",):
+            try:
+                setattr(c, name, getattr(d, name))
+            except TypeError:
+                pass
+    for q in range(6):
+        c.BC[q] = d.BC[q]
+    c.use_libm = use_libm
+    if d.cop:
+        Hia, Hib, Wi = setup.thermo_tables()
+        NS = d.NS
+        for q in range(NS * 21):
+            c.Hia[q] = Hia[q]
+        for q in range(NS * 6):
+            c.Hib[q] = Hib[q]
+        for n in range(NS):
+            c.Wi[n] = Wi[n]
+            c._Wi[n] = 1.0 / Wi[n]
+            c.Ri[n] = RU / Wi[n]
+        if d.NR > 0:
+            Nu_f, Nu_b, third, Rargus = setup.mechanism()
+            for r in range(d.NR):
+                s = 0.0
+                for j in range(NS):
+                    q = r * NS + j
+                    c.Nu_f[q] = int(Nu_f[q])
+                    c.Nu_b[q] = int(Nu_b[q])
+                    c.Nu_d[q] = int(Nu_b[q]) - int(Nu_f[q])
+                    c.React_ThirdCoef[q] = third[q]
+                    s += third[q]
+                c.third_ind[r] = 1 if s > 0.0 else 0
+                for q in range(6):
+                    c.Rargus[r * 6 + q] = Rargus[r * 6 + q]
+    for k, v in over.items():
+        setattr(c, k, v)
+    return c
+
+
+class Oracle:
+    """One case on the CPU oracle (AoS fields, reference layout)."""
+
+    def __init__(self, cfg):
+        self.cfg = cfg
+        self._h = lib().ora_create(C.byref(cfg))
+        self.E = lib().ora_num_eqn(self._h)
+        self.N = lib().ora_num_cells(self._h)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().ora_destroy(self._h)
+            self._h = None
+
+    def field(self, name, n=None):
+        sizes = {"U": self.E, "U1": self.E, "LU": self.E, "FluxF": self.E, "FluxG": self.E, "FluxH": self.E,
+                 "FluxFw": self.E, "FluxGw": self.E, "FluxHw": self.E, "eigen_local_x": self.E,
+                 "eigen_local_y": self.E, "eigen_local_z": self.E, "y": self.cfg.NS}
+        if name == "eigen_block":
+            cnt = 3 * MAXE
+        elif name == "uvw_c_max":
+            cnt = 6
+        else:
+            cnt = self.N * sizes.get(name, 1)
+        p = lib().ora_field(self._h, name.encode())
+        return np.ctypeslib.as_array(p, shape=(cnt,))
+
+    def set_state(self, U, T):
+        """Initial state; U1 = U as the reference's initial-condition kernels
+        set it (e.g. src/solver_Ini/sample/2D-EulerVortex/ini_sample.hpp:84-88)."""
+        self.field("U")[:] = U
+        self.field("U1")[:] = U
+        self.field("T")[:] = T
+
+    def initialize(self):
+        return lib().ora_initialize(self._h)
+
+    def steps(self, n):
+        dts = np.zeros(max(n, 1))
+        st = lib().ora_steps(self._h, n, dts.ctypes.data_as(_DP))
+        return st, dts[:n]
+
+    def boundary(self, which):
+        lib().ora_boundary(self._h, which)
+
+    def update_states(self, which):
+        return lib().ora_update_states(self._h, which)
+
+    def compute_lu(self, which):
+        lib().ora_compute_lu(self._h, which)
+
+    def rk3(self, stage, dt):
+        lib().ora_rk3(self._h, stage, dt)
+
+    def get_dt(self):
+        return lib().ora_get_dt(self._h)
+
+    def chemistry(self, dt):
+        lib().ora_chemistry(self._h, dt)
+
+    def reaction(self, dt):
+        return lib().ora_reaction(self._h, dt)

@@ -1,0 +1,722 @@
+// Host side of the C-ABI (include/xf_api.h): context, device buffers,
+// kernel dispatch over compile-time species counts, the stage seam and the
+// fused device-resident time-step loop (optionally one hipGraph per step).
+#include "xf_sweep_kernels.h"
+#include "xf_chem_kernels.h"
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+using namespace xf;
+
+namespace {
+
+struct EvPair {
+    const char *name;
+    hipEvent_t a, b;
+};
+
+struct Ops {
+    void (*states)(xf_ctx *, const double *U, int full);
+    void (*sweep)(xf_ctx *, int dir, const double *Ug, int mode, int stage, const double *Ua, const double *Ub,
+                  double *Uo);
+    void (*chem)(xf_ctx *, int record);
+};
+
+} // namespace
+
+struct xf_ctx {
+    xf_case_desc d;
+    Geom g;
+    int E, NCOP;
+    long long N, Ninner;
+    hipStream_t stream = nullptr;
+    Fields f{};
+    double *U2 = nullptr; // RK stage-2 output of the fused path
+    DevThermo h_th{};
+    DevThermo *d_th = nullptr;
+    DevMech h_mc{};
+    DevMech *d_mc = nullptr;
+    bool thermo_set = false, mech_set = false;
+    Ops ops{};
+    bool has_ops = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int last_code = 0, last_id = -1;
+    bool prof = false;
+    std::vector<EvPair> evs;
+    std::map<std::string, std::pair<double, int>> times;
+};
+
+#define XF_HIP(call)                                                                                                   \
+    do {                                                                                                               \
+        hipError_t e_ = (call);                                                                                        \
+        if (e_ != hipSuccess) {                                                                                        \
+            fprintf(stderr, "xfluids_amd: HIP error %s at %s:%d\n", hipGetErrorString(e_), __FILE__, __LINE__);       \
+            return XF_EHIP;                                                                                            \
+        }                                                                                                              \
+    } while (0)
+
+namespace {
+
+inline unsigned nblk(long long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+struct ProfScope {
+    xf_ctx *c;
+    const char *name;
+    hipEvent_t a{}, b{};
+    ProfScope(xf_ctx *c_, const char *n) : c(c_), name(n)
+    {
+        if (c->prof) {
+            (void)hipEventCreate(&a);
+            (void)hipEventCreate(&b);
+            (void)hipEventRecord(a, c->stream);
+        }
+    }
+    ~ProfScope()
+    {
+        if (c->prof) {
+            (void)hipEventRecord(b, c->stream);
+            c->evs.push_back({name, a, b});
+        }
+    }
+};
+
+template <int NS, bool COP> struct Launch {
+    static void states(xf_ctx *c, const double *U, int full)
+    {
+        ProfScope ps(c, "states");
+        hipLaunchKernelGGL((k_states<NS, COP>), dim3(nblk(c->N)), dim3(256), 0, c->stream, c->g, c->d_th, U, c->f,
+                           c->d.ncop_gamma, full);
+    }
+    template <int DIR, int SCH>
+    static void sweep_t(xf_ctx *c, const double *Ug, int mode, int stage, const double *Ua, const double *Ub, double *Uo)
+    {
+        constexpr int E = COP ? NS + 4 : 5;
+        using SS = SweepShape<DIR, E>;
+        const Geom &g = c->g;
+        dim3 grid;
+        if (DIR == 0)
+            grid = dim3((unsigned)((g.X_inner + SS::TS - 2) / (SS::TS - 1)), (unsigned)((g.Y_inner * g.Z_inner + SS::TW - 1) / SS::TW), 1);
+        else if (DIR == 1)
+            grid = dim3((unsigned)((g.X_inner + SS::TW - 1) / SS::TW), (unsigned)((g.Y_inner + SS::TS - 2) / (SS::TS - 1)), (unsigned)g.Z_inner);
+        else
+            grid = dim3((unsigned)((g.X_inner + SS::TW - 1) / SS::TW), (unsigned)((g.Z_inner + SS::TS - 2) / (SS::TS - 1)), (unsigned)g.Y_inner);
+        static const char *names[3] = {"sweep_x", "sweep_y", "sweep_z"};
+        ProfScope ps(c, names[DIR]);
+        hipLaunchKernelGGL((k_sweep<NS, COP, DIR, SCH>), grid, dim3(256), 0, c->stream, c->g, c->d_th, Ug, c->f,
+                           c->d.ncop_gamma, mode, stage, Ua, Ub, Uo);
+    }
+    static void sweep(xf_ctx *c, int dir, const double *Ug, int mode, int stage, const double *Ua, const double *Ub,
+                      double *Uo)
+    {
+        const bool z = c->d.scheme == XF_WENO5Z;
+        if (dir == 0) z ? sweep_t<0, XF_WENO5Z>(c, Ug, mode, stage, Ua, Ub, Uo) : sweep_t<0, XF_WENO5JS>(c, Ug, mode, stage, Ua, Ub, Uo);
+        if (dir == 1) z ? sweep_t<1, XF_WENO5Z>(c, Ug, mode, stage, Ua, Ub, Uo) : sweep_t<1, XF_WENO5JS>(c, Ug, mode, stage, Ua, Ub, Uo);
+        if (dir == 2) z ? sweep_t<2, XF_WENO5Z>(c, Ug, mode, stage, Ua, Ub, Uo) : sweep_t<2, XF_WENO5JS>(c, Ug, mode, stage, Ua, Ub, Uo);
+    }
+    static void chem(xf_ctx *c, int record)
+    {
+        if constexpr (COP) {
+            ProfScope ps(c, "chem");
+            hipLaunchKernelGGL((k_chem<NS>), dim3(nblk(c->Ninner)), dim3(256), 0, c->stream, c->g, c->d_th, c->d_mc,
+                               c->f, record);
+        }
+    }
+    static Ops ops() { return Ops{&states, &sweep, &chem}; }
+};
+
+bool select_ops(int NS, int cop, Ops &o)
+{
+    if (!cop) {
+        if (NS != 1) return false;
+        o = Launch<1, false>::ops();
+        return true;
+    }
+    switch (NS) {
+    case 2: o = Launch<2, true>::ops(); return true;
+    case 3: o = Launch<3, true>::ops(); return true;
+    case 9: o = Launch<9, true>::ops(); return true;
+    default: return false;
+    }
+}
+
+__global__ void k_zero_int(int *p, int n)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = (t == 1) ? 0x7fffffff : 0;
+}
+
+__global__ void k_debug_math(const double *x, double *ex, double *lg, int n)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) {
+        ex[t] = dexp(x[t]);
+        lg[t] = dlog(x[t]);
+    }
+}
+
+__global__ void k_set_double(double *p, double v)
+{
+    if (threadIdx.x == 0) p[0] = v;
+}
+
+int check_launch()
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        fprintf(stderr, "xfluids_amd: launch error %s\n", hipGetErrorString(e));
+        return XF_EHIP;
+    }
+    return 0;
+}
+
+// read and clear the device error word; returns an XF_ERR code or 0
+int take_error(xf_ctx *c)
+{
+    int h[2] = {0, 0x7fffffff};
+    if (hipMemcpyAsync(h, c->f.err, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream) != hipSuccess) return XF_EHIP;
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return XF_EHIP;
+    if (h[0] == 0) return 0;
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, c->f.err, 2);
+    int code = (h[0] & EBIT_RHO_Y) ? XF_ERR_RHO_Y
+               : (h[0] & EBIT_PRIM) ? XF_ERR_PRIM
+               : (h[0] & EBIT_CHEM) ? XF_ERR_CHEM
+                                    : XF_ERR_NAN;
+    c->last_code = code;
+    c->last_id = h[1];
+    return code;
+}
+
+void boundary_async(xf_ctx *c, double *U)
+{
+    ProfScope ps(c, "boundary");
+    const Geom &g = c->g;
+    if (g.DimX)
+        hipLaunchKernelGGL(k_boundary<0>, dim3(nblk((long long)g.Bx * g.Ymax * g.Zmax)), dim3(256), 0, c->stream, g, U,
+                           c->E, c->NCOP, c->d.BC[0], c->d.BC[1]);
+    if (g.DimY)
+        hipLaunchKernelGGL(k_boundary<1>, dim3(nblk((long long)g.By * g.Xmax * g.Zmax)), dim3(256), 0, c->stream, g, U,
+                           c->E, c->NCOP, c->d.BC[2], c->d.BC[3]);
+    if (g.DimZ)
+        hipLaunchKernelGGL(k_boundary<2>, dim3(nblk((long long)g.Bz * g.Xmax * g.Ymax)), dim3(256), 0, c->stream, g, U,
+                           c->E, c->NCOP, c->d.BC[4], c->d.BC[5]);
+}
+
+void states_async(xf_ctx *c, const double *U, int full)
+{
+    hipLaunchKernelGGL(k_reset_pass_red, dim3(1), dim3(64), 0, c->stream, c->f.red);
+    c->ops.states(c, U, full);
+}
+
+// GetLU: fold GLF, sweeps in x, y, z order.  rk_stage > 0 fuses the SSP-RK3
+// stage into the last sweep (Ua, Ub inputs, Uo output).
+void lu_async(xf_ctx *c, const double *Ug, int rk_stage, const double *Ua, const double *Ub, double *Uo)
+{
+    hipLaunchKernelGGL(k_fold_glf, dim3(1), dim3(64), 0, c->stream, c->f.red);
+    int dirs[3], nd = 0;
+    if (c->g.DimX) dirs[nd++] = 0;
+    if (c->g.DimY) dirs[nd++] = 1;
+    if (c->g.DimZ) dirs[nd++] = 2;
+    for (int q = 0; q < nd; q++) {
+        int mode = (q > 0) ? 1 : 0;
+        if (rk_stage > 0 && q == nd - 1) mode |= 2;
+        c->ops.sweep(c, dirs[q], Ug, mode, rk_stage, Ua, Ub, Uo);
+    }
+}
+
+void reaction_async(xf_ctx *c, int record)
+{
+    states_async(c, c->f.U, 0);
+    hipLaunchKernelGGL(k_dt_chem, dim3(1), dim3(64), 0, c->stream, c->f.dt, c->d.strang);
+    c->ops.chem(c, record);
+}
+
+void step_async(xf_ctx *c)
+{
+    Fields &f = c->f;
+    hipLaunchKernelGGL(k_dt, dim3(1), dim3(64), 0, c->stream, c->g, f.red, f.dt);
+    if (c->d.react && c->d.strang) reaction_async(c, 0);
+    // stage 1: U -> U1 ; stage 2: (U, U1) -> U2 ; stage 3: (U, U2) -> U
+    double *in[3] = {f.U, f.U1, c->U2};
+    for (int s = 1; s <= 3; s++) {
+        double *Ug = in[s - 1];
+        boundary_async(c, Ug);
+        states_async(c, Ug, 0);
+        double *Uo = (s == 1) ? f.U1 : (s == 2 ? c->U2 : f.U);
+        lu_async(c, Ug, s, f.U, (s == 1) ? f.U1 : Ug, Uo);
+    }
+    if (c->d.react) reaction_async(c, 0);
+}
+
+} // namespace
+
+extern "C" {
+
+const char *xf_error_string(int s)
+{
+    switch (s) {
+    case XF_OK: return "ok";
+    case XF_ERR_RHO_Y: return "negative/NaN density or NaN mass fraction";
+    case XF_ERR_PRIM: return "negative/NaN density, pressure or temperature";
+    case XF_ERR_NAN: return "NaN/Inf in conserved variables or flux divergence";
+    case XF_ERR_CHEM: return "chemistry substep bound reached";
+    case XF_EINVAL: return "invalid argument";
+    case XF_EHIP: return "HIP runtime error";
+    case XF_ENOKERNEL: return "no kernel instance for this configuration";
+    default: return "unknown";
+    }
+}
+
+int xf_create(const xf_case_desc *desc, xf_ctx **out)
+{
+    if (!desc || !out) return XF_EINVAL;
+    *out = nullptr;
+    const xf_case_desc &d = *desc;
+    if (d.NS < 1 || d.NS > XF_MAXNS || d.NR < 0 || d.NR > XF_MAXNR) return XF_EINVAL;
+    if (d.Xmax <= 0 || d.Ymax <= 0 || d.Zmax <= 0) return XF_EINVAL;
+    if ((d.DimX && d.Bwidth_X < 3) || (d.DimY && d.Bwidth_Y < 3) || (d.DimZ && d.Bwidth_Z < 3)) return XF_EINVAL;
+    if (d.Xmax != d.X_inner + 2 * d.Bwidth_X || d.Ymax != d.Y_inner + 2 * d.Bwidth_Y ||
+        d.Zmax != d.Z_inner + 2 * d.Bwidth_Z)
+        return XF_EINVAL;
+    if (d.precision != XF_PLAIN_FP64 && d.precision != XF_MIXED) return XF_EINVAL;
+    Ops ops;
+    if (!select_ops(d.NS, d.cop, ops)) return XF_ENOKERNEL;
+    xf_ctx *c = new xf_ctx();
+    c->d = d;
+    c->ops = ops;
+    c->has_ops = true;
+    c->E = d.cop ? d.NS + 4 : 5;
+    c->NCOP = d.cop ? d.NS - 1 : 0;
+    c->N = (long long)d.Xmax * d.Ymax * d.Zmax;
+    c->Ninner = (long long)d.X_inner * d.Y_inner * d.Z_inner;
+    Geom &g = c->g;
+    g.Xmax = d.Xmax, g.Ymax = d.Ymax, g.Zmax = d.Zmax;
+    g.X_inner = d.X_inner, g.Y_inner = d.Y_inner, g.Z_inner = d.Z_inner;
+    g.Bx = d.Bwidth_X, g.By = d.Bwidth_Y, g.Bz = d.Bwidth_Z;
+    g.DimX = d.DimX, g.DimY = d.DimY, g.DimZ = d.DimZ;
+    g._dx = d._dx, g._dy = d._dy, g._dz = d._dz, g.CFL = d.CFLnumber;
+    g.N = c->N;
+    auto fail = [&](int code) {
+        xf_destroy(c);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(XF_EHIP);
+    const size_t nE = (size_t)c->E * c->N * sizeof(double), n1 = (size_t)c->N * sizeof(double);
+    Fields &f = c->f;
+    double **dbl[] = {&f.rho, &f.u, &f.v, &f.w, &f.p, &f.H, &f.T, &f.c, &f.gamma, &f.e, &f.Cp, &f.gcv, &f.R};
+    if (hipMalloc(&f.U, nE) || hipMalloc(&f.U1, nE) || hipMalloc(&f.LU, nE) || hipMalloc(&c->U2, nE)) return fail(XF_EHIP);
+    for (auto pp : dbl)
+        if (hipMalloc(pp, n1)) return fail(XF_EHIP);
+    if (hipMalloc(&f.hi, (size_t)d.NS * n1) || hipMalloc(&f.red, RED_SIZE * sizeof(double)) ||
+        hipMalloc(&f.dt, 4 * sizeof(double)) || hipMalloc(&f.err, 2 * sizeof(int)) ||
+        hipMalloc(&f.substeps, (size_t)c->Ninner * sizeof(int)) || hipMalloc(&f.queue, 4 * sizeof(unsigned)) ||
+        hipMalloc(&c->d_th, sizeof(DevThermo)) || hipMalloc(&c->d_mc, sizeof(DevMech)))
+        return fail(XF_EHIP);
+    // zero everything once (eigen_block starts at 0, src/Fluids.cpp:427)
+    std::vector<char> zeros(std::max(nE, (size_t)d.NS * n1), 0);
+    if (hipMemcpy(f.U, zeros.data(), nE, hipMemcpyHostToDevice) || hipMemcpy(f.U1, zeros.data(), nE, hipMemcpyHostToDevice) ||
+        hipMemcpy(f.LU, zeros.data(), nE, hipMemcpyHostToDevice) || hipMemcpy(c->U2, zeros.data(), nE, hipMemcpyHostToDevice))
+        return fail(XF_EHIP);
+    for (auto pp : dbl)
+        if (hipMemcpy(*pp, zeros.data(), n1, hipMemcpyHostToDevice)) return fail(XF_EHIP);
+    if (hipMemcpy(f.hi, zeros.data(), (size_t)d.NS * n1, hipMemcpyHostToDevice) ||
+        hipMemcpy(f.red, zeros.data(), RED_SIZE * sizeof(double), hipMemcpyHostToDevice) ||
+        hipMemcpy(f.dt, zeros.data(), 4 * sizeof(double), hipMemcpyHostToDevice))
+        return fail(XF_EHIP);
+    int e0[2] = {0, 0x7fffffff};
+    if (hipMemcpy(f.err, e0, sizeof(e0), hipMemcpyHostToDevice)) return fail(XF_EHIP);
+    c->h_th.thermo = d.thermo;
+    c->h_mc.NR = d.NR;
+    if (hipMemcpy(c->d_th, &c->h_th, sizeof(DevThermo), hipMemcpyHostToDevice) ||
+        hipMemcpy(c->d_mc, &c->h_mc, sizeof(DevMech), hipMemcpyHostToDevice))
+        return fail(XF_EHIP);
+    *out = c;
+    return XF_OK;
+}
+
+void xf_destroy(xf_ctx *c)
+{
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    for (auto &e : c->evs) (void)hipEventDestroy(e.a), (void)hipEventDestroy(e.b);
+    Fields &f = c->f;
+    void *ptrs[] = {f.U, f.U1, f.LU, c->U2, f.rho, f.u, f.v, f.w, f.p, f.H, f.T, f.c, f.gamma, f.e, f.Cp, f.gcv, f.R,
+                    f.hi, f.red, f.dt, f.err, f.substeps, f.queue, c->d_th, c->d_mc};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int xf_num_eqn(const xf_ctx *c) { return c ? c->E : 0; }
+int xf_get_desc(const xf_ctx *c, xf_case_desc *out)
+{
+    if (!c || !out) return XF_EINVAL;
+    *out = c->d;
+    return XF_OK;
+}
+size_t xf_num_cells(const xf_ctx *c) { return c ? (size_t)c->N : 0; }
+void *xf_stream(xf_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+int xf_set_thermo(xf_ctx *c, const double *Hia, const double *Hib, const double *Wi)
+{
+    if (!c || !Hia || !Hib || !Wi) return XF_EINVAL;
+    const int NS = c->d.NS;
+    std::memcpy(c->h_th.Hia, Hia, sizeof(double) * NS * 21);
+    std::memcpy(c->h_th.Hib, Hib, sizeof(double) * NS * 6);
+    for (int n = 0; n < NS; n++) {
+        // src/read_ini/src/thermal.cpp:158-161
+        c->h_th.Wi[n] = Wi[n];
+        c->h_th._Wi[n] = 1.0 / Wi[n];
+        c->h_th.Ri[n] = Ru / Wi[n];
+    }
+    c->h_th.thermo = c->d.thermo;
+    XF_HIP(hipMemcpy(c->d_th, &c->h_th, sizeof(DevThermo), hipMemcpyHostToDevice));
+    c->thermo_set = true;
+    return XF_OK;
+}
+
+int xf_set_mechanism(xf_ctx *c, const int *Nu_f, const int *Nu_b, const double *third, const double *Rargus)
+{
+    if (!c) return XF_EINVAL;
+    const int NS = c->d.NS, NR = c->d.NR;
+    if (NR > 0 && (!Nu_f || !Nu_b || !third || !Rargus)) return XF_EINVAL;
+    DevMech &m = c->h_mc;
+    m.NR = NR;
+    for (int r = 0; r < NR; r++) {
+        double sum = 0.0;
+        for (int j = 0; j < NS; j++) {
+            m.Nu_f[r * NS + j] = Nu_f[r * NS + j];
+            m.Nu_b[r * NS + j] = Nu_b[r * NS + j];
+            m.Nu_d[r * NS + j] = Nu_b[r * NS + j] - Nu_f[r * NS + j]; // reaction.cpp:38
+            m.third[r * NS + j] = third[r * NS + j];
+            sum += third[r * NS + j];
+        }
+        m.third_ind[r] = (sum > 0.0) ? 1 : 0; // reaction.cpp:170-172
+        for (int q = 0; q < 6; q++) m.Rargus[r * 6 + q] = Rargus[r * 6 + q];
+    }
+    XF_HIP(hipMemcpy(c->d_mc, &c->h_mc, sizeof(DevMech), hipMemcpyHostToDevice));
+    c->mech_set = true;
+    return XF_OK;
+}
+
+static double *which_ptr(xf_ctx *c, int which)
+{
+    return which == 0 ? c->f.U : (which == 1 ? c->f.U1 : c->U2);
+}
+
+int xf_upload_state(xf_ctx *c, int which, const double *U_aos, const double *T)
+{
+    if (!c || which < 0 || which > 2) return XF_EINVAL;
+    const long long N = c->N;
+    const int E = c->E;
+    if (U_aos) {
+        std::vector<double> soa((size_t)E * N);
+        for (long long id = 0; id < N; id++)
+            for (int n = 0; n < E; n++) soa[(size_t)n * N + id] = U_aos[(size_t)E * id + n];
+        XF_HIP(hipMemcpy(which_ptr(c, which), soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice));
+        if (which == 0) {
+            // initial state: U1 = U as the reference's initial-condition kernels
+            // set it (e.g. 2D-EulerVortex/ini_sample.hpp:84-88); ghost cells of
+            // an Inflow boundary keep these values for the whole run
+            XF_HIP(hipMemcpy(c->f.U1, soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice));
+            XF_HIP(hipMemcpy(c->U2, soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice));
+        }
+    }
+    if (T) XF_HIP(hipMemcpy(c->f.T, T, (size_t)N * sizeof(double), hipMemcpyHostToDevice));
+    return XF_OK;
+}
+
+int xf_download_state(xf_ctx *c, int which, double *U_aos, double *T)
+{
+    if (!c || which < 0 || which > 2) return XF_EINVAL;
+    XF_HIP(hipStreamSynchronize(c->stream));
+    const long long N = c->N;
+    const int E = c->E;
+    if (U_aos) {
+        std::vector<double> soa((size_t)E * N);
+        XF_HIP(hipMemcpy(soa.data(), which_ptr(c, which), soa.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (long long id = 0; id < N; id++)
+            for (int n = 0; n < E; n++) U_aos[(size_t)E * id + n] = soa[(size_t)n * N + id];
+    }
+    if (T) XF_HIP(hipMemcpy(T, c->f.T, (size_t)N * sizeof(double), hipMemcpyDeviceToHost));
+    return XF_OK;
+}
+
+int xf_download_prims(xf_ctx *c, double *rho, double *p, double *u, double *v, double *w, double *H, double *cc,
+                      double *gamma, double *e, double *y_aos)
+{
+    if (!c) return XF_EINVAL;
+    XF_HIP(hipStreamSynchronize(c->stream));
+    const size_t n1 = (size_t)c->N * sizeof(double);
+    Fields &f = c->f;
+    double *dst[] = {rho, p, u, v, w, H, cc, gamma, e};
+    double *src[] = {f.rho, f.p, f.u, f.v, f.w, f.H, f.c, f.gamma, f.e};
+    for (int q = 0; q < 9; q++)
+        if (dst[q]) XF_HIP(hipMemcpy(dst[q], src[q], n1, hipMemcpyDeviceToHost));
+    if (y_aos) {
+        // Y of every padded cell from U, exactly as Getrhoyi (Update_device.hpp:7-29)
+        const int E = c->E, NS = c->d.NS;
+        const long long N = c->N;
+        std::vector<double> soa((size_t)E * N);
+        XF_HIP(hipMemcpy(soa.data(), f.U, soa.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (long long id = 0; id < N; id++) {
+            if (!c->d.cop) {
+                y_aos[id] = 1.0;
+                continue;
+            }
+            const double rho1 = 1.0 / soa[id];
+            double *y = y_aos + (size_t)NS * id;
+            y[NS - 1] = 1.0;
+            for (int ii = 5; ii < E; ii++) y[ii - 5] = soa[(size_t)ii * N + id] * rho1, y[NS - 1] += -y[ii - 5];
+        }
+    }
+    return XF_OK;
+}
+
+int xf_download_lu(xf_ctx *c, double *LU_aos)
+{
+    if (!c || !LU_aos) return XF_EINVAL;
+    XF_HIP(hipStreamSynchronize(c->stream));
+    const long long N = c->N;
+    const int E = c->E;
+    std::vector<double> soa((size_t)E * N);
+    XF_HIP(hipMemcpy(soa.data(), c->f.LU, soa.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (long long id = 0; id < N; id++)
+        for (int n = 0; n < E; n++) LU_aos[(size_t)E * id + n] = soa[(size_t)n * N + id];
+    return XF_OK;
+}
+
+int xf_download_maxima(xf_ctx *c, double *eigen_block, double *uvw_c_max)
+{
+    if (!c) return XF_EINVAL;
+    double red[RED_SIZE];
+    XF_HIP(hipStreamSynchronize(c->stream));
+    XF_HIP(hipMemcpy(red, c->f.red, sizeof(red), hipMemcpyDeviceToHost));
+    const int E = c->E;
+    if (eigen_block)
+        for (int d = 0; d < 3; d++)
+            for (int n = 0; n < XF_MAXNS + 4; n++)
+                eigen_block[d * (XF_MAXNS + 4) + n] =
+                    n < E ? red[RED_GLF + 3 * d + (n == 0 ? 0 : (n == E - 1 ? 2 : 1))] : 0.0;
+    if (uvw_c_max)
+        for (int d = 0; d < 3; d++) uvw_c_max[d] = red[RED_DT + d];
+    return XF_OK;
+}
+
+int xf_boundary(xf_ctx *c, int which)
+{
+    if (!c || which < 0 || which > 2) return XF_EINVAL;
+    boundary_async(c, which_ptr(c, which));
+    if (int e = check_launch()) return e;
+    XF_HIP(hipStreamSynchronize(c->stream));
+    return XF_OK;
+}
+
+int xf_update_states(xf_ctx *c, int which)
+{
+    if (!c || which < 0 || which > 2) return XF_EINVAL;
+    if (c->d.cop && !c->thermo_set) return XF_EINVAL;
+    states_async(c, which_ptr(c, which), 1);
+    if (int e = check_launch()) return e;
+    return take_error(c);
+}
+
+int xf_compute_lu(xf_ctx *c, int which)
+{
+    if (!c || which < 0 || which > 2) return XF_EINVAL;
+    lu_async(c, which_ptr(c, which), 0, nullptr, nullptr, nullptr);
+    if (int e = check_launch()) return e;
+    return take_error(c);
+}
+
+int xf_rk3_stage(xf_ctx *c, int stage, double dt)
+{
+    if (!c || stage < 1 || stage > 3) return XF_EINVAL;
+    hipLaunchKernelGGL(k_set_double, dim3(1), dim3(64), 0, c->stream, c->f.dt + 2, dt);
+    {
+        ProfScope ps(c, "rk3");
+        hipLaunchKernelGGL(k_rk3, dim3(nblk(c->Ninner)), dim3(256), 0, c->stream, c->g, c->E, c->f.U, c->f.U1, c->f.LU,
+                           c->f.dt + 2, stage);
+    }
+    if (int e = check_launch()) return e;
+    XF_HIP(hipStreamSynchronize(c->stream));
+    return XF_OK;
+}
+
+int xf_get_dt(xf_ctx *c, double *dt_out)
+{
+    if (!c || !dt_out) return XF_EINVAL;
+    hipLaunchKernelGGL(k_dt, dim3(1), dim3(64), 0, c->stream, c->g, c->f.red, c->f.dt);
+    if (int e = check_launch()) return e;
+    XF_HIP(hipMemcpyAsync(dt_out, c->f.dt, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    XF_HIP(hipStreamSynchronize(c->stream));
+    return XF_OK;
+}
+
+int xf_chemistry(xf_ctx *c, double dt_chem, int *substeps)
+{
+    if (!c || !c->d.cop || c->d.NR <= 0) return XF_EINVAL;
+    if (!c->thermo_set || !c->mech_set) return XF_EINVAL;
+    hipLaunchKernelGGL(k_set_double, dim3(1), dim3(64), 0, c->stream, c->f.dt + 1, dt_chem);
+    c->ops.chem(c, substeps ? 1 : 0);
+    if (int e = check_launch()) return e;
+    if (substeps)
+        XF_HIP(hipMemcpyAsync(substeps, c->f.substeps, (size_t)c->Ninner * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    return take_error(c);
+}
+
+int xf_estimate_nan(xf_ctx *c, int flag)
+{
+    if (!c || flag < 1 || flag > 4) return XF_EINVAL;
+    const double *UI = (flag <= 2) ? c->f.U1 : c->f.U;
+    hipLaunchKernelGGL(k_estimate_nan, dim3(nblk(c->Ninner)), dim3(256), 0, c->stream, c->g, c->E, UI, c->f.LU,
+                       c->f.err);
+    if (int e = check_launch()) return e;
+    return take_error(c);
+}
+
+int xf_reaction(xf_ctx *c, double dt)
+{
+    if (!c) return XF_EINVAL;
+    int e = xf_update_states(c, 0);
+    if (e) return e;
+    double tem_dt = dt;
+    if (c->d.strang) tem_dt *= 0.5;
+    e = xf_chemistry(c, tem_dt, nullptr);
+    if (e) return e;
+    return xf_estimate_nan(c, 4);
+}
+
+int xf_initialize(xf_ctx *c)
+{
+    if (!c) return XF_EINVAL;
+    int e = xf_boundary(c, 0);
+    if (e) return e;
+    return xf_update_states(c, 0);
+}
+
+static int steps_enqueue(xf_ctx *c, int nsteps, int use_graph)
+{
+    if (c->d.cop && !c->thermo_set) return XF_EINVAL;
+    if (c->d.react && (!c->mech_set || c->d.NR <= 0)) return XF_EINVAL;
+    if (use_graph) {
+        if (!c->gexec) {
+            const bool prof = c->prof;
+            c->prof = false;
+            XF_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            step_async(c);
+            XF_HIP(hipStreamEndCapture(c->stream, &c->graph));
+            c->prof = prof;
+            XF_HIP(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+        }
+        for (int s = 0; s < nsteps; s++) XF_HIP(hipGraphLaunch(c->gexec, c->stream));
+    } else {
+        for (int s = 0; s < nsteps; s++) step_async(c);
+    }
+    return check_launch();
+}
+
+int xf_steps(xf_ctx *c, int nsteps, int use_graph, double *dt_out)
+{
+    if (!c || nsteps < 0) return XF_EINVAL;
+    for (int s = 0; s < nsteps; s++) {
+        if (dt_out) {
+            int e = steps_enqueue(c, 1, use_graph);
+            if (e) return e;
+            XF_HIP(hipMemcpyAsync(dt_out + s, c->f.dt, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            e = take_error(c);
+            if (e) return e;
+        }
+    }
+    if (!dt_out) {
+        int e = steps_enqueue(c, nsteps, use_graph);
+        if (e) return e;
+    }
+    return take_error(c);
+}
+
+int xf_steps_async(xf_ctx *c, int nsteps, int use_graph)
+{
+    if (!c || nsteps < 0) return XF_EINVAL;
+    return steps_enqueue(c, nsteps, use_graph);
+}
+
+int xf_synchronize(xf_ctx *c)
+{
+    if (!c) return XF_EINVAL;
+    XF_HIP(hipStreamSynchronize(c->stream));
+    return take_error(c);
+}
+
+int xf_last_error(xf_ctx *c, int *code, int *i, int *j, int *k)
+{
+    if (!c) return XF_EINVAL;
+    if (code) *code = c->last_code;
+    const int id = c->last_id;
+    const int X = c->g.Xmax, Y = c->g.Ymax;
+    if (i) *i = id >= 0 ? id % X : -1;
+    if (j) *j = id >= 0 ? (id / X) % Y : -1;
+    if (k) *k = id >= 0 ? id / (X * Y) : -1;
+    return XF_OK;
+}
+
+int xf_set_profiling(xf_ctx *c, int on)
+{
+    if (!c) return XF_EINVAL;
+    XF_HIP(hipStreamSynchronize(c->stream));
+    for (auto &e : c->evs) (void)hipEventDestroy(e.a), (void)hipEventDestroy(e.b);
+    c->evs.clear();
+    c->times.clear();
+    c->prof = on != 0;
+    return XF_OK;
+}
+
+int xf_kernel_times(xf_ctx *c, const char **names, double *avg_ms, int *counts, int cap)
+{
+    if (!c) return XF_EINVAL;
+    XF_HIP(hipStreamSynchronize(c->stream));
+    for (auto &e : c->evs) {
+        float ms = 0.f;
+        XF_HIP(hipEventElapsedTime(&ms, e.a, e.b));
+        auto &t = c->times[e.name];
+        t.first += ms;
+        t.second += 1;
+        (void)hipEventDestroy(e.a), (void)hipEventDestroy(e.b);
+    }
+    c->evs.clear();
+    int n = 0;
+    for (auto &kv : c->times) {
+        if (n < cap) {
+            if (names) names[n] = kv.first.c_str(); // stable: map nodes live with the context
+            if (avg_ms) avg_ms[n] = kv.second.first / std::max(1, kv.second.second);
+            if (counts) counts[n] = kv.second.second;
+        }
+        n++;
+    }
+    return n;
+}
+
+int xf_debug_math(const double *x, double *exp_out, double *log_out, int n)
+{
+    if (!x || !exp_out || !log_out || n <= 0) return XF_EINVAL;
+    double *d = nullptr;
+    XF_HIP(hipMalloc(&d, 3 * (size_t)n * sizeof(double)));
+    hipError_t e = hipMemcpy(d, x, (size_t)n * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_debug_math, dim3(nblk(n)), dim3(256), 0, 0, d, d + n, d + 2 * n, n);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(exp_out, d + n, (size_t)n * sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(log_out, d + 2 * n, (size_t)n * sizeof(double), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    return e == hipSuccess ? XF_OK : XF_EHIP;
+}
+
+} // extern "C"

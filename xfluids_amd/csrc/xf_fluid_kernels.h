@@ -1,0 +1,346 @@
+// Fluid-stage kernels for gfx950: ghost fill, conservative->primitive
+// conversion with fused GLF/CFL reductions, SSP-RK3 update, dt.
+//
+// Reference stages restated (operation order preserved so the plain-FP64
+// form is bit-identical to the CPU restatement):
+//   FluidBCKernel{X,Y,Z}      src/solver_BCs/BCs_kernels.hpp:9-258
+//   Updaterhoyi + GetStates    src/solver_UpdateStates/Update_device.hpp:7-54
+//   Estimate{Yi,PrimitiveVar}  src/solver_UpdateStates/Estimate_kernels.hpp
+//   GetLocalEigen + GLF max    FDM_Method/positive-definite_eigen/Eigen_value.hpp:5-33,
+//                              FDM_Method/ConVenction_block.hpp:115-172
+//   GetDt                      src/solver_GetDt/GlobalDt_block.hpp:6-114
+//   UpdateURK3rdKernel         src/solver_UpdateStates/Update_kernels.hpp:64-94
+#pragma once
+#include "xf_thermo.h"
+
+namespace xf {
+
+// ---------------------------------------------------------------------------
+// wave64 max reduction (butterfly over DPP/permute) of non-negative doubles
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const double o = __shfl_xor(v, off, 64);
+        v = (o > v) ? o : v;
+    }
+    return v;
+}
+
+// atomic max of a non-negative double through its bit pattern (ordered as
+// unsigned 64-bit for +0 .. +inf)
+__device__ __forceinline__ void atomic_max_pos(double *addr, double v)
+{
+    atomicMax(reinterpret_cast<unsigned long long *>(addr), (unsigned long long)__double_as_longlong(v));
+}
+
+// ---------------------------------------------------------------------------
+// ghost fill along one axis: one thread per (ghost layer g, transverse a, b)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int bc_target(int bc, int g, int B, int inner, int mirror_offset, int index_inner, int sign)
+{
+    switch (bc) {
+    case XF_BC_SYMMETRY:
+    case XF_BC_NSLIPWALL:
+    case XF_BC_VISCWALL:
+    case XF_BC_SLIPWALL:
+        return 2 * (B + mirror_offset) - 1 - g;
+    case XF_BC_PERIODIC:
+        return g + sign * inner;
+    case XF_BC_OUTFLOW:
+        return index_inner;
+    default:
+        return g;
+    }
+}
+
+__device__ __forceinline__ void bc_apply(double *__restrict__ U, long long N, int E, int NCOP, long long id, long long tid,
+                                         int bc, int mom)
+{
+    switch (bc) {
+    case XF_BC_SYMMETRY:
+        for (int n = 0; n < E; n++) U[n * N + id] = U[n * N + tid];
+        U[mom * N + id] = -U[mom * N + tid];
+        break;
+    case XF_BC_PERIODIC:
+    case XF_BC_OUTFLOW:
+        for (int n = 0; n < E; n++) U[n * N + id] = U[n * N + tid];
+        break;
+    case XF_BC_NSLIPWALL:
+    case XF_BC_VISCWALL:
+    case XF_BC_SLIPWALL:
+        U[0 * N + id] = U[0 * N + tid];
+        U[1 * N + id] = -U[1 * N + tid];
+        U[2 * N + id] = -U[2 * N + tid];
+        U[3 * N + id] = -U[3 * N + tid];
+        U[4 * N + id] = U[4 * N + tid];
+        for (int n = E - NCOP; n < E; n++) U[n * N + id] = U[n * N + tid];
+        break;
+    default:
+        break;
+    }
+}
+
+// DIR 0: threads over (Bx, Ymax, Zmax); 1: (Xmax, By, Zmax); 2: (Xmax, Ymax, Bz)
+template <int DIR>
+__global__ void __launch_bounds__(256) k_boundary(Geom g, double *__restrict__ U, int E, int NCOP, int bc_lo, int bc_hi)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int B = DIR == 0 ? g.Bx : (DIR == 1 ? g.By : g.Bz);
+    const int inner = DIR == 0 ? g.X_inner : (DIR == 1 ? g.Y_inner : g.Z_inner);
+    const int Lmax = DIR == 0 ? g.Xmax : (DIR == 1 ? g.Ymax : g.Zmax);
+    const int A = DIR == 0 ? g.Ymax : g.Xmax;       // fastest transverse extent
+    const int C = DIR == 2 ? g.Ymax : g.Zmax;       // slowest transverse extent
+    const long long total = (long long)B * A * C;
+    if (t >= total) return;
+    const int a = (int)(t % A);
+    const int gg = (int)((t / A) % B);
+    const int cc = (int)(t / ((long long)A * B));
+    const int lo = gg, hi = gg + Lmax - B;
+    const int tlo = bc_target(bc_lo, lo, B, inner, 0, B, 1);
+    const int thi = bc_target(bc_hi, hi, B, inner, inner, Lmax - B - 1, -1);
+    auto idx = [&](int s) -> long long {
+        if (DIR == 0) return (long long)g.Xmax * g.Ymax * cc + (long long)g.Xmax * a + s;
+        if (DIR == 1) return (long long)g.Xmax * g.Ymax * cc + (long long)g.Xmax * s + a;
+        return (long long)g.Xmax * g.Ymax * s + (long long)g.Xmax * cc + a;
+    };
+    bc_apply(U, g.N, E, NCOP, idx(lo), idx(tlo), bc_lo, DIR + 1);
+    bc_apply(U, g.N, E, NCOP, idx(hi), idx(thi), bc_hi, DIR + 1);
+}
+
+// ---------------------------------------------------------------------------
+// conservative -> primitive over all padded cells
+// ---------------------------------------------------------------------------
+template <int NS, bool COP>
+__global__ void __launch_bounds__(256) k_states(Geom g, const DevThermo *__restrict__ th, const double *__restrict__ Ug,
+                                                Fields f, double ncop_gamma, int full)
+{
+    constexpr int E = COP ? NS + 4 : 5;
+    constexpr int NCOP = COP ? NS - 1 : 0;
+    const long long N = g.N;
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = id < N;
+    double glf[9], dtm[3];
+#pragma unroll
+    for (int q = 0; q < 9; q++) glf[q] = 0.0;
+#pragma unroll
+    for (int q = 0; q < 3; q++) dtm[q] = 0.0;
+    int ebits = 0;
+    if (live) {
+        double U[E];
+#pragma unroll
+        for (int n = 0; n < E; n++) U[n] = Ug[n * N + id];
+        // Getrhoyi
+        const double rho = U[0];
+        const double rho1 = 1.0 / rho;
+        double yi[COP ? NS : 1];
+        if (COP) {
+            yi[NCOP] = 1.0;
+#pragma unroll
+            for (int ii = 5; ii < E; ii++) yi[ii - 5] = U[ii] * rho1, yi[NCOP] += -yi[ii - 5];
+        }
+        // GetStates
+        const double u = U[1] * rho1, v = U[2] * rho1, w = U[3] * rho1;
+        const double tme = U[4] * rho1 - 0.5 * (u * u + v * v + w * w);
+        double T = f.T[id], p, gamma;
+        if (COP) {
+            const double R_ = mix_R<NS>(th, yi);
+            T = get_T<NS>(th, yi, tme, T);
+            p = rho * R_ * T;
+            const TPre t = th->thermo ? tpre<true>(T, 1) : tpre<false>(T, 0);
+            double Cp = 0.0, cpk[NS];
+#pragma unroll
+            for (int n = 0; n < NS; n++) {
+                cpk[n] = cp_k(th, t, th->Ri[n], n);
+                Cp += yi[n] * cpk[n];
+            }
+            gamma = mix_gamma_cp<NS>(th, yi, Cp);
+            // ingredients of the face sound speed (ReconstructSoundSpeed,
+            // positive-definite_eigen/Utils_device.hpp:102-140): h_k(T), Cp,
+            // get_CopGamma(yi, T) = Cp / Cv, R -- identical values to the
+            // per-face recomputation of the reference
+            double cv = 0.0, _W = 0.0;
+#pragma unroll
+            for (int n = 0; n < NS; n++) cv += yi[n] * cpk[n];
+#pragma unroll
+            for (int n = 0; n < NS; n++) _W += yi[n] * th->_Wi[n];
+            cv -= Ru * _W;
+#pragma unroll
+            for (int n = 0; n < NS; n++) f.hi[n * N + id] = h_k(th, t, th->Ri[n], n);
+            f.Cp[id] = Cp;
+            f.gcv[id] = Cp / cv;
+            f.R[id] = R_;
+            f.T[id] = T;
+        } else {
+            gamma = ncop_gamma;
+            p = (ncop_gamma - 1.0) * rho * tme;
+        }
+        const double H = (U[4] + p) * rho1;
+        const double c = sqrt(gamma * p * rho1);
+        f.rho[id] = rho;
+        f.u[id] = u;
+        f.v[id] = v;
+        f.w[id] = w;
+        f.p[id] = p;
+        f.H[id] = H;
+        if (full) {
+            f.c[id] = c;
+            f.gamma[id] = gamma;
+            f.e[id] = tme;
+        }
+        // validation on interior cells
+        const int i = (int)(id % g.Xmax);
+        const int j = (int)((id / g.Xmax) % g.Ymax);
+        const int k = (int)(id / ((long long)g.Xmax * g.Ymax));
+        const bool interior = i >= g.Bx && i < g.Xmax - g.Bx && j >= g.By && j < g.Ymax - g.By && k >= g.Bz &&
+                              k < g.Zmax - g.Bz;
+        if (interior) {
+            bool bad = (rho < 0) || isnan(rho) || isinf(rho);
+            if (COP) {
+#pragma unroll
+                for (int n = 0; n < NS; n++) bad = bad || isnan(yi[n]) || isinf(yi[n]);
+            }
+            if (bad) ebits |= EBIT_RHO_Y;
+            const double Tc = COP ? T : f.T[id];
+            const double vs[3] = {rho, p, Tc};
+#pragma unroll
+            for (int q = 0; q < 3; q++)
+                if (vs[q] < 0 || isnan(vs[q]) || isinf(vs[q])) ebits |= EBIT_PRIM;
+        }
+        // GLF (eigen_local = u_d -+ c, u_d) and CFL (|u_d| + sqrt(1.4 p / rho))
+        const double cl = sqrt(1.4 * p / rho);
+        const double vel[3] = {u, v, w};
+        const int dims[3] = {g.DimX, g.DimY, g.DimZ};
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            if (!dims[d]) continue;
+            const double uu = vel[d];
+            glf[3 * d + 0] = fabs(uu - c);
+            glf[3 * d + 1] = fabs(uu);
+            glf[3 * d + 2] = fabs(uu + c);
+            dtm[d] = fabs(uu) + cl;
+        }
+    }
+    // block reduction: wave butterfly, then one lane per wave folds into LDS
+    __shared__ double sred[4][12];
+    __shared__ int serr;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (threadIdx.x == 0) serr = 0;
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 9; q++) glf[q] = wave_max(glf[q]);
+#pragma unroll
+    for (int q = 0; q < 3; q++) dtm[q] = wave_max(dtm[q]);
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < 9; q++) sred[wv][q] = glf[q];
+#pragma unroll
+        for (int q = 0; q < 3; q++) sred[wv][9 + q] = dtm[q];
+    }
+    if (ebits) {
+        atomicOr(&serr, ebits);
+        atomicMin(&f.err[1], (int)id);
+    }
+    __syncthreads();
+    if (threadIdx.x < 12) {
+        double m = sred[0][threadIdx.x];
+        const int nw = (blockDim.x + 63) >> 6;
+        for (int q = 1; q < nw; q++) m = sred[q][threadIdx.x] > m ? sred[q][threadIdx.x] : m;
+        if (m > 0.0) atomic_max_pos(&f.red[threadIdx.x < 9 ? RED_GLFP + threadIdx.x : RED_DT + threadIdx.x - 9], m);
+    }
+    if (threadIdx.x == 0 && serr) atomicOr(&f.err[0], serr);
+}
+
+// ---------------------------------------------------------------------------
+// small control kernels
+// ---------------------------------------------------------------------------
+// zero the per-pass maxima (pending GLF, CFL) before an UpdateStates pass
+__global__ void k_reset_pass_red(double *red)
+{
+    if (threadIdx.x < 9) red[RED_GLFP + threadIdx.x] = 0.0;
+    if (threadIdx.x < 3) red[RED_DT + threadIdx.x] = 0.0;
+}
+
+// GetLU's GLF reduction: fold the last pass's maxima into the running maxima
+__global__ void k_fold_glf(double *red)
+{
+    if (threadIdx.x < 9) {
+        const double a = red[RED_GLF + threadIdx.x], b = red[RED_GLFP + threadIdx.x];
+        red[RED_GLF + threadIdx.x] = b > a ? b : a;
+    }
+}
+
+// GetDt: dt = CFL / sum_d max_d / Delta_d, written to device memory
+__global__ void k_dt(Geom g, const double *red, double *dt)
+{
+    if (threadIdx.x == 0) {
+        const double mx = g.DimX ? red[RED_DT + 0] : 0.0;
+        const double my = g.DimY ? red[RED_DT + 1] : 0.0;
+        const double mz = g.DimZ ? red[RED_DT + 2] : 0.0;
+        const double dtref = mx * g._dx + my * g._dy + mz * g._dz;
+        dt[0] = g.CFL / dtref;
+    }
+}
+
+// Strang half step on device: dt[1] = dt[0] * 0.5 (or dt[0] for Lie)
+__global__ void k_dt_chem(double *dt, int strang)
+{
+    if (threadIdx.x == 0) dt[1] = strang ? dt[0] * 0.5 : dt[0];
+}
+
+// ---------------------------------------------------------------------------
+// SSP-RK3 stage on interior cells (Update_kernels.hpp:64-94); dt from device
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void rk3_point(int stage, double dt, double &U, double &U1, double LU)
+{
+    if (stage == 1)
+        U1 = U + dt * LU;
+    else if (stage == 2)
+        U1 = 0.75 * U + 0.25 * U1 + 0.25 * dt * LU;
+    else
+        U = (U + 2.0 * U1 + 2.0 * dt * LU) * OT;
+}
+
+__global__ void __launch_bounds__(256) k_rk3(Geom g, int E, double *__restrict__ U, double *__restrict__ U1,
+                                             const double *__restrict__ LU, const double *__restrict__ dtp, int stage)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long total = (long long)g.X_inner * g.Y_inner * g.Z_inner;
+    if (t >= total) return;
+    const int i = g.Bx + (int)(t % g.X_inner);
+    const int j = g.By + (int)((t / g.X_inner) % g.Y_inner);
+    const int k = g.Bz + (int)(t / ((long long)g.X_inner * g.Y_inner));
+    const long long id = (long long)g.Xmax * g.Ymax * k + (long long)g.Xmax * j + i;
+    const double dt = dtp[0];
+    for (int n = 0; n < E; n++) {
+        double a = U[n * g.N + id], b = U1[n * g.N + id];
+        rk3_point(stage, dt, a, b, LU[n * g.N + id]);
+        if (stage == 3) U[n * g.N + id] = a;
+        else U1[n * g.N + id] = b;
+    }
+}
+
+// EstimateFluidNANKernel (src/Fluids.cpp:47-87) on interior cells
+__global__ void __launch_bounds__(256) k_estimate_nan(Geom g, int E, const double *__restrict__ UI,
+                                                      const double *__restrict__ LU, int *err)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long total = (long long)g.X_inner * g.Y_inner * g.Z_inner;
+    if (t >= total) return;
+    const int i = g.Bx + (int)(t % g.X_inner);
+    const int j = g.By + (int)((t / g.X_inner) % g.Y_inner);
+    const int k = g.Bz + (int)(t / ((long long)g.X_inner * g.Y_inner));
+    const long long id = (long long)g.Xmax * g.Ymax * k + (long long)g.Xmax * j + i;
+    bool bad = UI[id] < 0;
+    for (int n = 0; n < E; n++) {
+        const double a = UI[n * g.N + id], b = LU[n * g.N + id];
+        bad = bad || isnan(a) || isinf(a) || isnan(b) || isinf(b);
+    }
+    if (bad) {
+        atomicOr(&err[0], EBIT_NAN);
+        atomicMin(&err[1], (int)id);
+    }
+}
+
+} // namespace xf

@@ -1,0 +1,79 @@
+// Internal structures shared by the host launcher (xf_api.hip) and the
+// kernels (xf_kernels.hip).  Device layout is structure-of-arrays over the
+// padded grid: field n of cell id lives at ptr[n * ncell + id].
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include "../../include/xf_api.h"
+
+namespace xf {
+
+constexpr int MAXNS = XF_MAXNS;
+constexpr int MAXNR = XF_MAXNR;
+constexpr int MAXE = XF_MAXNS + 4;
+
+// Thermal tables (reference layout, src/include/global_setup.h:202-208)
+struct DevThermo {
+    double Hia[MAXNS * 21];
+    double Hib[MAXNS * 6];
+    double Wi[MAXNS], _Wi[MAXNS], Ri[MAXNS];
+    int thermo; // 1 NASA-9, 0 JANAF
+};
+
+// Reaction tables (reference layout, src/include/global_setup.h:210-215)
+struct DevMech {
+    double Rargus[MAXNR * 6];
+    double third[MAXNR * MAXNS];
+    int Nu_f[MAXNR * MAXNS], Nu_b[MAXNR * MAXNS], Nu_d[MAXNR * MAXNS];
+    int third_ind[MAXNR];
+    int NR;
+};
+
+// Geometry passed by value to every kernel
+struct Geom {
+    int Xmax, Ymax, Zmax;
+    int X_inner, Y_inner, Z_inner;
+    int Bx, By, Bz;
+    int DimX, DimY, DimZ;
+    double _dx, _dy, _dz, CFL;
+    long long N; // padded cells
+};
+
+// Device pointers of one case
+struct Fields {
+    double *U, *U1, *LU;               // E * N each
+    double *rho, *u, *v, *w, *p, *H, *T;
+    double *c, *gamma, *e;             // full primitives (download / checks)
+    double *hi;                        // NS * N species enthalpy at T (COP)
+    double *Cp, *gcv, *R;              // mixture Cp, Cp/Cv, R (COP)
+    double *red;                       // reduction slots, see RED_*
+    double *dt;                        // device-resident dt
+    int *err;                          // [0] code bits, [1] min cell id
+    int *substeps;                     // chemistry predictor passes per interior cell
+    unsigned int *queue;               // chemistry work-queue counter
+};
+
+// reduction slot layout (doubles, compared as uint64 bit patterns)
+// GLF running maxima per direction d: RED_GLF + 3*d + {0: |u-c|, 1: |u|, 2: |u+c|}
+//   (never reset: the reference zeroes eigen_block once, src/Fluids.cpp:427,
+//   and every later sycl::reduction combines with the stored value)
+// GLF maxima of the last UpdateStates pass: RED_GLFP + 3*d + {...}; folded
+//   into RED_GLF when GetLU runs (the Reaction's UpdateStates does not feed GLF)
+// CFL maxima of the last UpdateStates pass per direction d: RED_DT + d
+constexpr int RED_GLF = 0;
+constexpr int RED_GLFP = 9;
+constexpr int RED_DT = 18;
+constexpr int RED_SIZE = 32;
+
+// error code bits in err[0]
+constexpr int EBIT_RHO_Y = 1;
+constexpr int EBIT_PRIM = 2;
+constexpr int EBIT_NAN = 4;
+constexpr int EBIT_CHEM = 8;
+
+// universal gas constant (src/include/global_setup.h:48-54), folded at
+// compile time exactly as the reference's const expressions
+constexpr double R0 = 6.02214076e26 * 1.380649e-23;
+constexpr double Ru = 6.02214076e26 * 1.380649e-23 * 1.0E-3;
+
+} // namespace xf

@@ -1,0 +1,484 @@
+// Characteristic WENO face fluxes and flux divergence, one sweep direction
+// per launch, LDS-staged stencil pencils.
+//
+// Reference: ReconstructFlux{X,Y,Z} (FDM_Method/Reconstruction_kernels.hpp:8-199)
+// with MARCO_ROE (src/include/global_marco.h:26-34), MARCO_COPC2 /
+// ReconstructSoundSpeed (positive-definite_eigen/Utils_device.hpp:42-140),
+// MARCO_NOCOPC2 (Eigen_callback.h:87-91), MARCO_PREEIGEN (Eigen_callback.h:102-106),
+// RoeAverageLeft/Right_{x,y,z} (Eigen_matrix.hpp:7-455), MARCO_FLUXWALL_WENO5
+// (Eigen_callback.h:179-230), weno5old_GPU / weno5Z_{P,M}
+// (schemes/WENO5s_schemes.hpp), UpdateFluidLU (Reconstruction_kernels.hpp:201-234).
+//
+// GPU structure (plain FP64 form):
+//  * a block owns a tile of TS faces along the sweep axis x TW pencils; the
+//    TS+5 cells of each pencil are staged once into LDS as conserved U and
+//    physical flux F (F built from U, p and velocity exactly as GetPhysFlux);
+//  * one lane per face: Roe average, sound speed, then for each of the E
+//    characteristic fields the left row, the 6-point projections of U and F
+//    that WENO5 reads (points i-2..i+3; the reference also projects i-3 and
+//    i+4, which WENO5 never uses), the GLF split, WENO5 and the accumulation
+//    with the right column;
+//  * structurally-zero eigenvector entries are skipped: the reference adds
+//    +/-0 products to an accumulator that starts at +0.0 and is never -0.0
+//    afterwards, so skipping them leaves every sum bit-identical;
+//  * the TS face fluxes go back to LDS and the TS-1 interior cells of the
+//    tile form LU = (F_{i-1/2} - F_{i+1/2}) / dx in the reference's direction
+//    order (x, then y, then z), optionally fused with the SSP-RK3 stage.
+#pragma once
+#include "xf_fluid_kernels.h"
+
+namespace xf {
+
+template <int DIR, int E> struct SweepShape {
+    // X: 64 faces x 4 pencils; Y/Z: 16 faces x 16 pencils (wave = 16 x 4)
+    static constexpr int TS = DIR == 0 ? 64 : 16;
+    static constexpr int TW = 256 / TS;
+    static constexpr int NC = (TS + 5) * TW; // staged cells
+};
+
+// staged-cell LDS index: lanes adjacent in the wave hit adjacent words
+template <int DIR, int TS, int TW>
+__device__ __forceinline__ int lds_ix(int ls, int tcol)
+{
+    return DIR == 0 ? tcol * (TS + 5) + ls : ls * TW + tcol;
+}
+
+// WENO5-JS body (weno5old_BODY, USE_DOUBLE branch)
+__device__ __forceinline__ double weno5js_body(double v1, double v2, double v3, double v4, double v5)
+{
+    double a1, a2, a3;
+    const double dtwo = 2.0, dtre = 3.0;
+    a1 = v1 - dtwo * v2 + v3;
+    double s1 = 13.0 * a1 * a1;
+    a1 = v1 - 4.0 * v2 + dtre * v3;
+    s1 += dtre * a1 * a1;
+    a1 = v2 - dtwo * v3 + v4;
+    double s2 = 13.0 * a1 * a1;
+    a1 = v2 - v4;
+    s2 += dtre * a1 * a1;
+    a1 = v3 - dtwo * v4 + v5;
+    double s3 = 13.0 * a1 * a1;
+    a1 = dtre * v3 - 4.0 * v4 + v5;
+    s3 += dtre * a1 * a1;
+    const double tol = 1.0E-6;
+    s1 += tol, s2 += tol, s3 += tol;
+    a1 = 0.1 * s2 * s2 * s3 * s3;
+    a2 = 0.6 * s1 * s1 * s3 * s3;
+    a3 = 0.3 * s1 * s1 * s2 * s2;
+    const double tw1 = 1.0 / (a1 + a2 + a3);
+    a1 = a1 * tw1, a2 = a2 * tw1, a3 = a3 * tw1;
+    s1 = a1 * (dtwo * v1 - 7.0 * v2 + 11.0 * v3);
+    s2 = a2 * (-v2 + 5.0 * v3 + dtwo * v4);
+    s3 = a3 * (dtwo * v3 + 5.0 * v4 - v5);
+    return (s1 + s2 + s3);
+}
+
+// WENO5-Z body (weno5Z_P / weno5Z_M)
+__device__ __forceinline__ double weno5z_body(double v1, double v2, double v3, double v4, double v5)
+{
+    const double epsilon = 1.0e-6;
+    const double s1 = 13.0 * (v1 - 2.0 * v2 + v3) * (v1 - 2.0 * v2 + v3) + 3.0 * (v1 - 4.0 * v2 + 3.0 * v3) * (v1 - 4.0 * v2 + 3.0 * v3);
+    const double s2 = 13.0 * (v2 - 2.0 * v3 + v4) * (v2 - 2.0 * v3 + v4) + 3.0 * (v2 - v4) * (v2 - v4);
+    const double s3 = 13.0 * (v3 - 2.0 * v4 + v5) * (v3 - 2.0 * v4 + v5) + 3.0 * (3.0 * v3 - 4.0 * v4 + v5) * (3.0 * v3 - 4.0 * v4 + v5);
+    const double s5 = fabs(s1 - s3);
+    const double a1 = 0.1 * (1.0 + s5 / (s1 + epsilon));
+    const double a2 = 0.6 * (1.0 + s5 / (s2 + epsilon));
+    const double a3 = 0.3 * (1.0 + s5 / (s3 + epsilon));
+    const double w1 = a1 / (a1 + a2 + a3);
+    const double w2 = a2 / (a1 + a2 + a3);
+    const double w3 = a3 / (a1 + a2 + a3);
+    return w1 * (2.0 * v1 - 7.0 * v2 + 11.0 * v3) / 6.0 + w2 * (-v2 + 5.0 * v3 + 2.0 * v4) / 6.0 +
+           w3 * (2.0 * v3 + 5.0 * v4 - v5) / 6.0;
+}
+
+template <int SCHEME>
+__device__ __forceinline__ double weno_face(const double *pp, const double *mm)
+{
+    if (SCHEME == XF_WENO5Z)
+        return weno5z_body(pp[1], pp[2], pp[3], pp[4], pp[5]) + weno5z_body(mm[6], mm[5], mm[4], mm[3], mm[2]);
+    const double six = 1.0 / 6.0;
+    const double temf = weno5js_body(pp[1], pp[2], pp[3], pp[4], pp[5]);
+    const double temm = weno5js_body(mm[6], mm[5], mm[4], mm[3], mm[2]);
+    return (temf + temm) * six;
+}
+
+// get_DpDrho / get_DpDrhoi (Utils_device.hpp:14-35)
+__device__ __forceinline__ double dpdrho(double hN, double RN, double q2, double Cp, double R, double T, double gamma)
+{
+    const double RNT = RN * T;
+    return (gamma - 1.0) * (0.5 * q2 - hN + Cp * RNT / R);
+}
+__device__ __forceinline__ double dpdrhoi(double hin, double Rin, double hiN, double RiN, double T, double Cp, double R,
+                                          double gamma)
+{
+    const double hN_minus_hi = -hin + hiN;
+    const double Ri_minus_RN = (Rin - RiN);
+    return (gamma - 1.0) * (hN_minus_hi + Cp * Ri_minus_RN * T / R);
+}
+
+// Face-local Roe state
+template <int NS> struct RoeFace {
+    double _u, _v, _w, _H, _P, _rho, c2, b1, b3;
+    double _yi[NS];
+    double z[NS > 1 ? NS - 1 : 1];
+};
+
+// ReconstructSoundSpeed + SoundSpeedMultiSpecies (Utils_device.hpp:42-140)
+// from the per-cell ingredients stored by the states pass.
+template <int NS>
+__device__ __forceinline__ void cop_c2(const DevThermo *__restrict__ th, RoeFace<NS> &rf, double D, double D1,
+                                       const double *yl, const double *yr, const double *hl, const double *hr,
+                                       double T_l, double T_r, double gamma_l, double gamma_r, double R_l, double R_r,
+                                       double Cp_l, double Cp_r, double rl, double rr, double ul, double vl, double wl,
+                                       double ur, double vr, double wr, double pl, double pr, double Hl, double Hr)
+{
+    constexpr int NCOP = NS - 1;
+#pragma unroll
+    for (int n = 0; n < NS; n++) rf._yi[n] = (yl[n] + D * yr[n]) * D1;
+    const double Gamma = (gamma_l + D * gamma_r) * D1;
+    const double q2_l = ul * ul + vl * vl + wl * wl;
+    const double q2_r = ur * ur + vr * vr + wr * wr;
+    const double e_l = Hl - 0.5 * q2_l - pl / rl, e_r = Hr - 0.5 * q2_r - pr / rr;
+    const double RN = th->Ri[NCOP];
+    const double _dpdrho = (dpdrho(hl[NCOP], RN, q2_l, Cp_l, R_l, T_l, gamma_l) +
+                            D * dpdrho(hr[NCOP], RN, q2_r, Cp_r, R_r, T_r, gamma_r)) * D1;
+    double _dpdrhoi[NCOP > 0 ? NCOP : 1], drhoi[NCOP > 0 ? NCOP : 1];
+#pragma unroll
+    for (int nn = 0; nn < NCOP; nn++) {
+        const double Rin = th->Ri[nn];
+        _dpdrhoi[nn] = (dpdrhoi(hl[nn], Rin, hl[NCOP], RN, T_l, Cp_l, R_l, gamma_l) +
+                        D * dpdrhoi(hr[nn], Rin, hr[NCOP], RN, T_r, Cp_r, R_r, gamma_r)) * D1;
+        drhoi[nn] = rr * yr[nn] - rl * yl[nn];
+    }
+    const double _prho = (pl / rl + D * (pr / rr)) * D1 +
+                         0.5 * D * D1 * D1 * ((ur - ul) * (ur - ul) + (vr - vl) * (vr - vl) + (wr - wl) * (wr - wl));
+    const double _dpdE = ((gamma_l - 1.0) + D * (gamma_r - 1.0)) * D1;
+    const double _dpde = ((gamma_l - 1.0) * rl + D * ((gamma_r - 1.0) * rr)) * D1;
+    const double dp = pr - pl, drho = rr - rl, de = e_r - e_l;
+    double Sum_dpdrhoi = 0.0, Sum_dpdrhoi2 = 0.0, Sum_Yidpdrhoi = 0.0;
+#pragma unroll
+    for (int n = 0; n < NCOP; n++) {
+        Sum_dpdrhoi += _dpdrhoi[n] * drhoi[n];
+        Sum_dpdrhoi2 += _dpdrhoi[n] * drhoi[n] * _dpdrhoi[n] * drhoi[n];
+    }
+    const double temp1 = dp - (_dpdrho * drho + _dpde * de + Sum_dpdrhoi);
+    const double temp = temp1 / (_dpdrho * _dpdrho * drho * drho + _dpde * de * _dpde * de + Sum_dpdrhoi2 + 1e-19);
+#pragma unroll
+    for (int n = 0; n < NCOP; n++) Sum_Yidpdrhoi += rf._yi[n] * _dpdrhoi[n];
+    const double _dpdE_new = _dpdE + _dpdE * _dpdE * de * rf._rho * temp;
+    const double _dpdrho_new = _dpdrho + _dpdrho * _dpdrho * drho * temp;
+    const double csqr = _dpdrho_new + _dpdE_new * _prho + Sum_Yidpdrhoi;
+    rf.b1 = _dpdE_new / csqr;
+    double b3 = 0.0;
+#pragma unroll
+    for (int n = 0; n < NCOP; n++) {
+        const double nw = _dpdrhoi[n] + _dpdrhoi[n] * _dpdrhoi[n] * drhoi[n] * temp;
+        rf.z[n] = -nw / _dpdE_new;
+        b3 += rf._yi[n] * rf.z[n];
+    }
+    rf.b3 = b3 * rf.b1;
+    double c2 = csqr;
+    const double c2w = (c2 <= 0.0) ? 1.0 : 0.0;
+    c2 = Gamma * rf._P * rf._rho * c2w + (1.0 - c2w) * c2;
+    rf.c2 = c2;
+}
+
+} // namespace xf
+
+namespace xf {
+
+// GLF dissipation of characteristic field n (Artificial_type 3): the running
+// maximum of |eigen_local[n]|; fields 1..E-2 share the |u_n| maximum.
+template <int DIR, int E>
+__device__ __forceinline__ double glf_alpha(const double *red, int n)
+{
+    return red[RED_GLF + 3 * DIR + (n == 0 ? 0 : (n == E - 1 ? 2 : 1))];
+}
+
+template <int NS, bool COP, int DIR, int SCHEME>
+__global__ void __launch_bounds__(256) k_sweep(Geom g, const DevThermo *__restrict__ th, const double *__restrict__ Ug,
+                                               Fields f, double ncop_gamma, int mode, int stage, const double *Ua,
+                                               const double *Ub, double *Uo)
+{
+    constexpr int E = COP ? NS + 4 : 5;
+    constexpr int NCOP = COP ? NS - 1 : 0;
+    using SS = SweepShape<DIR, E>;
+    constexpr int TS = SS::TS, TW = SS::TW, NC = SS::NC;
+    __shared__ double sU[E * NC];
+    __shared__ double sF[E * NC];
+
+    const long long N = g.N;
+    const int tid = threadIdx.x;
+    const int fidx = DIR == 0 ? tid % TS : tid / TW;
+    const int tcol = DIR == 0 ? tid / TS : tid % TW;
+    const int B = DIR == 0 ? g.Bx : (DIR == 1 ? g.By : g.Bz);
+    const int inner = DIR == 0 ? g.X_inner : (DIR == 1 ? g.Y_inner : g.Z_inner);
+    const int Lmax = DIR == 0 ? g.Xmax : (DIR == 1 ? g.Ymax : g.Zmax);
+    const int c0 = B + (TS - 1) * (int)(DIR == 0 ? blockIdx.x : blockIdx.y);
+    const long long stride = DIR == 0 ? 1LL : (DIR == 1 ? (long long)g.Xmax : (long long)g.Xmax * g.Ymax);
+    const double *vel_n = DIR == 0 ? f.u : (DIR == 1 ? f.v : f.w);
+
+    auto pencil_base = [&](int q, bool &ok) -> long long {
+        if (DIR == 0) {
+            const int r = (int)blockIdx.y * TW + q;
+            ok = r < g.Y_inner * g.Z_inner;
+            const int j = g.By + r % g.Y_inner, k = g.Bz + r / g.Y_inner;
+            return (long long)g.Xmax * g.Ymax * k + (long long)g.Xmax * j;
+        } else if (DIR == 1) {
+            const int i = g.Bx + (int)blockIdx.x * TW + q;
+            ok = i < g.Bx + g.X_inner;
+            return (long long)g.Xmax * g.Ymax * (g.Bz + (int)blockIdx.z) + i;
+        } else {
+            const int i = g.Bx + (int)blockIdx.x * TW + q;
+            ok = i < g.Bx + g.X_inner;
+            return (long long)g.Xmax * (g.By + (int)blockIdx.z) + i;
+        }
+    };
+
+    // ---- stage U and the physical flux F of TS+5 cells per pencil ----------
+    for (int q = tid; q < NC; q += 256) {
+        const int pc = DIR == 0 ? q / (TS + 5) : q % TW;
+        const int ls = DIR == 0 ? q % (TS + 5) : q / TW;
+        bool ok;
+        const long long base = pencil_base(pc, ok);
+        const int s = c0 - 3 + ls;
+        const int li = lds_ix<DIR, TS, TW>(ls, pc);
+        if (ok && s < Lmax) {
+            const long long id = base + (long long)s * stride;
+            double U[E];
+#pragma unroll
+            for (int n = 0; n < E; n++) U[n] = Ug[n * N + id];
+            const double vel[3] = {f.u[id], f.v[id], f.w[id]};
+            const double p = f.p[id];
+            const double Um = U[1 + DIR];
+            sU[0 * NC + li] = U[0];
+            sF[0 * NC + li] = Um;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                sU[(1 + k) * NC + li] = U[1 + k];
+                sF[(1 + k) * NC + li] = (k == DIR) ? Um * vel[k] + p : Um * vel[k];
+            }
+            sU[4 * NC + li] = U[4];
+            sF[4 * NC + li] = (U[4] + p) * vel[DIR];
+            if (COP) {
+                const double rho1 = 1.0 / U[0];
+#pragma unroll
+                for (int ii = 5; ii < E; ii++) {
+                    sU[ii * NC + li] = U[ii];
+                    sF[ii * NC + li] = Um * (U[ii] * rho1);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int n = 0; n < E; n++) sU[n * NC + li] = 0.0, sF[n * NC + li] = 0.0;
+        }
+    }
+    __syncthreads();
+
+    // ---- one characteristic face flux per lane ------------------------------
+    bool okp;
+    const long long pbase = pencil_base(tcol, okp);
+    const int sl = c0 + fidx - 1;
+    const bool face_ok = okp && (sl <= B + inner - 1);
+    double flux[E];
+#pragma unroll
+    for (int n = 0; n < E; n++) flux[n] = 0.0;
+    if (face_ok) {
+        const long long id_l = pbase + (long long)sl * stride;
+        const long long id_r = id_l + stride;
+        const int li_l = lds_ix<DIR, TS, TW>(fidx + 2, tcol);
+        const int li_r = lds_ix<DIR, TS, TW>(fidx + 3, tcol);
+        const double rl = sU[li_l], rr = sU[li_r];
+        const double ul = f.u[id_l], vl = f.v[id_l], wl = f.w[id_l], pl = f.p[id_l], Hl = f.H[id_l];
+        const double ur = f.u[id_r], vr = f.v[id_r], wr = f.w[id_r], pr = f.p[id_r], Hr = f.H[id_r];
+        // MARCO_ROE
+        const double D = sqrt(rr / rl);
+        const double D1 = 1.0 / (D + 1.0);
+        RoeFace<COP ? NS : 1> rf;
+        rf._u = (ul + D * ur) * D1;
+        rf._v = (vl + D * vr) * D1;
+        rf._w = (wl + D * wr) * D1;
+        rf._H = (Hl + D * Hr) * D1;
+        rf._P = (pl + D * pr) * D1;
+        rf._rho = sqrt(rr * rl);
+        if (COP) {
+            double yl[COP ? NS : 1], yr[COP ? NS : 1], hl[COP ? NS : 1], hr[COP ? NS : 1];
+            const double r1l = 1.0 / rl, r1r = 1.0 / rr;
+            yl[NCOP] = 1.0;
+            yr[NCOP] = 1.0;
+#pragma unroll
+            for (int k = 0; k < NCOP; k++) {
+                yl[k] = sU[(5 + k) * NC + li_l] * r1l, yl[NCOP] += -yl[k];
+                yr[k] = sU[(5 + k) * NC + li_r] * r1r, yr[NCOP] += -yr[k];
+            }
+#pragma unroll
+            for (int k = 0; k < (COP ? NS : 1); k++) hl[k] = f.hi[k * N + id_l], hr[k] = f.hi[k * N + id_r];
+            cop_c2<COP ? NS : 1>(th, rf, D, D1, yl, yr, hl, hr, f.T[id_l], f.T[id_r], f.gcv[id_l], f.gcv[id_r],
+                                 f.R[id_l], f.R[id_r], f.Cp[id_l], f.Cp[id_r], rl, rr, ul, vl, wl, ur, vr, wr, pl, pr,
+                                 Hl, Hr);
+        } else {
+            rf._yi[0] = 1.0;
+            rf.c2 = ncop_gamma * rf._P / rf._rho;
+            rf.b1 = (ncop_gamma - 1.0) / rf.c2;
+            rf.b3 = 0.0;
+        }
+        // MARCO_PREEIGEN (identical for every row, evaluated once)
+        const double _u = rf._u, _v = rf._v, _w = rf._w, _H = rf._H, b1 = rf.b1, b3 = rf.b3;
+        const double q2 = _u * _u + _v * _v + _w * _w;
+        const double _c = sqrt(rf.c2);
+        const double b2 = 1.0 + b1 * q2 - b1 * _H;
+        const double _c1 = 1.0 / _c;
+        const double vr3[3] = {_u, _v, _w};
+        const double un = vr3[DIR];
+        const double un_c = un * _c1;
+
+#pragma unroll
+        for (int n = 0; n < E; n++) {
+            const double av = glf_alpha<DIR, E>(f.red, n);
+            double uf[7], ff[7];
+            const bool dense = (n == 0) || (n == E - 1) || (n == 1 + DIR);
+            if (dense) {
+                double L[E];
+                if (n == 0) {
+                    L[0] = 0.5 * (b2 + un_c + b3);
+#pragma unroll
+                    for (int k = 0; k < 3; k++)
+                        L[1 + k] = (k == DIR) ? -0.5 * (b1 * vr3[k] + _c1) : -0.5 * (b1 * vr3[k]);
+                    L[4] = 0.5 * b1;
+#pragma unroll
+                    for (int m = 0; m < NCOP; m++) L[5 + m] = -0.5 * b1 * rf.z[m];
+                } else if (n == E - 1) {
+                    L[0] = 0.5 * (b2 - un_c + b3);
+#pragma unroll
+                    for (int k = 0; k < 3; k++)
+                        L[1 + k] = (k == DIR) ? 0.5 * (-b1 * vr3[k] + _c1) : 0.5 * (-b1 * vr3[k]);
+                    L[4] = 0.5 * b1;
+#pragma unroll
+                    for (int m = 0; m < NCOP; m++) L[5 + m] = -0.5 * b1 * rf.z[m];
+                } else {
+                    L[0] = (1.0 - b2 - b3) / b1;
+                    L[1] = _u;
+                    L[2] = _v;
+                    L[3] = _w;
+                    L[4] = -1.0;
+#pragma unroll
+                    for (int m = 0; m < NCOP; m++) L[5 + m] = rf.z[m];
+                }
+#pragma unroll
+                for (int m = 1; m <= 6; m++) {
+                    const int li = lds_ix<DIR, TS, TW>(fidx + m - 1, tcol);
+                    double a = 0.0, b = 0.0;
+#pragma unroll
+                    for (int n1 = 0; n1 < E; n1++) {
+                        a = a + sU[n1 * NC + li] * L[n1];
+                        b = b + sF[n1 * NC + li] * L[n1];
+                    }
+                    uf[m] = a;
+                    ff[m] = b;
+                }
+            } else {
+                // two structural non-zeros: L[0] and +-1 at component kk
+                double L0;
+                int kk;
+                bool neg;
+                if (n >= 1 && n <= 3) {
+                    kk = n;
+                    if (DIR == 0) { L0 = (n == 2) ? _v : -_w; neg = (n == 2); }
+                    else if (DIR == 1) { L0 = (n == 1) ? -_u : _w; neg = (n == 3); }
+                    else { L0 = (n == 1) ? _u : -_v; neg = (n == 1); }
+                } else {
+                    kk = 5 + (n - 4);
+                    L0 = -rf._yi[n - 4];
+                    neg = false;
+                }
+#pragma unroll
+                for (int m = 1; m <= 6; m++) {
+                    const int li = lds_ix<DIR, TS, TW>(fidx + m - 1, tcol);
+                    const double a0 = 0.0 + sU[li] * L0;
+                    const double b0 = 0.0 + sF[li] * L0;
+                    const double au = sU[kk * NC + li], bf = sF[kk * NC + li];
+                    uf[m] = neg ? a0 + (-au) : a0 + au;
+                    ff[m] = neg ? b0 + (-bf) : b0 + bf;
+                }
+            }
+            double pp[7], mm[7];
+#pragma unroll
+            for (int m = 1; m <= 5; m++) pp[m] = 0.5 * (ff[m] + av * uf[m]);
+#pragma unroll
+            for (int m = 2; m <= 6; m++) mm[m] = 0.5 * (ff[m] - av * uf[m]);
+            const double fl = weno_face<SCHEME>(pp, mm);
+            // right column n, accumulated in the reference's summation order
+            if (n == 0 || n == E - 1) {
+                const double sg = (n == 0) ? -1.0 : 1.0;
+                flux[0] += fl * 1.0;
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    const double Rk = (k == DIR) ? (n == 0 ? vr3[k] - _c : vr3[k] + _c) : vr3[k];
+                    flux[1 + k] += fl * Rk;
+                }
+                flux[4] += fl * (n == 0 ? _H - un * _c : _H + un * _c);
+#pragma unroll
+                for (int m = 0; m < NCOP; m++) flux[5 + m] += fl * rf._yi[m];
+                (void)sg;
+            } else if (n == 1 + DIR) {
+                flux[0] += fl * b1;
+                flux[1] += fl * (_u * b1);
+                flux[2] += fl * (_v * b1);
+                flux[3] += fl * (_w * b1);
+                flux[4] += fl * (_H * b1 - 1.0);
+#pragma unroll
+                for (int m = 0; m < NCOP; m++) flux[5 + m] += fl * (b1 * rf._yi[m]);
+            } else if (n >= 1 && n <= 3) {
+                // (component, sign) and the energy entry
+                double R4;
+                int kk;
+                bool neg;
+                if (DIR == 0) { kk = n; neg = (n == 2); R4 = (n == 2) ? -_v : _w; }
+                else if (DIR == 1) { kk = n; neg = (n == 3); R4 = (n == 1) ? _u : -_w; }
+                else { kk = n; neg = (n == 1); R4 = (n == 1) ? -_u : _v; }
+                flux[kk] += neg ? -fl : fl;
+                flux[4] += fl * R4;
+            } else {
+                flux[4] += fl * rf.z[n - 4];
+                flux[5 + (n - 4)] += fl;
+            }
+        }
+    }
+
+    // ---- flux divergence on the TS-1 cells of the tile ----------------------
+    __syncthreads();
+    double *sFw = sU; // reuse the staging area (E * NC >= E * 256)
+    const int fw = DIR == 0 ? tcol * TS + fidx : fidx * TW + tcol;
+#pragma unroll
+    for (int n = 0; n < E; n++) sFw[n * 256 + fw] = flux[n];
+    __syncthreads();
+    if (fidx >= 1 && okp && sl <= B + inner - 1) {
+        const long long id = pbase + (long long)sl * stride;
+        const int fwm = DIR == 0 ? tcol * TS + fidx - 1 : (fidx - 1) * TW + tcol;
+        const double dl = DIR == 0 ? g._dx : (DIR == 1 ? g._dy : g._dz);
+        const double dt = (mode & 2) ? f.dt[0] : 0.0;
+        bool bad = false;
+#pragma unroll
+        for (int n = 0; n < E; n++) {
+            const double d = (sFw[n * 256 + fwm] - sFw[n * 256 + fw]) * dl;
+            const double LUt = (mode & 1) ? f.LU[n * N + id] + d : 0.0 + d;
+            if (mode & 2) {
+                // fused SSP-RK3 stage: Ua = U, Ub = stage input U1 (stage 2, 3), Uo = output
+                double a = Ua[n * N + id], b = (stage == 1) ? 0.0 : Ub[n * N + id];
+                bad = bad || isnan(LUt) || isinf(LUt);
+                rk3_point(stage, dt, a, b, LUt);
+                const double o = (stage == 3) ? a : b;
+                bad = bad || isnan(o) || isinf(o) || (n == 0 && o < 0);
+                Uo[n * N + id] = o;
+            } else {
+                f.LU[n * N + id] = LUt;
+            }
+        }
+        if (bad) {
+            atomicOr(&f.err[0], EBIT_NAN);
+            atomicMin(&f.err[1], (int)id);
+        }
+    }
+}
+
+} // namespace xf

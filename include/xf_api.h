@@ -169,7 +169,17 @@ int xf_last_error(xf_ctx *ctx, int *code, int *i, int *j, int *k);
 /* Average device time (ms) per launch of each kernel class since the last
    reset, measured with HIP events on the context stream when profiling is on. */
 int xf_set_profiling(xf_ctx *ctx, int on);
+/* Restrict profiling events to the comma-separated kernel classes in csv
+   ("states,sweep_x,sweep_y,sweep_z,boundary,rk3,chem"); NULL or "" = all. */
+int xf_set_profile_filter(xf_ctx *ctx, const char *csv);
 int xf_kernel_times(xf_ctx *ctx, const char **names, double *avg_ms, int *counts, int cap);
+/* Average device time (ms) of one kernel class, launched reps times back to
+   back on the context stream between two HIP events, in the configuration
+   of RK stage 1 of the fused step: "states" (conservative->primitive on U)
+   or "sweep_x"/"sweep_y"/"sweep_z" (face fluxes; the last active direction
+   carries the fused RK update U1 = U + dt LU).  Overwrites only LU, U1 and
+   the primitives, which every stage recomputes. */
+int xf_time_kernel(xf_ctx *ctx, const char *name, int reps, double *avg_ms);
 /* Evaluate the device exp and log (the fixed-sequence FP64 functions every
    kernel uses) on n host values; for bit-level checks against the CPU. */
 int xf_debug_math(const double *x, double *exp_out, double *log_out, int n);

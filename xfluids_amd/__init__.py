@@ -107,6 +107,8 @@ _SIGS = {
     "xf_stream": (_V, [_V]),
     "xf_last_error": (C.c_int, [_V, _IP, _IP, _IP, _IP]),
     "xf_set_profiling": (C.c_int, [_V, C.c_int]),
+    "xf_set_profile_filter": (C.c_int, [_V, C.c_char_p]),
+    "xf_time_kernel": (C.c_int, [_V, C.c_char_p, C.c_int, _DP]),
     "xf_kernel_times": (C.c_int, [_V, C.POINTER(C.c_char_p), _DP, _IP, C.c_int]),
     "xf_setup_load": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(_V), C.c_char_p, C.c_size_t]),
     "xf_setup_free": (None, [_V]),
@@ -352,8 +354,18 @@ class Context:
     def stream(self):
         return lib().xf_stream(self._h)
 
-    def set_profiling(self, on=True):
+    def set_profiling(self, on=True, only=None):
+        """Per-launch HIP-event timing on the context stream; only = list of
+        kernel classes to time (others run without events)."""
         self._chk(lib().xf_set_profiling(self._h, 1 if on else 0), "xf_set_profiling")
+        csv = ",".join(only) if only else ""
+        self._chk(lib().xf_set_profile_filter(self._h, csv.encode()), "xf_set_profile_filter")
+
+    def time_kernel(self, name, reps=50):
+        """Average device ms of one kernel class over reps back-to-back launches."""
+        ms = C.c_double()
+        self._chk(lib().xf_time_kernel(self._h, name.encode(), reps, C.byref(ms)), "xf_time_kernel")
+        return ms.value
 
     def kernel_times(self):
         cap = 32

@@ -47,6 +47,7 @@ struct xf_ctx {
     hipGraphExec_t gexec = nullptr;
     int last_code = 0, last_id = -1;
     bool prof = false;
+    std::string prof_filter; // ",name,name," or empty for all
     std::vector<EvPair> evs;
     std::map<std::string, std::pair<double, int>> times;
 };
@@ -68,9 +69,11 @@ struct ProfScope {
     xf_ctx *c;
     const char *name;
     hipEvent_t a{}, b{};
+    bool on;
     ProfScope(xf_ctx *c_, const char *n) : c(c_), name(n)
     {
-        if (c->prof) {
+        on = c->prof && (c->prof_filter.empty() || c->prof_filter.find("," + std::string(n) + ",") != std::string::npos);
+        if (on) {
             (void)hipEventCreate(&a);
             (void)hipEventCreate(&b);
             (void)hipEventRecord(a, c->stream);
@@ -78,7 +81,7 @@ struct ProfScope {
     }
     ~ProfScope()
     {
-        if (c->prof) {
+        if (on) {
             (void)hipEventRecord(b, c->stream);
             c->evs.push_back({name, a, b});
         }
@@ -191,32 +194,31 @@ int take_error(xf_ctx *c)
     return code;
 }
 
-void boundary_async(xf_ctx *c, double *U)
+// FluidBoundaryCondition: all ghost cells in one launch; reset_red also
+// zeroes the per-pass maxima for the UpdateStates pass that follows
+void boundary_async(xf_ctx *c, double *U, int reset_red)
 {
     ProfScope ps(c, "boundary");
     const Geom &g = c->g;
-    if (g.DimX)
-        hipLaunchKernelGGL(k_boundary<0>, dim3(nblk((long long)g.Bx * g.Ymax * g.Zmax)), dim3(256), 0, c->stream, g, U,
-                           c->E, c->NCOP, c->d.BC[0], c->d.BC[1]);
-    if (g.DimY)
-        hipLaunchKernelGGL(k_boundary<1>, dim3(nblk((long long)g.By * g.Xmax * g.Zmax)), dim3(256), 0, c->stream, g, U,
-                           c->E, c->NCOP, c->d.BC[2], c->d.BC[3]);
-    if (g.DimZ)
-        hipLaunchKernelGGL(k_boundary<2>, dim3(nblk((long long)g.Bz * g.Xmax * g.Ymax)), dim3(256), 0, c->stream, g, U,
-                           c->E, c->NCOP, c->d.BC[4], c->d.BC[5]);
+    const long long n = (g.DimZ ? 2LL * g.Bz * g.Ymax * g.Xmax : 0) + (g.DimY ? 2LL * g.By * g.Xmax * g.Z_inner : 0) +
+                        (g.DimX ? 2LL * g.Bx * g.Y_inner * g.Z_inner : 0);
+    hipLaunchKernelGGL(k_boundary_all, dim3(std::max(1u, nblk(n))), dim3(256), 0, c->stream, g, U, c->E, c->d.BC[0],
+                       c->d.BC[1], c->d.BC[2], c->d.BC[3], c->d.BC[4], c->d.BC[5], c->f.red, reset_red);
 }
 
-void states_async(xf_ctx *c, const double *U, int full)
+// UpdateStates; reset = 0 when the preceding boundary launch already zeroed
+// the per-pass maxima
+void states_async(xf_ctx *c, const double *U, int full, int reset = 1)
 {
-    hipLaunchKernelGGL(k_reset_pass_red, dim3(1), dim3(64), 0, c->stream, c->f.red);
+    if (reset) hipLaunchKernelGGL(k_reset_pass_red, dim3(1), dim3(64), 0, c->stream, c->f.red);
     c->ops.states(c, U, full);
 }
 
-// GetLU: fold GLF, sweeps in x, y, z order.  rk_stage > 0 fuses the SSP-RK3
-// stage into the last sweep (Ua, Ub inputs, Uo output).
+// GetLU: sweeps in x, y, z order (each folds its direction's GLF maxima).
+// rk_stage > 0 fuses the SSP-RK3 stage into the last sweep (Ua, Ub inputs,
+// Uo output).
 void lu_async(xf_ctx *c, const double *Ug, int rk_stage, const double *Ua, const double *Ub, double *Uo)
 {
-    hipLaunchKernelGGL(k_fold_glf, dim3(1), dim3(64), 0, c->stream, c->f.red);
     int dirs[3], nd = 0;
     if (c->g.DimX) dirs[nd++] = 0;
     if (c->g.DimY) dirs[nd++] = 1;
@@ -244,8 +246,8 @@ void step_async(xf_ctx *c)
     double *in[3] = {f.U, f.U1, c->U2};
     for (int s = 1; s <= 3; s++) {
         double *Ug = in[s - 1];
-        boundary_async(c, Ug);
-        states_async(c, Ug, 0);
+        boundary_async(c, Ug, 1);
+        states_async(c, Ug, 0, 0);
         double *Uo = (s == 1) ? f.U1 : (s == 2 ? c->U2 : f.U);
         lu_async(c, Ug, s, f.U, (s == 1) ? f.U1 : Ug, Uo);
     }
@@ -513,7 +515,7 @@ int xf_download_maxima(xf_ctx *c, double *eigen_block, double *uvw_c_max)
 int xf_boundary(xf_ctx *c, int which)
 {
     if (!c || which < 0 || which > 2) return XF_EINVAL;
-    boundary_async(c, which_ptr(c, which));
+    boundary_async(c, which_ptr(c, which), 0);
     if (int e = check_launch()) return e;
     XF_HIP(hipStreamSynchronize(c->stream));
     return XF_OK;
@@ -675,6 +677,58 @@ int xf_set_profiling(xf_ctx *c, int on)
     c->evs.clear();
     c->times.clear();
     c->prof = on != 0;
+    return XF_OK;
+}
+
+int xf_time_kernel(xf_ctx *c, const char *name, int reps, double *avg_ms)
+{
+    if (!c || !name || reps <= 0 || !avg_ms) return XF_EINVAL;
+    if (c->d.cop && !c->thermo_set) return XF_EINVAL;
+    const std::string n(name);
+    int dirs[3], nd = 0;
+    if (c->g.DimX) dirs[nd++] = 0;
+    if (c->g.DimY) dirs[nd++] = 1;
+    if (c->g.DimZ) dirs[nd++] = 2;
+    int dir = -1;
+    if (n == "sweep_x") dir = 0;
+    else if (n == "sweep_y") dir = 1;
+    else if (n == "sweep_z") dir = 2;
+    else if (n != "states") return XF_EINVAL;
+    int q = -1;
+    for (int i = 0; i < nd; i++)
+        if (dirs[i] == dir) q = i;
+    if (dir >= 0 && q < 0) return XF_EINVAL;
+    const bool prof = c->prof;
+    c->prof = false;
+    hipEvent_t a, b;
+    XF_HIP(hipEventCreate(&a));
+    XF_HIP(hipEventCreate(&b));
+    XF_HIP(hipEventRecord(a, c->stream));
+    for (int r = 0; r < reps; r++) {
+        if (dir < 0) {
+            c->ops.states(c, c->f.U, 0);
+        } else {
+            int mode = (q > 0) ? 1 : 0;
+            if (q == nd - 1) mode |= 2;
+            c->ops.sweep(c, dir, c->f.U, mode, 1, c->f.U, c->f.U1, c->f.U1);
+        }
+    }
+    XF_HIP(hipEventRecord(b, c->stream));
+    XF_HIP(hipEventSynchronize(b));
+    float ms = 0.f;
+    XF_HIP(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    c->prof = prof;
+    *avg_ms = ms / reps;
+    if (int e = check_launch()) return e;
+    return take_error(c);
+}
+
+int xf_set_profile_filter(xf_ctx *c, const char *csv)
+{
+    if (!c) return XF_EINVAL;
+    c->prof_filter = (csv && *csv) ? "," + std::string(csv) + "," : std::string();
     return XF_OK;
 }
 

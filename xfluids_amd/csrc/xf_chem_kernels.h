@@ -225,7 +225,7 @@ __device__ __forceinline__ int chem_cell(const DevThermo *__restrict__ th, const
     constexpr int E = NS + 4;
     double y[NS + 1];
     y[0] = f.T[id];
-    const double rho = f.rho[id];
+    const double rho = f.U[id]; // rho = U[0] (Getrhoyi)
     const double rho1 = 1.0 / rho;
     y[1 + NS - 1] = 1.0;
 #pragma unroll

@@ -82,31 +82,72 @@ __device__ __forceinline__ void bc_apply(double *__restrict__ U, long long N, in
     }
 }
 
-// DIR 0: threads over (Bx, Ymax, Zmax); 1: (Xmax, By, Zmax); 2: (Xmax, Ymax, Bz)
-template <int DIR>
-__global__ void __launch_bounds__(256) k_boundary(Geom g, double *__restrict__ U, int E, int NCOP, int bc_lo, int bc_hi)
+// All ghost cells in one pass.  The reference fills x, then y, then z
+// (BCs_block.cpp:50-212), each pass reading cells the previous one may have
+// written (corners).  Composing the three axis maps per ghost cell gives the
+// same values in a single launch: starting from the cell, apply the z map if
+// it is a z ghost, then the y map, then the x map; the value is the source
+// cell's value with the momentum sign flips of every map applied.  Sources
+// are interior cells or Inflow ghosts (which no pass writes), so no thread
+// reads a cell another thread writes.  Copies and negations are exact.
+// Thread 0 also zeroes the per-pass reduction slots for the following
+// UpdateStates pass when reset_red is set.
+__device__ __forceinline__ void bc_axis(int bc, int &s, int B, int inner, int Lmax, int mom, int &flip)
 {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int B = DIR == 0 ? g.Bx : (DIR == 1 ? g.By : g.Bz);
-    const int inner = DIR == 0 ? g.X_inner : (DIR == 1 ? g.Y_inner : g.Z_inner);
-    const int Lmax = DIR == 0 ? g.Xmax : (DIR == 1 ? g.Ymax : g.Zmax);
-    const int A = DIR == 0 ? g.Ymax : g.Xmax;       // fastest transverse extent
-    const int C = DIR == 2 ? g.Ymax : g.Zmax;       // slowest transverse extent
-    const long long total = (long long)B * A * C;
-    if (t >= total) return;
-    const int a = (int)(t % A);
-    const int gg = (int)((t / A) % B);
-    const int cc = (int)(t / ((long long)A * B));
-    const int lo = gg, hi = gg + Lmax - B;
-    const int tlo = bc_target(bc_lo, lo, B, inner, 0, B, 1);
-    const int thi = bc_target(bc_hi, hi, B, inner, inner, Lmax - B - 1, -1);
-    auto idx = [&](int s) -> long long {
-        if (DIR == 0) return (long long)g.Xmax * g.Ymax * cc + (long long)g.Xmax * a + s;
-        if (DIR == 1) return (long long)g.Xmax * g.Ymax * cc + (long long)g.Xmax * s + a;
-        return (long long)g.Xmax * g.Ymax * s + (long long)g.Xmax * cc + a;
-    };
-    bc_apply(U, g.N, E, NCOP, idx(lo), idx(tlo), bc_lo, DIR + 1);
-    bc_apply(U, g.N, E, NCOP, idx(hi), idx(thi), bc_hi, DIR + 1);
+    if (s >= B && s < Lmax - B) return;
+    const bool lo = s < B;
+    const int t = lo ? bc_target(bc, s, B, inner, 0, B, 1) : bc_target(bc, s, B, inner, inner, Lmax - B - 1, -1);
+    if (bc == XF_BC_SYMMETRY) flip ^= 1 << mom;
+    if (bc == XF_BC_NSLIPWALL || bc == XF_BC_VISCWALL || bc == XF_BC_SLIPWALL) flip ^= (1 << 1) | (1 << 2) | (1 << 3);
+    s = t;
+}
+
+__global__ void __launch_bounds__(256) k_boundary_all(Geom g, double *__restrict__ U, int E, int bx0, int bx1, int by0,
+                                                      int by1, int bz0, int bz1, double *red, int reset_red)
+{
+    if (reset_red && blockIdx.x == 0 && threadIdx.x < 12) {
+        if (threadIdx.x < 9) red[RED_GLFP + threadIdx.x] = 0.0;
+        else red[RED_DT + threadIdx.x - 9] = 0.0;
+    }
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    // enumerate the ghost shell: z slabs, then y slabs inside interior z,
+    // then x slabs inside interior y and z
+    const long long nz = g.DimZ ? 2LL * g.Bz * g.Ymax * g.Xmax : 0;
+    const long long ny = g.DimY ? 2LL * g.By * g.Xmax * g.Z_inner : 0;
+    const long long nx = g.DimX ? 2LL * g.Bx * g.Y_inner * g.Z_inner : 0;
+    int i, j, k;
+    if (t < nz) {
+        const long long per = (long long)g.Ymax * g.Xmax;
+        const int gz = (int)(t / per);
+        const long long r = t % per;
+        k = gz < g.Bz ? gz : gz - g.Bz + g.Zmax - g.Bz;
+        j = (int)(r / g.Xmax), i = (int)(r % g.Xmax);
+    } else if ((t -= nz) < ny) {
+        const long long per = (long long)g.Xmax * g.Z_inner;
+        const int gy = (int)(t / per);
+        const long long r = t % per;
+        j = gy < g.By ? gy : gy - g.By + g.Ymax - g.By;
+        k = g.Bz + (int)(r / g.Xmax), i = (int)(r % g.Xmax);
+    } else if ((t -= ny) < nx) {
+        const long long per = (long long)g.Y_inner * g.Z_inner;
+        const int gx = (int)(t / per);
+        const long long r = t % per;
+        i = gx < g.Bx ? gx : gx - g.Bx + g.Xmax - g.Bx;
+        k = g.Bz + (int)(r / g.Y_inner), j = g.By + (int)(r % g.Y_inner);
+    } else
+        return;
+    int si = i, sj = j, sk = k, flip = 0;
+    if (g.DimZ) bc_axis(k < g.Bz ? bz0 : bz1, sk, g.Bz, g.Z_inner, g.Zmax, 3, flip);
+    if (g.DimY) bc_axis(sj < g.By ? by0 : by1, sj, g.By, g.Y_inner, g.Ymax, 2, flip);
+    if (g.DimX) bc_axis(si < g.Bx ? bx0 : bx1, si, g.Bx, g.X_inner, g.Xmax, 1, flip);
+    if (si == i && sj == j && sk == k) return; // Inflow along every ghost axis: untouched
+    const long long N = g.N;
+    const long long id = (long long)g.Xmax * g.Ymax * k + (long long)g.Xmax * j + i;
+    const long long sid = (long long)g.Xmax * g.Ymax * sk + (long long)g.Xmax * sj + si;
+    for (int n = 0; n < E; n++) {
+        const double v = U[n * N + sid];
+        U[n * N + id] = ((flip >> n) & 1) ? -v : v;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -178,13 +219,16 @@ __global__ void __launch_bounds__(256) k_states(Geom g, const DevThermo *__restr
         }
         const double H = (U[4] + p) * rho1;
         const double c = sqrt(gamma * p * rho1);
-        f.rho[id] = rho;
-        f.u[id] = u;
-        f.v[id] = v;
-        f.w[id] = w;
+        // the sweeps rebuild u, v, w and H from U and 1/rho with the same
+        // operations, so the time-step path stores only p (and the COP
+        // ingredients above); the full set is written for the stage seam
         f.p[id] = p;
-        f.H[id] = H;
         if (full) {
+            f.rho[id] = rho;
+            f.u[id] = u;
+            f.v[id] = v;
+            f.w[id] = w;
+            f.H[id] = H;
             f.c[id] = c;
             f.gamma[id] = gamma;
             f.e[id] = tme;

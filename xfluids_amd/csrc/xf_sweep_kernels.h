@@ -189,10 +189,16 @@ namespace xf {
 
 // GLF dissipation of characteristic field n (Artificial_type 3): the running
 // maximum of |eigen_local[n]|; fields 1..E-2 share the |u_n| maximum.
+// GetLU's GLF reduction happens here: the running maximum is folded with
+// the maximum of the last UpdateStates pass.  Every block computes the same
+// max(running, pass) and block 0 stores it back; readers before or after the
+// store see the same value (max is idempotent).
 template <int DIR, int E>
 __device__ __forceinline__ double glf_alpha(const double *red, int n)
 {
-    return red[RED_GLF + 3 * DIR + (n == 0 ? 0 : (n == E - 1 ? 2 : 1))];
+    const int q = 3 * DIR + (n == 0 ? 0 : (n == E - 1 ? 2 : 1));
+    const double a = red[RED_GLF + q], b = red[RED_GLFP + q];
+    return b > a ? b : a;
 }
 
 template <int NS, bool COP, int DIR, int SCHEME>
@@ -206,6 +212,7 @@ __global__ void __launch_bounds__(256) k_sweep(Geom g, const DevThermo *__restri
     constexpr int TS = SS::TS, TW = SS::TW, NC = SS::NC;
     __shared__ double sU[E * NC];
     __shared__ double sF[E * NC];
+    __shared__ double sP[NC], sR1[NC];
 
     const long long N = g.N;
     const int tid = threadIdx.x;
@@ -248,9 +255,13 @@ __global__ void __launch_bounds__(256) k_sweep(Geom g, const DevThermo *__restri
             double U[E];
 #pragma unroll
             for (int n = 0; n < E; n++) U[n] = Ug[n * N + id];
-            const double vel[3] = {f.u[id], f.v[id], f.w[id]};
+            // velocities exactly as GetStates: u = U1 * (1 / rho)
+            const double rho1 = 1.0 / U[0];
+            const double vel[3] = {U[1] * rho1, U[2] * rho1, U[3] * rho1};
             const double p = f.p[id];
             const double Um = U[1 + DIR];
+            sP[li] = p;
+            sR1[li] = rho1;
             sU[0 * NC + li] = U[0];
             sF[0 * NC + li] = Um;
 #pragma unroll
@@ -261,7 +272,6 @@ __global__ void __launch_bounds__(256) k_sweep(Geom g, const DevThermo *__restri
             sU[4 * NC + li] = U[4];
             sF[4 * NC + li] = (U[4] + p) * vel[DIR];
             if (COP) {
-                const double rho1 = 1.0 / U[0];
 #pragma unroll
                 for (int ii = 5; ii < E; ii++) {
                     sU[ii * NC + li] = U[ii];
@@ -271,9 +281,16 @@ __global__ void __launch_bounds__(256) k_sweep(Geom g, const DevThermo *__restri
         } else {
 #pragma unroll
             for (int n = 0; n < E; n++) sU[n * NC + li] = 0.0, sF[n * NC + li] = 0.0;
+            sP[li] = 0.0;
+            sR1[li] = 0.0;
         }
     }
     __syncthreads();
+
+    if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && tid < 3) {
+        const double a = f.red[RED_GLF + 3 * DIR + tid], b = f.red[RED_GLFP + 3 * DIR + tid];
+        f.red[RED_GLF + 3 * DIR + tid] = b > a ? b : a;
+    }
 
     // ---- one characteristic face flux per lane ------------------------------
     bool okp;
@@ -288,9 +305,13 @@ __global__ void __launch_bounds__(256) k_sweep(Geom g, const DevThermo *__restri
         const long long id_r = id_l + stride;
         const int li_l = lds_ix<DIR, TS, TW>(fidx + 2, tcol);
         const int li_r = lds_ix<DIR, TS, TW>(fidx + 3, tcol);
+        // primitives of the two cells, rebuilt as GetStates forms them
         const double rl = sU[li_l], rr = sU[li_r];
-        const double ul = f.u[id_l], vl = f.v[id_l], wl = f.w[id_l], pl = f.p[id_l], Hl = f.H[id_l];
-        const double ur = f.u[id_r], vr = f.v[id_r], wr = f.w[id_r], pr = f.p[id_r], Hr = f.H[id_r];
+        const double r1l = sR1[li_l], r1r = sR1[li_r];
+        const double pl = sP[li_l], pr = sP[li_r];
+        const double ul = sU[NC + li_l] * r1l, vl = sU[2 * NC + li_l] * r1l, wl = sU[3 * NC + li_l] * r1l;
+        const double ur = sU[NC + li_r] * r1r, vr = sU[2 * NC + li_r] * r1r, wr = sU[3 * NC + li_r] * r1r;
+        const double Hl = (sU[4 * NC + li_l] + pl) * r1l, Hr = (sU[4 * NC + li_r] + pr) * r1r;
         // MARCO_ROE
         const double D = sqrt(rr / rl);
         const double D1 = 1.0 / (D + 1.0);
@@ -303,7 +324,6 @@ __global__ void __launch_bounds__(256) k_sweep(Geom g, const DevThermo *__restri
         rf._rho = sqrt(rr * rl);
         if (COP) {
             double yl[COP ? NS : 1], yr[COP ? NS : 1], hl[COP ? NS : 1], hr[COP ? NS : 1];
-            const double r1l = 1.0 / rl, r1r = 1.0 / rr;
             yl[NCOP] = 1.0;
             yr[NCOP] = 1.0;
 #pragma unroll
@@ -332,84 +352,65 @@ __global__ void __launch_bounds__(256) k_sweep(Geom g, const DevThermo *__restri
         const double un = vr3[DIR];
         const double un_c = un * _c1;
 
-#pragma unroll
-        for (int n = 0; n < E; n++) {
-            const double av = glf_alpha<DIR, E>(f.red, n);
-            double uf[7], ff[7];
-            const bool dense = (n == 0) || (n == E - 1) || (n == 1 + DIR);
-            if (dense) {
-                double L[E];
-                if (n == 0) {
-                    L[0] = 0.5 * (b2 + un_c + b3);
-#pragma unroll
-                    for (int k = 0; k < 3; k++)
-                        L[1 + k] = (k == DIR) ? -0.5 * (b1 * vr3[k] + _c1) : -0.5 * (b1 * vr3[k]);
-                    L[4] = 0.5 * b1;
-#pragma unroll
-                    for (int m = 0; m < NCOP; m++) L[5 + m] = -0.5 * b1 * rf.z[m];
-                } else if (n == E - 1) {
-                    L[0] = 0.5 * (b2 - un_c + b3);
-#pragma unroll
-                    for (int k = 0; k < 3; k++)
-                        L[1 + k] = (k == DIR) ? 0.5 * (-b1 * vr3[k] + _c1) : 0.5 * (-b1 * vr3[k]);
-                    L[4] = 0.5 * b1;
-#pragma unroll
-                    for (int m = 0; m < NCOP; m++) L[5 + m] = -0.5 * b1 * rf.z[m];
-                } else {
-                    L[0] = (1.0 - b2 - b3) / b1;
-                    L[1] = _u;
-                    L[2] = _v;
-                    L[3] = _w;
-                    L[4] = -1.0;
-#pragma unroll
-                    for (int m = 0; m < NCOP; m++) L[5 + m] = rf.z[m];
-                }
-#pragma unroll
-                for (int m = 1; m <= 6; m++) {
-                    const int li = lds_ix<DIR, TS, TW>(fidx + m - 1, tcol);
-                    double a = 0.0, b = 0.0;
-#pragma unroll
-                    for (int n1 = 0; n1 < E; n1++) {
-                        a = a + sU[n1 * NC + li] * L[n1];
-                        b = b + sF[n1 * NC + li] * L[n1];
-                    }
-                    uf[m] = a;
-                    ff[m] = b;
-                }
-            } else {
-                // two structural non-zeros: L[0] and +-1 at component kk
-                double L0;
-                int kk;
-                bool neg;
-                if (n >= 1 && n <= 3) {
-                    kk = n;
-                    if (DIR == 0) { L0 = (n == 2) ? _v : -_w; neg = (n == 2); }
-                    else if (DIR == 1) { L0 = (n == 1) ? -_u : _w; neg = (n == 3); }
-                    else { L0 = (n == 1) ? _u : -_v; neg = (n == 1); }
-                } else {
-                    kk = 5 + (n - 4);
-                    L0 = -rf._yi[n - 4];
-                    neg = false;
-                }
-#pragma unroll
-                for (int m = 1; m <= 6; m++) {
-                    const int li = lds_ix<DIR, TS, TW>(fidx + m - 1, tcol);
-                    const double a0 = 0.0 + sU[li] * L0;
-                    const double b0 = 0.0 + sF[li] * L0;
-                    const double au = sU[kk * NC + li], bf = sF[kk * NC + li];
-                    uf[m] = neg ? a0 + (-au) : a0 + au;
-                    ff[m] = neg ? b0 + (-bf) : b0 + bf;
-                }
-            }
+        // GLF split of the six projected points and the WENO5 face value
+        auto split_weno = [&](double av, const double *uf, const double *ff) -> double {
             double pp[7], mm[7];
 #pragma unroll
             for (int m = 1; m <= 5; m++) pp[m] = 0.5 * (ff[m] + av * uf[m]);
 #pragma unroll
             for (int m = 2; m <= 6; m++) mm[m] = 0.5 * (ff[m] - av * uf[m]);
-            const double fl = weno_face<SCHEME>(pp, mm);
+            return weno_face<SCHEME>(pp, mm);
+        };
+        // dense rows: n = 0 (u-c), n = 1 + DIR (normal), n = E - 1 (u+c);
+        // called with literal n, so every branch on n folds at compile time
+        auto dense_row = [&](const int n) {
+            double L[E];
+            if (n == 0) {
+                L[0] = 0.5 * (b2 + un_c + b3);
+#pragma unroll
+                for (int k = 0; k < 3; k++)
+                    L[1 + k] = (k == DIR) ? -0.5 * (b1 * vr3[k] + _c1) : -0.5 * (b1 * vr3[k]);
+                L[4] = 0.5 * b1;
+#pragma unroll
+                for (int m = 0; m < NCOP; m++) L[5 + m] = -0.5 * b1 * rf.z[m];
+            } else if (n == E - 1) {
+                L[0] = 0.5 * (b2 - un_c + b3);
+#pragma unroll
+                for (int k = 0; k < 3; k++)
+                    L[1 + k] = (k == DIR) ? 0.5 * (-b1 * vr3[k] + _c1) : 0.5 * (-b1 * vr3[k]);
+                L[4] = 0.5 * b1;
+#pragma unroll
+                for (int m = 0; m < NCOP; m++) L[5 + m] = -0.5 * b1 * rf.z[m];
+            } else {
+                L[0] = (1.0 - b2 - b3) / b1;
+                L[1] = _u;
+                L[2] = _v;
+                L[3] = _w;
+                L[4] = -1.0;
+#pragma unroll
+                for (int m = 0; m < NCOP; m++) L[5 + m] = rf.z[m];
+            }
+            // projections, component-outer: every accumulator still sums
+            // over n1 in increasing order; the empty asm is a compiler-only
+            // fence that keeps the LDS loads of one component from being
+            // hoisted above the previous one (register pressure)
+            double uf[7], ff[7];
+#pragma unroll
+            for (int m = 1; m <= 6; m++) uf[m] = 0.0, ff[m] = 0.0;
+#pragma unroll
+            for (int n1 = 0; n1 < E; n1++) {
+                const double Ln = L[n1];
+#pragma unroll
+                for (int m = 1; m <= 6; m++) {
+                    const int li = lds_ix<DIR, TS, TW>(fidx + m - 1, tcol);
+                    uf[m] = uf[m] + sU[n1 * NC + li] * Ln;
+                    ff[m] = ff[m] + sF[n1 * NC + li] * Ln;
+                }
+                asm volatile("" ::: "memory");
+            }
+            const double fl = split_weno(glf_alpha<DIR, E>(f.red, n), uf, ff);
             // right column n, accumulated in the reference's summation order
             if (n == 0 || n == E - 1) {
-                const double sg = (n == 0) ? -1.0 : 1.0;
                 flux[0] += fl * 1.0;
 #pragma unroll
                 for (int k = 0; k < 3; k++) {
@@ -419,8 +420,7 @@ __global__ void __launch_bounds__(256) k_sweep(Geom g, const DevThermo *__restri
                 flux[4] += fl * (n == 0 ? _H - un * _c : _H + un * _c);
 #pragma unroll
                 for (int m = 0; m < NCOP; m++) flux[5 + m] += fl * rf._yi[m];
-                (void)sg;
-            } else if (n == 1 + DIR) {
+            } else {
                 flux[0] += fl * b1;
                 flux[1] += fl * (_u * b1);
                 flux[2] += fl * (_v * b1);
@@ -428,21 +428,59 @@ __global__ void __launch_bounds__(256) k_sweep(Geom g, const DevThermo *__restri
                 flux[4] += fl * (_H * b1 - 1.0);
 #pragma unroll
                 for (int m = 0; m < NCOP; m++) flux[5 + m] += fl * (b1 * rf._yi[m]);
-            } else if (n >= 1 && n <= 3) {
-                // (component, sign) and the energy entry
-                double R4;
-                int kk;
-                bool neg;
-                if (DIR == 0) { kk = n; neg = (n == 2); R4 = (n == 2) ? -_v : _w; }
-                else if (DIR == 1) { kk = n; neg = (n == 3); R4 = (n == 1) ? _u : -_w; }
-                else { kk = n; neg = (n == 1); R4 = (n == 1) ? -_u : _v; }
-                flux[kk] += neg ? -fl : fl;
-                flux[4] += fl * R4;
-            } else {
-                flux[4] += fl * rf.z[n - 4];
-                flux[5 + (n - 4)] += fl;
             }
+        };
+
+        dense_row(0);
+        // the E-3 rows with two structural non-zeros (L[0] and +-1 at kk; the
+        // right column has +-1 at kk and an energy entry R4), in a runtime
+        // loop; contributions go to flux[kk] through a masked add (+0.0 on
+        // the other components is exact: no accumulator is ever -0.0)
+#pragma unroll 1
+        for (int n = 1; n <= E - 2; n++) {
+            if (n == 1 + DIR) {
+                dense_row(1 + DIR);
+                continue;
+            }
+            double L0, R4;
+            int kk;
+            bool neg;
+            if (n <= 3) {
+                kk = n;
+                if (DIR == 0) { L0 = (n == 2) ? _v : -_w; neg = (n == 2); R4 = (n == 2) ? -_v : _w; }
+                else if (DIR == 1) { L0 = (n == 1) ? -_u : _w; neg = (n == 3); R4 = (n == 1) ? _u : -_w; }
+                else { L0 = (n == 1) ? _u : -_v; neg = (n == 1); R4 = (n == 1) ? -_u : _v; }
+            } else {
+                const int m = n - 4;
+                double ym = 0.0, zm = 0.0;
+#pragma unroll
+                for (int q = 0; q < NCOP; q++) {
+                    ym = (q == m) ? rf._yi[q] : ym;
+                    zm = (q == m) ? rf.z[q] : zm;
+                }
+                kk = 5 + m;
+                L0 = -ym;
+                R4 = zm;
+                neg = false;
+            }
+            double uf[7], ff[7];
+#pragma unroll
+            for (int m = 1; m <= 6; m++) {
+                const int li = lds_ix<DIR, TS, TW>(fidx + m - 1, tcol);
+                const double a0 = 0.0 + sU[li] * L0;
+                const double b0 = 0.0 + sF[li] * L0;
+                const double au = sU[kk * NC + li], bf = sF[kk * NC + li];
+                uf[m] = neg ? a0 + (-au) : a0 + au;
+                ff[m] = neg ? b0 + (-bf) : b0 + bf;
+            }
+            const double fl = split_weno(glf_alpha<DIR, E>(f.red, n), uf, ff);
+            const double t = neg ? -fl : fl;
+#pragma unroll
+            for (int k = 1; k < E; k++)
+                if (k != 4) flux[k] += (k == kk) ? t : 0.0;
+            flux[4] += fl * R4;
         }
+        dense_row(E - 1);
     }
 
     // ---- flux divergence on the TS-1 cells of the tile ----------------------

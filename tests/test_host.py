@@ -1,0 +1,132 @@
+"""Host-side checks that need no GPU: the C-ABI library loads and exports
+every function include/*.h declares, the settings reader, and the runtime.dat
+subset this package ships.
+
+When the upstream tree is present (build container only), its settings files
+and runtime.dat tables are read as data and compared; elsewhere those tests
+skip.
+"""
+import glob
+import json
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import xfluids_amd as xf
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+UPSTREAM = "/root/reference"
+
+
+def header_functions():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        names |= set(re.findall(r"\b(xf_\w+)\s*\(", text))
+    return names
+
+
+def test_library_exports_every_declared_function():
+    assert os.path.exists(xf.LIB_PATH), "build the HIP library first"
+    out = subprocess.run(["nm", "-D", "--defined-only", xf.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    exported = {ln.split()[-1] for ln in out.splitlines() if " T " in ln}
+    declared = header_functions()
+    assert len(declared) >= 40
+    assert declared <= exported, sorted(declared - exported)
+    # the Python binding declares a signature for each of them
+    assert declared <= set(xf.exported_symbols()), sorted(declared - set(xf.exported_symbols()))
+    xf.lib()  # loads without a GPU; nothing is called
+
+
+def test_error_strings_and_invalid_arguments():
+    L = xf.lib()
+    for code in (0, 1, 2, 3, 5, -1, -2, -3):
+        assert L.xf_error_string(code)
+    with pytest.raises(Exception):
+        xf.Setup(os.path.join(ROOT, "no-such-file.json"))
+
+
+def test_json_comments_cut_the_line_at_the_first_slash(tmp_path):
+    # settings lines are cut at their first '/', so a path-valued string has
+    # to be written as an array of path components
+    p = tmp_path / "c.json"
+    p.write_text('{\n "run": {"CFLnumber": 0.3}, // trailing comment\n'
+                 ' "equations": {"Mixture": ["Reaction", "H2O_19_reaction"]},\n'
+                 ' "mesh": {"DOMAIN_Size": [1.0, 0, 0], "Resolution": [64, 0, 0],\n'
+                 '          "Boundarys": [1, 1, 2, 2, 2, 2]}  // another\n}\n')
+    s = xf.Setup(str(p))
+    d = s.desc
+    assert abs(d.CFLnumber - 0.3) < 1e-15
+    assert s.species == ["H2", "O2", "O", "H", "OH", "HO2", "H2O2", "H2O", "AR"]
+    assert d.Xmax == 64 + 2 * d.Bwidth_X and d.Ymax == 1 and d.Zmax == 1
+    assert d.cop == 1 and d.NS == 9 and d.NR == 19
+
+
+def test_case_files_load():
+    for f in sorted(glob.glob(os.path.join(xf.CASES, "*.json"))):
+        s = xf.Setup(f)
+        d = s.desc
+        for dim, n, B, m in ((d.DimX, d.X_inner, d.Bwidth_X, d.Xmax), (d.DimY, d.Y_inner, d.Bwidth_Y, d.Ymax),
+                             (d.DimZ, d.Z_inner, d.Bwidth_Z, d.Zmax)):
+            assert m == (n + 2 * B if dim else 1)
+
+
+@pytest.mark.skipif(not os.path.isdir(UPSTREAM), reason="upstream tree not present")
+def test_upstream_settings_geometry():
+    files = sorted(glob.glob(os.path.join(UPSTREAM, "settings", "*.json")))
+    assert files
+    for f in files:
+        txt = "\n".join(line.split("/")[0] for line in open(f).read().splitlines())
+        js = json.loads(re.sub(r",(\s*[}\]])", r"\1", txt))
+        res = js["mesh"]["Resolution"]
+        size = js["mesh"].get("DOMAIN_Size", [1.0, 1.0, 1.0])  # default of read_json.cpp:107
+        s = xf.Setup(f)
+        d = s.desc
+        assert list(d.BC) == js["mesh"]["Boundarys"], f
+        assert abs(d.CFLnumber - js["run"]["CFLnumber"]) < 1e-15
+        for k, (dim, inner, mx, h) in enumerate(((d.DimX, d.X_inner, d.Xmax, d.dx), (d.DimY, d.Y_inner, d.Ymax, d.dy),
+                                                 (d.DimZ, d.Z_inner, d.Zmax, d.dz))):
+            if res[k]:
+                assert dim and inner == res[k] and mx == res[k] + 8, f
+                assert abs(h - size[k] / res[k]) <= 1e-15 * h, f
+            else:
+                assert not dim and mx == 1, f
+
+
+def _blocks(path):
+    toks = open(path).read().split()
+    out = {}
+    for i, t in enumerate(toks):
+        if t.startswith("*") and t != "*END":
+            vals = []
+            for v in toks[i + 1:]:
+                if v.startswith("*"):
+                    break
+                try:
+                    vals.append(float(v))
+                except ValueError:
+                    pass
+            out[t] = vals
+    return out
+
+
+@pytest.mark.skipif(not os.path.isdir(UPSTREAM), reason="upstream tree not present")
+@pytest.mark.parametrize("fname", ["thermal_dynamics.dat", "thermal_dynamics_janaf.dat"])
+def test_runtime_subset_matches_upstream_tables(fname):
+    ours = _blocks(os.path.join(xf.RUNTIME_DAT, fname))
+    theirs = _blocks(os.path.join(UPSTREAM, "runtime.dat", fname))
+    assert len(ours) >= 9
+    for k, v in ours.items():
+        assert np.array_equal(np.array(v), np.array(theirs[k][:len(v)])), k
+
+
+@pytest.mark.skipif(not os.path.isdir(UPSTREAM), reason="upstream tree not present")
+@pytest.mark.parametrize("mech", ["H2O_19_reaction", "H2O-N2_19_reaction"])
+def test_runtime_subset_mechanisms_match_upstream(mech):
+    a = open(os.path.join(xf.RUNTIME_DAT, "Reaction", mech, "reaction_list.dat")).read()
+    b = open(os.path.join(UPSTREAM, "runtime.dat", "Reaction", mech, "reaction_list.dat")).read()
+    assert a == b

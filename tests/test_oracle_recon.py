@@ -1,0 +1,121 @@
+"""Pins of the oracle's reconstruction pieces.
+
+* WENO5-JS / WENO5-Z: on linear data all smoothness indicators are equal, the
+  nonlinear weights reduce to the ideal (0.1, 0.6, 0.3) and the face value is
+  the linear fifth-order upwind reconstruction (2 -13 47 27 -3)/60, which is
+  exact for linear data; across a step the value stays within the data range;
+  on random data they equal the published schemes (Jiang & Shu 1996 weights
+  d_k / (eps + beta_k)^2; Borges et al. 2008 weights d_k (1 + tau5 / (beta_k +
+  eps)) with q = 1), written with the smoothness indicators scaled by 12 and
+  eps = 1e-6 in those units, as WENO5s_schemes.hpp:12-69 and :246-316 state them.
+* Roe eigen-system: the left rows and right columns the oracle builds are
+  inverse matrices (L R = R L = I) for x, y, z, single species and Ns = 3, 9.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle_lib import OraConfig, lib
+
+DP = C.POINTER(C.c_double)
+
+
+def weno(fn, pp, mm):
+    a = np.ascontiguousarray(pp, dtype=np.float64)
+    b = np.ascontiguousarray(mm, dtype=np.float64)
+    return fn(a.ctypes.data_as(DP), b.ctypes.data_as(DP))
+
+
+@pytest.mark.parametrize("scheme", ["js", "z"])
+def test_weno_linear_data_is_ideal_upwind(scheme):
+    L = lib()
+    fn = L.ora_weno5js if scheme == "js" else L.ora_weno5z
+    rng = np.random.default_rng(3)
+    for _ in range(200):
+        a, b = rng.normal() * 10, rng.normal() * 3
+        v = a + b * np.arange(-2, 6, dtype=np.float64)      # points i-3 .. i+4 in pp[0..7]
+        got = weno(fn, v, np.zeros(8))
+        v1, v2, v3, v4, v5 = v[1:6]
+        ideal = (2 * v1 - 13 * v2 + 47 * v3 + 27 * v4 - 3 * v5) / 60.0
+        exact = a + b * (1.5)                                # f at i + 1/2 (v3 is at index 1)
+        assert abs(got - ideal) <= 1e-12 * max(1.0, abs(ideal))
+        assert abs(got - exact) <= 1e-12 * max(1.0, abs(exact))
+        # the minus side reads mm[6..2] mirrored
+        got_m = weno(fn, np.zeros(8), v[::-1].copy())
+        w = v[::-1]
+        ideal_m = (2 * w[6] - 13 * w[5] + 47 * w[4] + 27 * w[3] - 3 * w[2]) / 60.0
+        assert abs(got_m - ideal_m) <= 1e-12 * max(1.0, abs(ideal_m))
+
+
+def _textbook(v, scheme):
+    v1, v2, v3, v4, v5 = v
+    b = 12.0 * np.array([13 / 12 * (v1 - 2 * v2 + v3) ** 2 + 1 / 4 * (v1 - 4 * v2 + 3 * v3) ** 2,
+                         13 / 12 * (v2 - 2 * v3 + v4) ** 2 + 1 / 4 * (v2 - v4) ** 2,
+                         13 / 12 * (v3 - 2 * v4 + v5) ** 2 + 1 / 4 * (3 * v3 - 4 * v4 + v5) ** 2])
+    d = np.array([0.1, 0.6, 0.3])
+    if scheme == "js":
+        a = d / (b + 1e-6) ** 2
+    else:
+        a = d * (1 + abs(b[0] - b[2]) / (b + 1e-6))
+    w = a / a.sum()
+    q = np.array([2 * v1 - 7 * v2 + 11 * v3, -v2 + 5 * v3 + 2 * v4, 2 * v3 + 5 * v4 - v5]) / 6
+    return float(w @ q)
+
+
+@pytest.mark.parametrize("scheme", ["js", "z"])
+def test_weno_random_data_matches_published_scheme(scheme):
+    L = lib()
+    fn = L.ora_weno5js if scheme == "js" else L.ora_weno5z
+    rng = np.random.default_rng(11)
+    for k in range(500):
+        amp = 10.0 ** rng.uniform(-4, 2)
+        pp = rng.normal(size=8) * amp
+        mm = rng.normal(size=8) * amp
+        want = _textbook(pp[1:6], scheme) + _textbook(mm[6:1:-1], scheme)
+        got = weno(fn, pp, mm)
+        assert abs(got - want) <= 1e-12 * (np.abs(pp).max() + np.abs(mm).max()), (k, got, want)
+
+
+@pytest.mark.parametrize("scheme", ["js", "z"])
+def test_weno_step_is_bounded(scheme):
+    L = lib()
+    fn = L.ora_weno5js if scheme == "js" else L.ora_weno5z
+    for jump in range(1, 7):
+        v = np.where(np.arange(8) < jump, 1.0, 0.0)
+        got = weno(fn, v, np.zeros(8))
+        assert -1e-3 <= got <= 1.0 + 1e-3
+
+
+@pytest.mark.parametrize("cop,NS", [(0, 1), (1, 3), (1, 9)])
+@pytest.mark.parametrize("d", [0, 1, 2])
+def test_eigenvectors_are_inverse(cop, NS, d):
+    L = lib()
+    cfg = OraConfig()
+    cfg.cop, cfg.NS = cop, NS
+    E = NS + 4 if cop else 5
+    rng = np.random.default_rng(10 * NS + d)
+    for _ in range(20):
+        rho = rng.uniform(0.1, 5.0)
+        u, v, w = rng.normal(size=3) * 200
+        c2 = rng.uniform(200.0, 1200.0) ** 2
+        gam = rng.uniform(1.1, 1.67)
+        z = np.zeros(16)
+        yi = np.zeros(16)
+        if cop:
+            z[:NS - 1] = rng.normal(size=NS - 1) * 1e5
+            yi[:NS - 1] = rng.dirichlet(np.ones(NS))[:NS - 1]
+        b1 = (gam - 1) / c2
+        b3 = b1 * float(np.dot(yi[:NS - 1], z[:NS - 1])) if cop else 0.0
+        H = rng.uniform(1e4, 1e6)
+        Lm, Rm = np.zeros((E, E)), np.zeros((E, E))
+        lrow, rcol, ev = np.zeros(20), np.zeros(20), np.zeros(1)
+        for n in range(E):
+            L.ora_eigen_lr(C.byref(cfg), d, n, lrow.ctypes.data_as(DP), rcol.ctypes.data_as(DP),
+                           ev.ctypes.data_as(DP), z.ctypes.data_as(DP), yi.ctypes.data_as(DP), c2, rho, u, v, w, H,
+                           b1, b3)
+            Lm[n] = lrow[:E]
+            Rm[:, n] = rcol[:E]
+        scale = np.abs(Lm).max() * np.abs(Rm).max()
+        assert np.abs(Lm @ Rm - np.eye(E)).max() <= 1e-13 * scale
+        assert np.abs(Rm @ Lm - np.eye(E)).max() <= 1e-13 * scale

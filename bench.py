@@ -1,6 +1,6 @@
 """Throughput of the XFluids per-step hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--form plain|mixed]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--copy-probe]
 
 Workload: 2D inert shock-bubble interaction, 3 species (H2, O2, N2) with
 NASA-7 (JANAF) thermodynamics, 1024 x 512 cells, WENO5-Z characteristic
@@ -29,19 +29,35 @@ NX, NY = 1024, 512
 # Compulsory HBM bytes per cell-update of one full step for Ns = 3, E = 7:
 # 8 * (8E + 6) (stage traffic of U, U1 and the T guess).
 ALG_BYTES_PER_CELL = 8 * (8 * 7 + 6)
+# Algorithmic FP64 operations (add/sub, mul, div, sqrt) of this workload,
+# counted on the operation-counting build of the CPU restatement
+# (`python oracle/opcount/count_flops.py --case shock-bubble`): per
+# characteristic face flux (x and y alike).  The count includes work the
+# reference performs whose result is unused (terms multiplied by zero,
+# projections of the stencil points i-3 and i+4); the HIP sweep skips it.
+FACE_FLOPS = 4121
+PROFILES = os.path.join(ROOT, "profiles")
+TRAFFIC_FILE = os.path.join(PROFILES, "pmc-traffic-shock-bubble-1024x512.md")
 
 
-def face_flops(E, ncop, scheme):
-    """FP64 operations per characteristic face flux of the plain algorithm
-    (counted per operation class; see DESIGN.md, "Algorithmic flops")."""
-    from xfluids_amd.opcount import face_flop_count
-    return face_flop_count(E, ncop, scheme)
+def measured_traffic():
+    """HBM bytes per launch per kernel class from the committed rocprofv3
+    counter summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
+    factor), or {} when none is committed."""
+    out = {}
+    if os.path.exists(TRAFFIC_FILE):
+        for ln in open(TRAFFIC_FILE):
+            ln = ln.strip()
+            if ln.startswith("traffic[") and "=" in ln:
+                k, v = ln.split("=", 1)
+                out[k.strip()[8:-1]] = float(v.split()[0])
+    return out
 
 
 def run(args):
     s = xf.Setup(xf.case_path(CASE), nx=NX, ny=NY)
     d = s.desc
-    precision = xf.XF_MIXED if args.form == "mixed" else xf.XF_PLAIN_FP64
+    precision = xf.XF_PLAIN_FP64
     U, T = s.initial(CASE, seed=args.seed)
     ctx = s.context(precision=precision)
     ctx.upload(U, T)
@@ -72,20 +88,31 @@ def roofline(d, kt):
     # average launch duration is also the largest share of the step
     name = max(kt, key=lambda k: kt[k][0])
     avg_ms, cnt = kt[name]
-    E, ncop = d.E, (d.NS - 1 if d.cop else 0)
+    traffic = measured_traffic().get(name)
     if name.startswith("sweep"):
         faces = {"sweep_x": (d.X_inner + 1) * d.Y_inner * d.Z_inner,
                  "sweep_y": d.X_inner * (d.Y_inner + 1) * d.Z_inner,
                  "sweep_z": d.X_inner * d.Y_inner * (d.Z_inner + 1)}[name]
-        flops = faces * face_flops(E, ncop, d.scheme)
+        flops = faces * FACE_FLOPS
         achieved = flops / (avg_ms * 1e-3) / 1e12
         return {"kernel": name, "bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
-                "avg_launch_ms": avg_ms, "timed_launches": cnt, "alg_flops_per_launch": flops}
-    bytes_ = d.Xmax * d.Ymax * d.Zmax * 8 * (E + 10)
+                "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
+                "avg_launch_ms": avg_ms, "timed_launches": cnt, "alg_flops_per_launch": flops,
+                "faces_per_launch": faces, "flops_per_face": FACE_FLOPS}
+    bytes_ = d.Xmax * d.Ymax * d.Zmax * 8 * (d.E + 10)
     achieved = bytes_ / (avg_ms * 1e-3) / 1e9
     return {"kernel": name, "bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-            "frac": achieved / PEAK_HBM_GBS, "traffic": None, "avg_launch_ms": avg_ms, "timed_launches": cnt}
+            "frac": achieved / PEAK_HBM_GBS, "traffic": traffic, "avg_launch_ms": avg_ms,
+            "timed_launches": cnt}
+
+
+def copy_probe(nbytes=1 << 30, reps=5):
+    """A plain device copy of known size (the library's streaming copy
+    kernel): HBM copy bandwidth, and the scale for the FETCH_SIZE and
+    WRITE_SIZE counters when run under rocprofv3 --pmc."""
+    ms = xf.copy_probe(nbytes, reps)
+    return {"bytes_read": nbytes, "bytes_written": nbytes, "reps": reps, "avg_ms": ms,
+            "GB_per_s": 2 * nbytes / (ms * 1e-3) / 1e9}
 
 
 def accuracy_check():
@@ -138,10 +165,13 @@ def cpu_baseline(sample_steps):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--kernel-reps", type=int, default=60)
-    ap.add_argument("--form", choices=["plain", "mixed"], default="plain")
+    ap.add_argument("--form", choices=["plain"], default="plain")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the reduced-grid check against the oracle")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run a 1 GiB device copy (counter scaling under rocprofv3 --pmc)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
@@ -168,7 +198,10 @@ def main():
         "roofline": roofline(d, kt),
         "kernels_ms": {k: v[0] for k, v in kt.items()},
     }
-    line["accuracy"] = accuracy_check()
+    if args.copy_probe:
+        line["copy_probe"] = copy_probe()
+    if not args.no_accuracy:
+        line["accuracy"] = accuracy_check()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.cpu_steps)
     print(json.dumps(line))

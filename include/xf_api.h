@@ -183,6 +183,11 @@ int xf_time_kernel(xf_ctx *ctx, const char *name, int reps, double *avg_ms);
 /* Evaluate the device exp and log (the fixed-sequence FP64 functions every
    kernel uses) on n host values; for bit-level checks against the CPU. */
 int xf_debug_math(const double *x, double *exp_out, double *log_out, int n);
+/* Streaming copy of nbytes (a multiple of 16) between two fresh device
+   buffers, reps times after one warm-up launch on the null stream; average
+   ms per launch.  A plain copy of known size: the HBM bandwidth reference
+   and the scale for the memory-traffic counters.  No reference counterpart. */
+int xf_copy_probe(size_t nbytes, int reps, double *avg_ms);
 
 #ifdef __cplusplus
 }

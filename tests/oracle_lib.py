@@ -38,38 +38,43 @@ _lib = None
 _DP = C.POINTER(C.c_double)
 
 
+def load(path):
+    """ctypes handle of an oracle build (the plain or the counting library)."""
+    if not os.path.exists(path):
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
+    L = C.CDLL(path)
+    v = C.c_void_p
+    P = C.POINTER(OraConfig)
+    sig = {
+        "ora_create": (v, [P]), "ora_destroy": (None, [v]), "ora_num_eqn": (C.c_int, [v]),
+        "ora_num_cells": (C.c_size_t, [v]), "ora_field": (_DP, [v, C.c_char_p]),
+        "ora_boundary": (None, [v, C.c_int]), "ora_update_states": (C.c_int, [v, C.c_int]),
+        "ora_compute_lu": (None, [v, C.c_int]), "ora_rk3": (None, [v, C.c_int, C.c_double]),
+        "ora_get_dt": (C.c_double, [v]), "ora_chemistry": (None, [v, C.c_double]),
+        "ora_reaction": (C.c_int, [v, C.c_double]), "ora_estimate_nan": (C.c_int, [v, C.c_int]),
+        "ora_initialize": (C.c_int, [v]), "ora_steps": (C.c_int, [v, C.c_int, _DP]),
+        "ora_exp": (C.c_double, [C.c_double]), "ora_log": (C.c_double, [C.c_double]),
+        "ora_cp_i": (C.c_double, [P, C.c_double, C.c_int]), "ora_h_i": (C.c_double, [P, C.c_double, C.c_int]),
+        "ora_s_i": (C.c_double, [P, C.c_double, C.c_int]), "ora_g_i": (C.c_double, [P, C.c_double, C.c_int]),
+        "ora_e_i": (C.c_double, [P, C.c_double, C.c_int]),
+        "ora_mix_cp": (C.c_double, [P, _DP, C.c_double]), "ora_mix_cv": (C.c_double, [P, _DP, C.c_double]),
+        "ora_mix_h": (C.c_double, [P, _DP, C.c_double]), "ora_mix_e": (C.c_double, [P, _DP, C.c_double]),
+        "ora_mix_R": (C.c_double, [P, _DP]), "ora_get_T": (C.c_double, [P, _DP, C.c_double, C.c_double]),
+        "ora_weno5js": (C.c_double, [_DP, _DP]), "ora_weno5z": (C.c_double, [_DP, _DP]),
+        "ora_eigen_lr": (None, [P, C.c_int, C.c_int, _DP, _DP, _DP, _DP, _DP] + [C.c_double] * 8),
+        "ora_evalcv": (None, [P, _DP, _DP, _DP, C.c_double, C.c_double]),
+        "ora_chemeq2": (C.c_int, [P, _DP, C.c_double, C.c_double, C.c_double]),
+    }
+    for n, (r, a) in sig.items():
+        f = getattr(L, n)
+        f.restype, f.argtypes = r, a
+    return L
+
+
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(ORACLE_SO):
-            subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
-        L = C.CDLL(ORACLE_SO)
-        v = C.c_void_p
-        P = C.POINTER(OraConfig)
-        sig = {
-            "ora_create": (v, [P]), "ora_destroy": (None, [v]), "ora_num_eqn": (C.c_int, [v]),
-            "ora_num_cells": (C.c_size_t, [v]), "ora_field": (_DP, [v, C.c_char_p]),
-            "ora_boundary": (None, [v, C.c_int]), "ora_update_states": (C.c_int, [v, C.c_int]),
-            "ora_compute_lu": (None, [v, C.c_int]), "ora_rk3": (None, [v, C.c_int, C.c_double]),
-            "ora_get_dt": (C.c_double, [v]), "ora_chemistry": (None, [v, C.c_double]),
-            "ora_reaction": (C.c_int, [v, C.c_double]), "ora_estimate_nan": (C.c_int, [v, C.c_int]),
-            "ora_initialize": (C.c_int, [v]), "ora_steps": (C.c_int, [v, C.c_int, _DP]),
-            "ora_exp": (C.c_double, [C.c_double]), "ora_log": (C.c_double, [C.c_double]),
-            "ora_cp_i": (C.c_double, [P, C.c_double, C.c_int]), "ora_h_i": (C.c_double, [P, C.c_double, C.c_int]),
-            "ora_s_i": (C.c_double, [P, C.c_double, C.c_int]), "ora_g_i": (C.c_double, [P, C.c_double, C.c_int]),
-            "ora_e_i": (C.c_double, [P, C.c_double, C.c_int]),
-            "ora_mix_cp": (C.c_double, [P, _DP, C.c_double]), "ora_mix_cv": (C.c_double, [P, _DP, C.c_double]),
-            "ora_mix_h": (C.c_double, [P, _DP, C.c_double]), "ora_mix_e": (C.c_double, [P, _DP, C.c_double]),
-            "ora_mix_R": (C.c_double, [P, _DP]), "ora_get_T": (C.c_double, [P, _DP, C.c_double, C.c_double]),
-            "ora_weno5js": (C.c_double, [_DP, _DP]), "ora_weno5z": (C.c_double, [_DP, _DP]),
-            "ora_eigen_lr": (None, [P, C.c_int, C.c_int, _DP, _DP, _DP, _DP, _DP] + [C.c_double] * 8),
-            "ora_evalcv": (None, [P, _DP, _DP, _DP, C.c_double, C.c_double]),
-            "ora_chemeq2": (C.c_int, [P, _DP, C.c_double, C.c_double, C.c_double]),
-        }
-        for n, (r, a) in sig.items():
-            f = getattr(L, n)
-            f.restype, f.argtypes = r, a
-        _lib = L
+        _lib = load(ORACLE_SO)
     return _lib
 
 
@@ -122,15 +127,16 @@ def config_from_setup(setup, use_libm=0, **over):
 class Oracle:
     """One case on the CPU oracle (AoS fields, reference layout)."""
 
-    def __init__(self, cfg):
+    def __init__(self, cfg, L=None):
         self.cfg = cfg
-        self._h = lib().ora_create(C.byref(cfg))
-        self.E = lib().ora_num_eqn(self._h)
-        self.N = lib().ora_num_cells(self._h)
+        self.L = L or lib()
+        self._h = self.L.ora_create(C.byref(cfg))
+        self.E = self.L.ora_num_eqn(self._h)
+        self.N = self.L.ora_num_cells(self._h)
 
     def __del__(self):
         if getattr(self, "_h", None):
-            lib().ora_destroy(self._h)
+            self.L.ora_destroy(self._h)
             self._h = None
 
     def field(self, name, n=None):
@@ -143,7 +149,7 @@ class Oracle:
             cnt = 6
         else:
             cnt = self.N * sizes.get(name, 1)
-        p = lib().ora_field(self._h, name.encode())
+        p = self.L.ora_field(self._h, name.encode())
         return np.ctypeslib.as_array(p, shape=(cnt,))
 
     def set_state(self, U, T):
@@ -154,30 +160,30 @@ class Oracle:
         self.field("T")[:] = T
 
     def initialize(self):
-        return lib().ora_initialize(self._h)
+        return self.L.ora_initialize(self._h)
 
     def steps(self, n):
         dts = np.zeros(max(n, 1))
-        st = lib().ora_steps(self._h, n, dts.ctypes.data_as(_DP))
+        st = self.L.ora_steps(self._h, n, dts.ctypes.data_as(_DP))
         return st, dts[:n]
 
     def boundary(self, which):
-        lib().ora_boundary(self._h, which)
+        self.L.ora_boundary(self._h, which)
 
     def update_states(self, which):
-        return lib().ora_update_states(self._h, which)
+        return self.L.ora_update_states(self._h, which)
 
     def compute_lu(self, which):
-        lib().ora_compute_lu(self._h, which)
+        self.L.ora_compute_lu(self._h, which)
 
     def rk3(self, stage, dt):
-        lib().ora_rk3(self._h, stage, dt)
+        self.L.ora_rk3(self._h, stage, dt)
 
     def get_dt(self):
-        return lib().ora_get_dt(self._h)
+        return self.L.ora_get_dt(self._h)
 
     def chemistry(self, dt):
-        lib().ora_chemistry(self._h, dt)
+        self.L.ora_chemistry(self._h, dt)
 
     def reaction(self, dt):
-        return lib().ora_reaction(self._h, dt)
+        return self.L.ora_reaction(self._h, dt)

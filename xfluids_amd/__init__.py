@@ -123,6 +123,7 @@ _SIGS = {
     "xf_setup_context": (C.c_int, [_V, C.c_int, C.c_int, C.POINTER(_V)]),
     "xf_evolve_staged": (C.c_int, [_V, C.c_int, _DP]),
     "xf_debug_math": (C.c_int, [_DP, _DP, _DP, C.c_int]),
+    "xf_copy_probe": (C.c_int, [C.c_size_t, C.c_int, _DP]),
 }
 
 
@@ -384,3 +385,12 @@ def debug_math(x):
     if st:
         raise XfError(st, "xf_debug_math")
     return e, l
+
+
+def copy_probe(nbytes=1 << 30, reps=5):
+    """Average ms of a streaming device copy of nbytes (xf_copy_probe)."""
+    ms = C.c_double(0.0)
+    st = lib().xf_copy_probe(nbytes, reps, C.byref(ms))
+    if st != 0:
+        raise XfError(st, "xf_copy_probe")
+    return ms.value

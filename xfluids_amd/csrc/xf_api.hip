@@ -162,6 +162,13 @@ __global__ void k_debug_math(const double *x, double *ex, double *lg, int n)
     }
 }
 
+// streaming copy of n double2 (16-byte loads and stores, grid-stride)
+__global__ void __launch_bounds__(256) k_copy(const double2 *__restrict__ src, double2 *__restrict__ dst, long long n)
+{
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) dst[t] = src[t];
+}
+
 __global__ void k_set_double(double *p, double v)
 {
     if (threadIdx.x == 0) p[0] = v;
@@ -285,6 +292,10 @@ int xf_create(const xf_case_desc *desc, xf_ctx **out)
         d.Zmax != d.Z_inner + 2 * d.Bwidth_Z)
         return XF_EINVAL;
     if (d.precision != XF_PLAIN_FP64 && d.precision != XF_MIXED) return XF_EINVAL;
+    if (d.chem_driver != XF_CHEM_PER_CELL && d.chem_driver != XF_CHEM_QUEUE) return XF_EINVAL;
+    // the mixed-precision sweeps and the work-queue chemistry driver have no
+    // kernels in this library yet: refuse them rather than run the plain form
+    if (d.precision != XF_PLAIN_FP64 || d.chem_driver != XF_CHEM_PER_CELL) return XF_ENOKERNEL;
     Ops ops;
     if (!select_ops(d.NS, d.cop, ops)) return XF_ENOKERNEL;
     xf_ctx *c = new xf_ctx();
@@ -770,6 +781,40 @@ int xf_debug_math(const double *x, double *exp_out, double *log_out, int n)
     if (e == hipSuccess) e = hipMemcpy(exp_out, d + n, (size_t)n * sizeof(double), hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(log_out, d + 2 * n, (size_t)n * sizeof(double), hipMemcpyDeviceToHost);
     (void)hipFree(d);
+    return e == hipSuccess ? XF_OK : XF_EHIP;
+}
+
+int xf_copy_probe(size_t nbytes, int reps, double *avg_ms)
+{
+    if (nbytes < 16 || (nbytes & 15) || reps <= 0 || !avg_ms) return XF_EINVAL;
+    double2 *a = nullptr, *b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const long long n = (long long)(nbytes / 16);
+    hipError_t e = hipMalloc(&a, nbytes);
+    if (e == hipSuccess) e = hipMalloc(&b, nbytes);
+    if (e == hipSuccess) e = hipMemset(a, 0, nbytes);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    // 8 blocks of 256 lanes per CU-slot over 256 CUs, grid-stride beyond that
+    const long long want = (n + 255) / 256;
+    const unsigned blocks = (unsigned)(want < 256 * 32 ? want : 256 * 32);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, 0, a, b, n);  // warm-up
+        e = hipEventRecord(e0, 0);
+    }
+    for (int r = 0; e == hipSuccess && r < reps; r++) {
+        hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, 0, a, b, n);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(e1, 0);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e == hipSuccess) *avg_ms = (double)ms / reps;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
     return e == hipSuccess ? XF_OK : XF_EHIP;
 }
 

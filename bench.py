@@ -1,6 +1,6 @@
 """Throughput of the XFluids per-step hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--copy-probe]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--form both|plain|mixed] [--no-cpu]
 
 Workload: 2D inert shock-bubble interaction, 3 species (H2, O2, N2) with
 NASA-7 (JANAF) thermodynamics, 1024 x 512 cells, WENO5-Z characteristic
@@ -8,6 +8,11 @@ fluxes over Ns+4 = 7 variables, SSP-RK3, CFL 0.4 (cases/shock-bubble.json).
 One step = dt reduction + 3 RK stages (ghost fill, conservative->primitive
 with Newton temperature, x and y characteristic sweeps, RK update).
 Inputs are resident in HBM before the timed region.  Prints one JSON line.
+
+Two forms run by default: the plain FP64 form (the headline `value`,
+bit-identical to the CPU restatement) and the mixed-precision form (FP32
+smoothness indicators, weights and characteristic projections), each with its
+error against the FP64 oracle on a reduced grid of the same workload.
 """
 import argparse
 import json
@@ -54,10 +59,10 @@ def measured_traffic():
     return out
 
 
-def run(args):
+def run(args, form):
     s = xf.Setup(xf.case_path(CASE), nx=NX, ny=NY)
     d = s.desc
-    precision = xf.XF_PLAIN_FP64
+    precision = xf.XF_MIXED if form == "mixed" else xf.XF_PLAIN_FP64
     U, T = s.initial(CASE, seed=args.seed)
     ctx = s.context(precision=precision)
     ctx.upload(U, T)
@@ -115,30 +120,31 @@ def copy_probe(nbytes=1 << 30, reps=5):
             "GB_per_s": 2 * nbytes / (ms * 1e-3) / 1e9}
 
 
-def accuracy_check():
-    """Plain form vs the CPU oracle on a reduced grid (L1 relative error)."""
+def accuracy_check(form, steps=10):
+    """A form vs the CPU oracle on a reduced grid of the same workload (the
+    oracle runs the full 1024x512 grid too slowly): L1/Linf of rho, p, T and
+    mass-fraction errors and ranges, bitwise fraction of U."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_lib import Oracle, config_from_setup
+    from parity import field_errors
     s = xf.Setup(xf.case_path(CASE), nx=128, ny=64)
     d = s.desc
     U, T = s.initial(CASE, seed=3)
-    ctx = s.context()
+    ctx = s.context(precision=xf.XF_MIXED if form == "mixed" else xf.XF_PLAIN_FP64)
     ctx.upload(U, T)
     ctx.initialize()
-    ctx.steps(10)
+    ctx.steps(steps, graph=True)
     Ug, Tg = ctx.download(0)
     ctx.close()
     o = Oracle(config_from_setup(s))
     o.set_state(U, T)
     o.initialize()
-    o.steps(10)
-    Uo, To = o.field("U").reshape(-1, d.E), o.field("T")
-    Ug = Ug.reshape(-1, d.E)
-    l1_rho = float(np.abs(Ug[:, 0] - Uo[:, 0]).sum() / np.abs(Uo[:, 0]).sum())
-    l1_T = float(np.abs(Tg - To).sum() / np.abs(To).sum())
-    bitwise = float(np.mean(Ug.view(np.uint64) == Uo.view(np.uint64)))
-    return {"vs": "cpu oracle", "grid": "128x64", "steps": 10, "l1_rel_rho": l1_rho, "l1_rel_T": l1_T,
-            "bitwise_fraction_U": bitwise}
+    o.steps(steps)
+    Uo, To = o.field("U"), o.field("T")
+    out = {"vs": "cpu oracle (FP64)", "grid": "128x64", "steps": steps}
+    out.update(field_errors(s, Uo, To, Ug, Tg))
+    out["bitwise_fraction_U"] = float(np.mean(Ug.view(np.uint64) == Uo.view(np.uint64)))
+    return out
 
 
 def cpu_baseline(sample_steps):
@@ -168,7 +174,8 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--kernel-reps", type=int, default=60)
-    ap.add_argument("--form", choices=["plain"], default="plain")
+    ap.add_argument("--form", choices=["both", "plain", "mixed"], default="both",
+                    help="forms to run; the first is the headline value (plain when both)")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the reduced-grid check against the oracle")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 1 GiB device copy (counter scaling under rocprofv3 --pmc)")
@@ -179,7 +186,10 @@ def main():
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
         sys.exit(2)
-    d, value, ms, kt, finite = run(args)
+    forms = ["plain", "mixed"] if args.form == "both" else [args.form]
+    res = {f: run(args, f) for f in forms}
+    head = forms[0]
+    d, value, ms, kt, finite = res[head]
     line = {
         "metric": "Mcell-updates/s per RK3 step, plain FP64 vs mixed/batched, at stated L1 err",
         "value": value,
@@ -194,14 +204,19 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "2D inert shock-bubble, Ns=3 (H2,O2,N2), JANAF, WENO5-Z, SSP-RK3, CFL 0.4",
-                   "grid": "%dx%d" % (NX, NY), "form": args.form, "finite": finite},
+                   "grid": "%dx%d" % (NX, NY), "form": head, "finite": finite},
         "roofline": roofline(d, kt),
         "kernels_ms": {k: v[0] for k, v in kt.items()},
     }
+    line["forms"] = {}
+    for f in forms:
+        _, v, m, k, fin = res[f]
+        line["forms"][f] = {"value": v, "ms_per_step": m, "kernels_ms": {n: t[0] for n, t in k.items()},
+                            "finite": fin}
+        if not args.no_accuracy:
+            line["forms"][f]["accuracy"] = accuracy_check(f)
     if args.copy_probe:
         line["copy_probe"] = copy_probe()
-    if not args.no_accuracy:
-        line["accuracy"] = accuracy_check()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.cpu_steps)
     print(json.dumps(line))

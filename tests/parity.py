@@ -4,7 +4,7 @@ import sys
 import numpy as np
 
 import xfluids_amd as xf
-from oracle_lib import Oracle, config_from_setup
+from oracle_lib import RU, Oracle, config_from_setup
 
 # Workloads at sizes the oracle finishes in seconds: (case file, workload,
 # overrides, steps)
@@ -49,3 +49,45 @@ def compare(a, b, E=None):
         scale = max(np.abs(a).max(), 1e-300)
         rel = np.array([np.abs(a - b).max() / scale])
     return eq, rel
+
+
+def field_errors(s, Uo, To, Ug, Tg):
+    """Errors of a GPU state against the oracle's over interior cells: L1
+    (sum |d| / sum |ref|) and Linf (max |d| / max |ref|) of rho, p, T, and
+    of the mass fractions (absolute), with their ranges on both sides."""
+    d = s.desc
+    E = d.E
+    shp = (d.Zmax, d.Ymax, d.Xmax, E)
+    sl = (slice(d.Bwidth_Z, d.Bwidth_Z + d.Z_inner), slice(d.Bwidth_Y, d.Bwidth_Y + d.Y_inner),
+          slice(d.Bwidth_X, d.Bwidth_X + d.X_inner))
+    Uo = np.asarray(Uo).reshape(shp)[sl].reshape(-1, E)
+    Ug = np.asarray(Ug).reshape(shp)[sl].reshape(-1, E)
+    To = np.asarray(To).reshape(shp[:3])[sl].ravel()
+    Tg = np.asarray(Tg).reshape(shp[:3])[sl].ravel()
+
+    def prims(U, T):
+        rho = U[:, 0]
+        if d.cop:
+            Y = np.concatenate([U[:, 5:] / rho[:, None], 1.0 - (U[:, 5:] / rho[:, None]).sum(axis=1)[:, None]],
+                               axis=1)
+            _, _, W = s.thermo_tables()
+            p = rho * RU * (Y / np.asarray(W)[None, :]).sum(axis=1) * T
+            return rho, p, T, Y
+        ke = 0.5 * (U[:, 1] ** 2 + U[:, 2] ** 2 + U[:, 3] ** 2) / rho
+        p = (d.ncop_gamma - 1.0) * (U[:, 4] - ke)
+        return rho, p, None, None
+
+    ro, po, To_, Yo = prims(Uo, To)
+    rg, pg, Tg_, Yg = prims(Ug, Tg)
+    out = {}
+    for name, a, b in (("rho", ro, rg), ("p", po, pg), ("T", To_, Tg_)):
+        if a is None:
+            continue
+        out["l1_rel_" + name] = float(np.abs(b - a).sum() / np.abs(a).sum())
+        out["linf_rel_" + name] = float(np.abs(b - a).max() / np.abs(a).max())
+    if Yo is not None:
+        out["l1_abs_Y"] = float(np.abs(Yg - Yo).mean())
+        out["linf_abs_Y"] = float(np.abs(Yg - Yo).max())
+        out["Y_range_oracle"] = [float(Yo.min()), float(Yo.max())]
+        out["Y_range_gpu"] = [float(Yg.min()), float(Yg.max())]
+    return out

@@ -95,7 +95,7 @@ template <int NS, bool COP> struct Launch {
         hipLaunchKernelGGL((k_states<NS, COP>), dim3(nblk(c->N)), dim3(256), 0, c->stream, c->g, c->d_th, U, c->f,
                            c->d.ncop_gamma, full);
     }
-    template <int DIR, int SCH>
+    template <int DIR, int SCH, int PREC>
     static void sweep_t(xf_ctx *c, const double *Ug, int mode, int stage, const double *Ua, const double *Ub, double *Uo)
     {
         constexpr int E = COP ? NS + 4 : 5;
@@ -110,16 +110,27 @@ template <int NS, bool COP> struct Launch {
             grid = dim3((unsigned)((g.X_inner + SS::TW - 1) / SS::TW), (unsigned)((g.Z_inner + SS::TS - 2) / (SS::TS - 1)), (unsigned)g.Y_inner);
         static const char *names[3] = {"sweep_x", "sweep_y", "sweep_z"};
         ProfScope ps(c, names[DIR]);
-        hipLaunchKernelGGL((k_sweep<NS, COP, DIR, SCH>), grid, dim3(256), 0, c->stream, c->g, c->d_th, Ug, c->f,
-                           c->d.ncop_gamma, mode, stage, Ua, Ub, Uo);
+        hipLaunchKernelGGL((k_sweep<NS, COP, DIR, SCH, PREC>), grid, dim3(256), 0, c->stream, c->g, c->d_th, Ug,
+                           c->f, c->d.ncop_gamma, mode, stage, Ua, Ub, Uo);
+    }
+    template <int DIR, int PREC>
+    static void sweep_p(xf_ctx *c, const double *Ug, int mode, int stage, const double *Ua, const double *Ub, double *Uo)
+    {
+        if (c->d.scheme == XF_WENO5Z) sweep_t<DIR, XF_WENO5Z, PREC>(c, Ug, mode, stage, Ua, Ub, Uo);
+        else sweep_t<DIR, XF_WENO5JS, PREC>(c, Ug, mode, stage, Ua, Ub, Uo);
+    }
+    template <int DIR>
+    static void sweep_d(xf_ctx *c, const double *Ug, int mode, int stage, const double *Ua, const double *Ub, double *Uo)
+    {
+        if (c->d.precision == XF_MIXED) sweep_p<DIR, XF_MIXED>(c, Ug, mode, stage, Ua, Ub, Uo);
+        else sweep_p<DIR, XF_PLAIN_FP64>(c, Ug, mode, stage, Ua, Ub, Uo);
     }
     static void sweep(xf_ctx *c, int dir, const double *Ug, int mode, int stage, const double *Ua, const double *Ub,
                       double *Uo)
     {
-        const bool z = c->d.scheme == XF_WENO5Z;
-        if (dir == 0) z ? sweep_t<0, XF_WENO5Z>(c, Ug, mode, stage, Ua, Ub, Uo) : sweep_t<0, XF_WENO5JS>(c, Ug, mode, stage, Ua, Ub, Uo);
-        if (dir == 1) z ? sweep_t<1, XF_WENO5Z>(c, Ug, mode, stage, Ua, Ub, Uo) : sweep_t<1, XF_WENO5JS>(c, Ug, mode, stage, Ua, Ub, Uo);
-        if (dir == 2) z ? sweep_t<2, XF_WENO5Z>(c, Ug, mode, stage, Ua, Ub, Uo) : sweep_t<2, XF_WENO5JS>(c, Ug, mode, stage, Ua, Ub, Uo);
+        if (dir == 0) sweep_d<0>(c, Ug, mode, stage, Ua, Ub, Uo);
+        if (dir == 1) sweep_d<1>(c, Ug, mode, stage, Ua, Ub, Uo);
+        if (dir == 2) sweep_d<2>(c, Ug, mode, stage, Ua, Ub, Uo);
     }
     static void chem(xf_ctx *c, int record)
     {
@@ -293,9 +304,9 @@ int xf_create(const xf_case_desc *desc, xf_ctx **out)
         return XF_EINVAL;
     if (d.precision != XF_PLAIN_FP64 && d.precision != XF_MIXED) return XF_EINVAL;
     if (d.chem_driver != XF_CHEM_PER_CELL && d.chem_driver != XF_CHEM_QUEUE) return XF_EINVAL;
-    // the mixed-precision sweeps and the work-queue chemistry driver have no
-    // kernels in this library yet: refuse them rather than run the plain form
-    if (d.precision != XF_PLAIN_FP64 || d.chem_driver != XF_CHEM_PER_CELL) return XF_ENOKERNEL;
+    // the work-queue chemistry driver has no kernel in this library yet:
+    // refuse it rather than run the per-cell driver
+    if (d.chem_driver != XF_CHEM_PER_CELL) return XF_ENOKERNEL;
     Ops ops;
     if (!select_ops(d.NS, d.cop, ops)) return XF_ENOKERNEL;
     xf_ctx *c = new xf_ctx();

@@ -102,6 +102,57 @@ __device__ __forceinline__ double weno_face(const double *pp, const double *mm)
     return (temf + temm) * six;
 }
 
+// ---- mixed-precision form: FP32 WENO5 ------------------------------------
+// The smoothness indicators are evaluated on the stencil divided by its
+// largest magnitude m, where each is at most ~400, and the reference's
+// epsilon (1e-6 in its 12-beta units) becomes 1e-6 / m^2: the nonlinear
+// weights are those of the FP64 definition up to FP32 rounding, with no
+// overflow for any data magnitude.  WENO5-JS weights are normalised through
+// the ratios (s_min / s_k)^2; WENO5-Z keeps tau5 = |beta_1 - beta_3|.
+// Divisions use the hardware reciprocal (1 ulp).  A stencil below 1e-30 in
+// magnitude takes the ideal weights, as the FP64 form does when epsilon
+// dominates every indicator.
+__device__ __forceinline__ float rcp_f32(float x) { return __builtin_amdgcn_rcpf(x); }
+
+template <int SCHEME>
+__device__ __forceinline__ float weno5_body_f32(float v1, float v2, float v3, float v4, float v5)
+{
+    const float q1 = 2.0f * v1 - 7.0f * v2 + 11.0f * v3;
+    const float q2 = -v2 + 5.0f * v3 + 2.0f * v4;
+    const float q3 = 2.0f * v3 + 5.0f * v4 - v5;
+    const float m = fmaxf(fmaxf(fmaxf(fabsf(v1), fabsf(v2)), fmaxf(fabsf(v3), fabsf(v4))), fabsf(v5));
+    if (!(m > 1e-30f)) return (0.1f * q1 + 0.6f * q2 + 0.3f * q3) * (1.0f / 6.0f);
+    const float inv = rcp_f32(m);
+    const float u1 = v1 * inv, u2 = v2 * inv, u3 = v3 * inv, u4 = v4 * inv, u5 = v5 * inv;
+    float t = u1 - 2.0f * u2 + u3, r = u1 - 4.0f * u2 + 3.0f * u3;
+    float s1 = 13.0f * t * t + 3.0f * r * r;
+    t = u2 - 2.0f * u3 + u4, r = u2 - u4;
+    float s2 = 13.0f * t * t + 3.0f * r * r;
+    t = u3 - 2.0f * u4 + u5, r = 3.0f * u3 - 4.0f * u4 + u5;
+    float s3 = 13.0f * t * t + 3.0f * r * r;
+    const float eps = fmaxf(fminf(1e-6f * inv * inv, 1e30f), 1e-30f);
+    float a1, a2, a3;
+    if (SCHEME == XF_WENO5Z) {
+        const float tau = fabsf(s1 - s3);
+        a1 = 0.1f * (1.0f + tau * rcp_f32(s1 + eps));
+        a2 = 0.6f * (1.0f + tau * rcp_f32(s2 + eps));
+        a3 = 0.3f * (1.0f + tau * rcp_f32(s3 + eps));
+    } else {
+        s1 += eps, s2 += eps, s3 += eps;
+        const float smin = fminf(fminf(s1, s2), s3);
+        const float r1 = smin * rcp_f32(s1), r2 = smin * rcp_f32(s2), r3 = smin * rcp_f32(s3);
+        a1 = 0.1f * r1 * r1, a2 = 0.6f * r2 * r2, a3 = 0.3f * r3 * r3;
+    }
+    return (a1 * q1 + a2 * q2 + a3 * q3) * rcp_f32(a1 + a2 + a3) * (1.0f / 6.0f);
+}
+
+template <int SCHEME>
+__device__ __forceinline__ float weno_face_f32(const float *pp, const float *mm)
+{
+    return weno5_body_f32<SCHEME>(pp[1], pp[2], pp[3], pp[4], pp[5]) +
+           weno5_body_f32<SCHEME>(mm[6], mm[5], mm[4], mm[3], mm[2]);
+}
+
 // get_DpDrho / get_DpDrhoi (Utils_device.hpp:14-35)
 __device__ __forceinline__ double dpdrho(double hN, double RN, double q2, double Cp, double R, double T, double gamma)
 {
@@ -193,6 +244,26 @@ namespace xf {
 // the maximum of the last UpdateStates pass.  Every block computes the same
 // max(running, pass) and block 0 stores it back; readers before or after the
 // store see the same value (max is idempotent).
+// XCD-aware tile order.  Workgroups are dispatched round-robin over the 8
+// XCDs (linear id b runs on XCD b % 8), and each XCD has its own L2.  The
+// tile a workgroup works on is chosen so that XCD x owns one contiguous run
+// of tiles in row-major order: neighbouring tiles, which share halo cells and
+// the cache lines that straddle tile edges, then meet in the same L2.  The
+// map is a bijection of [0, nb) for any nb.
+constexpr int kXcds = 8;
+__device__ __forceinline__ void xcd_tile(int &tx, int &ty, int &tz)
+{
+    const int gx = (int)gridDim.x, gy = (int)gridDim.y;
+    const int nb = gx * gy * (int)gridDim.z;
+    const int b = (int)blockIdx.x + gx * ((int)blockIdx.y + gy * (int)blockIdx.z);
+    const int q = nb / kXcds, r = nb % kXcds;
+    const int x = b % kXcds, slot = b / kXcds;
+    const int l = (x < r) ? x * (q + 1) + slot : r * (q + 1) + (x - r) * q + slot;
+    tx = l % gx;
+    ty = (l / gx) % gy;
+    tz = l / (gx * gy);
+}
+
 template <int DIR, int E>
 __device__ __forceinline__ double glf_alpha(const double *red, int n)
 {
@@ -201,7 +272,7 @@ __device__ __forceinline__ double glf_alpha(const double *red, int n)
     return b > a ? b : a;
 }
 
-template <int NS, bool COP, int DIR, int SCHEME>
+template <int NS, bool COP, int DIR, int SCHEME, int PREC>
 __global__ void __launch_bounds__(256) k_sweep(Geom g, const DevThermo *__restrict__ th, const double *__restrict__ Ug,
                                                Fields f, double ncop_gamma, int mode, int stage, const double *Ua,
                                                const double *Ub, double *Uo)
@@ -210,8 +281,11 @@ __global__ void __launch_bounds__(256) k_sweep(Geom g, const DevThermo *__restri
     constexpr int NCOP = COP ? NS - 1 : 0;
     using SS = SweepShape<DIR, E>;
     constexpr int TS = SS::TS, TW = SS::TW, NC = SS::NC;
+    constexpr bool MIX = PREC == XF_MIXED;
     __shared__ double sU[E * NC];
-    __shared__ double sF[E * NC];
+    __shared__ double sF[MIX ? 1 : E * NC];   // FP64 flux: plain form only
+    __shared__ float sUf[MIX ? E * NC : 1];   // FP32 copies: mixed form only
+    __shared__ float sFf[MIX ? E * NC : 1];
     __shared__ double sP[NC], sR1[NC];
 
     const long long N = g.N;
@@ -221,24 +295,26 @@ __global__ void __launch_bounds__(256) k_sweep(Geom g, const DevThermo *__restri
     const int B = DIR == 0 ? g.Bx : (DIR == 1 ? g.By : g.Bz);
     const int inner = DIR == 0 ? g.X_inner : (DIR == 1 ? g.Y_inner : g.Z_inner);
     const int Lmax = DIR == 0 ? g.Xmax : (DIR == 1 ? g.Ymax : g.Zmax);
-    const int c0 = B + (TS - 1) * (int)(DIR == 0 ? blockIdx.x : blockIdx.y);
+    int tx, ty, tz;
+    xcd_tile(tx, ty, tz);
+    const int c0 = B + (TS - 1) * (DIR == 0 ? tx : ty);
     const long long stride = DIR == 0 ? 1LL : (DIR == 1 ? (long long)g.Xmax : (long long)g.Xmax * g.Ymax);
     const double *vel_n = DIR == 0 ? f.u : (DIR == 1 ? f.v : f.w);
 
     auto pencil_base = [&](int q, bool &ok) -> long long {
         if (DIR == 0) {
-            const int r = (int)blockIdx.y * TW + q;
+            const int r = ty * TW + q;
             ok = r < g.Y_inner * g.Z_inner;
             const int j = g.By + r % g.Y_inner, k = g.Bz + r / g.Y_inner;
             return (long long)g.Xmax * g.Ymax * k + (long long)g.Xmax * j;
         } else if (DIR == 1) {
-            const int i = g.Bx + (int)blockIdx.x * TW + q;
+            const int i = g.Bx + tx * TW + q;
             ok = i < g.Bx + g.X_inner;
-            return (long long)g.Xmax * g.Ymax * (g.Bz + (int)blockIdx.z) + i;
+            return (long long)g.Xmax * g.Ymax * (g.Bz + tz) + i;
         } else {
-            const int i = g.Bx + (int)blockIdx.x * TW + q;
+            const int i = g.Bx + tx * TW + q;
             ok = i < g.Bx + g.X_inner;
-            return (long long)g.Xmax * (g.By + (int)blockIdx.z) + i;
+            return (long long)g.Xmax * (g.By + tz) + i;
         }
     };
 
@@ -262,32 +338,37 @@ __global__ void __launch_bounds__(256) k_sweep(Geom g, const DevThermo *__restri
             const double Um = U[1 + DIR];
             sP[li] = p;
             sR1[li] = rho1;
-            sU[0 * NC + li] = U[0];
-            sF[0 * NC + li] = Um;
+            double F[E];
+            F[0] = Um;
 #pragma unroll
-            for (int k = 0; k < 3; k++) {
-                sU[(1 + k) * NC + li] = U[1 + k];
-                sF[(1 + k) * NC + li] = (k == DIR) ? Um * vel[k] + p : Um * vel[k];
-            }
-            sU[4 * NC + li] = U[4];
-            sF[4 * NC + li] = (U[4] + p) * vel[DIR];
-            if (COP) {
+            for (int k = 0; k < 3; k++) F[1 + k] = (k == DIR) ? Um * vel[k] + p : Um * vel[k];
+            F[4] = (U[4] + p) * vel[DIR];
 #pragma unroll
-                for (int ii = 5; ii < E; ii++) {
-                    sU[ii * NC + li] = U[ii];
-                    sF[ii * NC + li] = Um * (U[ii] * rho1);
+            for (int ii = 5; ii < E; ii++) F[ii] = Um * (U[ii] * rho1);
+#pragma unroll
+            for (int n = 0; n < E; n++) {
+                sU[n * NC + li] = U[n];
+                if (MIX) {
+                    sUf[n * NC + li] = (float)U[n];
+                    sFf[n * NC + li] = (float)F[n];
+                } else {
+                    sF[n * NC + li] = F[n];
                 }
             }
         } else {
 #pragma unroll
-            for (int n = 0; n < E; n++) sU[n * NC + li] = 0.0, sF[n * NC + li] = 0.0;
+            for (int n = 0; n < E; n++) {
+                sU[n * NC + li] = 0.0;
+                if (MIX) sUf[n * NC + li] = 0.0f, sFf[n * NC + li] = 0.0f;
+                else sF[n * NC + li] = 0.0;
+            }
             sP[li] = 0.0;
             sR1[li] = 0.0;
         }
     }
     __syncthreads();
 
-    if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && tid < 3) {
+    if (tx == 0 && ty == 0 && tz == 0 && tid < 3) {
         const double a = f.red[RED_GLF + 3 * DIR + tid], b = f.red[RED_GLFP + 3 * DIR + tid];
         f.red[RED_GLF + 3 * DIR + tid] = b > a ? b : a;
     }
@@ -361,6 +442,14 @@ __global__ void __launch_bounds__(256) k_sweep(Geom g, const DevThermo *__restri
             for (int m = 2; m <= 6; m++) mm[m] = 0.5 * (ff[m] - av * uf[m]);
             return weno_face<SCHEME>(pp, mm);
         };
+        auto split_weno_f32 = [&](float av, const float *uf, const float *ff) -> double {
+            float pp[7], mm[7];
+#pragma unroll
+            for (int m = 1; m <= 5; m++) pp[m] = 0.5f * fmaf(av, uf[m], ff[m]);
+#pragma unroll
+            for (int m = 2; m <= 6; m++) mm[m] = 0.5f * fmaf(-av, uf[m], ff[m]);
+            return (double)weno_face_f32<SCHEME>(pp, mm);
+        };
         // dense rows: n = 0 (u-c), n = 1 + DIR (normal), n = E - 1 (u+c);
         // called with literal n, so every branch on n folds at compile time
         auto dense_row = [&](const int n) {
@@ -394,21 +483,41 @@ __global__ void __launch_bounds__(256) k_sweep(Geom g, const DevThermo *__restri
             // over n1 in increasing order; the empty asm is a compiler-only
             // fence that keeps the LDS loads of one component from being
             // hoisted above the previous one (register pressure)
-            double uf[7], ff[7];
+            double fl;
+            if (MIX) {
+                // FP32 projections (fused multiply-add) and FP32 WENO
+                float uf[7], ff[7];
 #pragma unroll
-            for (int m = 1; m <= 6; m++) uf[m] = 0.0, ff[m] = 0.0;
+                for (int m = 1; m <= 6; m++) uf[m] = 0.0f, ff[m] = 0.0f;
 #pragma unroll
-            for (int n1 = 0; n1 < E; n1++) {
-                const double Ln = L[n1];
+                for (int n1 = 0; n1 < E; n1++) {
+                    const float Ln = (float)L[n1];
 #pragma unroll
-                for (int m = 1; m <= 6; m++) {
-                    const int li = lds_ix<DIR, TS, TW>(fidx + m - 1, tcol);
-                    uf[m] = uf[m] + sU[n1 * NC + li] * Ln;
-                    ff[m] = ff[m] + sF[n1 * NC + li] * Ln;
+                    for (int m = 1; m <= 6; m++) {
+                        const int li = lds_ix<DIR, TS, TW>(fidx + m - 1, tcol);
+                        uf[m] = fmaf(sUf[n1 * NC + li], Ln, uf[m]);
+                        ff[m] = fmaf(sFf[n1 * NC + li], Ln, ff[m]);
+                    }
+                    asm volatile("" ::: "memory");
                 }
-                asm volatile("" ::: "memory");
+                fl = split_weno_f32((float)glf_alpha<DIR, E>(f.red, n), uf, ff);
+            } else {
+                double uf[7], ff[7];
+#pragma unroll
+                for (int m = 1; m <= 6; m++) uf[m] = 0.0, ff[m] = 0.0;
+#pragma unroll
+                for (int n1 = 0; n1 < E; n1++) {
+                    const double Ln = L[n1];
+#pragma unroll
+                    for (int m = 1; m <= 6; m++) {
+                        const int li = lds_ix<DIR, TS, TW>(fidx + m - 1, tcol);
+                        uf[m] = uf[m] + sU[n1 * NC + li] * Ln;
+                        ff[m] = ff[m] + sF[n1 * NC + li] * Ln;
+                    }
+                    asm volatile("" ::: "memory");
+                }
+                fl = split_weno(glf_alpha<DIR, E>(f.red, n), uf, ff);
             }
-            const double fl = split_weno(glf_alpha<DIR, E>(f.red, n), uf, ff);
             // right column n, accumulated in the reference's summation order
             if (n == 0 || n == E - 1) {
                 flux[0] += fl * 1.0;
@@ -463,17 +572,31 @@ __global__ void __launch_bounds__(256) k_sweep(Geom g, const DevThermo *__restri
                 R4 = zm;
                 neg = false;
             }
-            double uf[7], ff[7];
+            double fl;
+            if (MIX) {
+                const float L0f = (float)L0;
+                float uf[7], ff[7];
 #pragma unroll
-            for (int m = 1; m <= 6; m++) {
-                const int li = lds_ix<DIR, TS, TW>(fidx + m - 1, tcol);
-                const double a0 = 0.0 + sU[li] * L0;
-                const double b0 = 0.0 + sF[li] * L0;
-                const double au = sU[kk * NC + li], bf = sF[kk * NC + li];
-                uf[m] = neg ? a0 + (-au) : a0 + au;
-                ff[m] = neg ? b0 + (-bf) : b0 + bf;
+                for (int m = 1; m <= 6; m++) {
+                    const int li = lds_ix<DIR, TS, TW>(fidx + m - 1, tcol);
+                    const float au = sUf[kk * NC + li], bf = sFf[kk * NC + li];
+                    uf[m] = fmaf(sUf[li], L0f, neg ? -au : au);
+                    ff[m] = fmaf(sFf[li], L0f, neg ? -bf : bf);
+                }
+                fl = split_weno_f32((float)glf_alpha<DIR, E>(f.red, n), uf, ff);
+            } else {
+                double uf[7], ff[7];
+#pragma unroll
+                for (int m = 1; m <= 6; m++) {
+                    const int li = lds_ix<DIR, TS, TW>(fidx + m - 1, tcol);
+                    const double a0 = 0.0 + sU[li] * L0;
+                    const double b0 = 0.0 + sF[li] * L0;
+                    const double au = sU[kk * NC + li], bf = sF[kk * NC + li];
+                    uf[m] = neg ? a0 + (-au) : a0 + au;
+                    ff[m] = neg ? b0 + (-bf) : b0 + bf;
+                }
+                fl = split_weno(glf_alpha<DIR, E>(f.red, n), uf, ff);
             }
-            const double fl = split_weno(glf_alpha<DIR, E>(f.red, n), uf, ff);
             const double t = neg ? -fl : fl;
 #pragma unroll
             for (int k = 1; k < E; k++)

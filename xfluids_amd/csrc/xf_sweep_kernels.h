@@ -73,7 +73,28 @@ __device__ __forceinline__ double weno5js_body(double v1, double v2, double v3, 
     return (s1 + s2 + s3);
 }
 
-// WENO5-Z body (weno5Z_P / weno5Z_M)
+// Correctly rounded quotients without the full division sequence.
+// Markstein (IBM J. Res. Dev. 34, 1990): if y = RN(1/b) and q0 = RN(a y) is
+// within one ulp of a/b, the remainder r = a - b q0 is exact (one fma) and
+// RN(q0 + r y) = RN(a/b).  Valid while a - b q0 does not underflow.
+__device__ __forceinline__ double quot_from_recip(double a, double b, double y)
+{
+    const double q0 = a * y;
+    return fma(fma(-b, q0, a), y, q0);
+}
+// RN(x / 6) for x = 0 (either sign) or 2^-960 <= |x| < inf; other x are
+// flagged in `slow` and the caller takes the IEEE division for them
+__device__ __forceinline__ double div6_fast(double x, bool &slow)
+{
+    const double q = quot_from_recip(x, 6.0, 1.0 / 6.0);
+    slow = slow || (x != 0.0 && !(fabs(x) >= 0x1p-960 && fabs(x) <= 0x1p1000));
+    return x == 0.0 ? x : q;
+}
+
+// WENO5-Z body (weno5Z_P / weno5Z_M).  The three weights share the divisor
+// S = a1 + a2 + a3 >= 1 and have numerators >= 0.1, so they come from one
+// correctly rounded reciprocal RN(1/S); the divisions by 6 use div6_fast.  Every
+// quotient is the IEEE-rounded one, bit for bit.
 __device__ __forceinline__ double weno5z_body(double v1, double v2, double v3, double v4, double v5)
 {
     const double epsilon = 1.0e-6;
@@ -84,11 +105,18 @@ __device__ __forceinline__ double weno5z_body(double v1, double v2, double v3, d
     const double a1 = 0.1 * (1.0 + s5 / (s1 + epsilon));
     const double a2 = 0.6 * (1.0 + s5 / (s2 + epsilon));
     const double a3 = 0.3 * (1.0 + s5 / (s3 + epsilon));
-    const double w1 = a1 / (a1 + a2 + a3);
-    const double w2 = a2 / (a1 + a2 + a3);
-    const double w3 = a3 / (a1 + a2 + a3);
-    return w1 * (2.0 * v1 - 7.0 * v2 + 11.0 * v3) / 6.0 + w2 * (-v2 + 5.0 * v3 + 2.0 * v4) / 6.0 +
-           w3 * (2.0 * v3 + 5.0 * v4 - v5) / 6.0;
+    const double S = a1 + a2 + a3;
+    const double rS = 1.0 / S;
+    const double w1 = quot_from_recip(a1, S, rS);
+    const double w2 = quot_from_recip(a2, S, rS);
+    const double w3 = quot_from_recip(a3, S, rS);
+    const double x1 = w1 * (2.0 * v1 - 7.0 * v2 + 11.0 * v3);
+    const double x2 = w2 * (-v2 + 5.0 * v3 + 2.0 * v4);
+    const double x3 = w3 * (2.0 * v3 + 5.0 * v4 - v5);
+    bool slow = false;
+    double q1 = div6_fast(x1, slow), q2 = div6_fast(x2, slow), q3 = div6_fast(x3, slow);
+    if (slow) q1 = x1 / 6.0, q2 = x2 / 6.0, q3 = x3 / 6.0;
+    return q1 + q2 + q3;
 }
 
 template <int SCHEME>
@@ -272,8 +300,15 @@ __device__ __forceinline__ double glf_alpha(const double *red, int n)
     return b > a ? b : a;
 }
 
+// Waves per SIMD the sweep is compiled for (the register budget per lane is
+// 512 / waves): the species count sets how much state a face lane carries
+template <int NS> struct SweepOccupancy {
+    static constexpr int waves = NS == 1 ? 4 : (NS <= 3 ? 3 : 2);
+};
+
 template <int NS, bool COP, int DIR, int SCHEME, int PREC>
-__global__ void __launch_bounds__(256) k_sweep(Geom g, const DevThermo *__restrict__ th, const double *__restrict__ Ug,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SweepOccupancy<NS>::waves)))
+k_sweep(Geom g, const DevThermo *__restrict__ th, const double *__restrict__ Ug,
                                                Fields f, double ncop_gamma, int mode, int stage, const double *Ua,
                                                const double *Ub, double *Uo)
 {

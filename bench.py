@@ -67,6 +67,8 @@ def run(args, form):
     ctx = s.context(precision=precision)
     ctx.upload(U, T)
     ctx.initialize()
+    # tile shape per sweep direction, chosen by timing on this state
+    tuned = ctx.autotune(reps=10) if not args.no_tune else (ctx.tile_shape(), None)
     # warm-up (also instantiates the step graph)
     ctx.steps(args.warmup, graph=True)
     ctx.synchronize()
@@ -85,7 +87,7 @@ def run(args, form):
     cells = d.X_inner * d.Y_inner * d.Z_inner
     value = cells * args.steps / (t1 - t0) / 1e6
     finite = bool(np.all(np.isfinite(Uf)))
-    return d, value, ms_per_step, kt, finite
+    return d, value, ms_per_step, kt, finite, tuned
 
 
 def roofline(d, kt):
@@ -147,6 +149,80 @@ def accuracy_check(form, steps=10):
     return out
 
 
+REACTIVE = [
+    # (label, case file / workload, overrides, timed steps)
+    ("reactive shock tube H2/O2/Ar, Ns=9 NR=19, 8192 cells, Strang", "reactive-shock-tube", dict(nx=8192), 40),
+    ("reactive channel H2/O2/Ar with hot kernel, Ns=9 NR=19, 512x512, Strang", "reactive-channel",
+     dict(nx=512, ny=512), 10),
+]
+REACTIVE_FORMS = [("plain", "per-cell"), ("plain", "queue"), ("mixed", "queue")]
+
+
+def reactive_leg(label, case, over, steps, warmup=2, seed=1):
+    """Full reactive steps (dt, two Strang chemistry half-steps, three RK
+    stages) for each form; chemistry kernel time from HIP events in a few
+    ungraphed steps; per-cell predictor-pass statistics of one call."""
+    s = xf.Setup(xf.case_path(case), **over)
+    d = s.desc
+    U, T = s.initial(case, seed=seed)
+    cells = d.X_inner * d.Y_inner * d.Z_inner
+    out = {"workload": label, "cells": cells, "steps": steps, "forms": {}}
+    for prec, drv in REACTIVE_FORMS:
+        ctx = s.context(precision=xf.XF_MIXED if prec == "mixed" else xf.XF_PLAIN_FP64,
+                        chem_driver=xf.XF_CHEM_QUEUE if drv == "queue" else xf.XF_CHEM_PER_CELL)
+        ctx.upload(U, T)
+        ctx.initialize()
+        shapes, _ = ctx.autotune(reps=5)
+        ctx.steps(warmup, graph=True)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.steps_async(steps, graph=True)
+        ctx.synchronize()
+        t1 = time.perf_counter()
+        ctx.set_profiling(True, only=["chem", "sweep_x", "sweep_y", "sweep_z", "states"])
+        ctx.steps(2)
+        kt = ctx.kernel_times()
+        ctx.set_profiling(False)
+        ctx.update_states(0)
+        sub = ctx.chemistry(0.5 * ctx.get_dt(), record=True)
+        ctx.close()
+        out["forms"]["%s/%s" % (prec, drv)] = {
+            "value": cells * steps / (t1 - t0) / 1e6, "ms_per_step": (t1 - t0) * 1e3 / steps,
+            "tile_shapes": [int(x) for x in shapes],
+            "kernels_ms": {k: v[0] for k, v in kt.items()},
+            "chem_passes": {"min": int(sub.min()), "median": float(np.median(sub)),
+                            "p99": float(np.percentile(sub, 99)), "max": int(sub.max()), "mean": float(sub.mean())}}
+    base = out["forms"]["plain/per-cell"]["value"]
+    for k, v in out["forms"].items():
+        v["speedup_vs_plain_per_cell"] = v["value"] / base
+    return out
+
+
+def reactive_accuracy(steps=20):
+    """Mixed form with the queue driver vs the FP64 oracle, reactive shock
+    tube on 1024 cells (the oracle runs it in about a second)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import Oracle, config_from_setup
+    from parity import field_errors
+    s = xf.Setup(xf.case_path("reactive-shock-tube"), nx=1024)
+    U, T = s.initial("reactive-shock-tube", seed=1)
+    o = Oracle(config_from_setup(s))
+    o.set_state(U, T)
+    o.initialize()
+    o.steps(steps)
+    out = {"vs": "cpu oracle (FP64)", "workload": "reactive shock tube, 1024 cells", "steps": steps}
+    for prec, drv in (("plain", "queue"), ("mixed", "queue")):
+        ctx = s.context(precision=xf.XF_MIXED if prec == "mixed" else xf.XF_PLAIN_FP64,
+                        chem_driver=xf.XF_CHEM_QUEUE)
+        ctx.upload(U, T)
+        ctx.initialize()
+        ctx.steps(steps, graph=True)
+        Ug, Tg = ctx.download(0)
+        ctx.close()
+        out["%s/%s" % (prec, drv)] = field_errors(s, o.field("U"), o.field("T"), Ug, Tg)
+    return out
+
+
 def cpu_baseline(sample_steps):
     """Oracle (plain C, FP64, OpenMP) on the full 1024x512 grid for a few steps."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -177,6 +253,8 @@ def main():
     ap.add_argument("--form", choices=["both", "plain", "mixed"], default="both",
                     help="forms to run; the first is the headline value (plain when both)")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the reduced-grid check against the oracle")
+    ap.add_argument("--no-reactive", action="store_true", help="skip the reactive workloads")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default sweep tile shapes")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 1 GiB device copy (counter scaling under rocprofv3 --pmc)")
     ap.add_argument("--seed", type=int, default=1)
@@ -189,7 +267,7 @@ def main():
     forms = ["plain", "mixed"] if args.form == "both" else [args.form]
     res = {f: run(args, f) for f in forms}
     head = forms[0]
-    d, value, ms, kt, finite = res[head]
+    d, value, ms, kt, finite, _ = res[head]
     line = {
         "metric": "Mcell-updates/s per RK3 step, plain FP64 vs mixed/batched, at stated L1 err",
         "value": value,
@@ -210,11 +288,17 @@ def main():
     }
     line["forms"] = {}
     for f in forms:
-        _, v, m, k, fin = res[f]
+        _, v, m, k, fin, tuned = res[f]
         line["forms"][f] = {"value": v, "ms_per_step": m, "kernels_ms": {n: t[0] for n, t in k.items()},
-                            "finite": fin}
+                            "finite": fin, "tile_shapes": [int(x) for x in tuned[0]]}
+        if tuned[1] is not None:
+            line["forms"][f]["autotune_ms"] = {"sweep_x": tuned[1][0, :2].tolist(), "sweep_y": tuned[1][1].tolist()}
         if not args.no_accuracy:
             line["forms"][f]["accuracy"] = accuracy_check(f)
+    if not args.no_reactive:
+        line["reactive"] = [reactive_leg(*r) for r in REACTIVE]
+        if not args.no_accuracy:
+            line["reactive_accuracy"] = reactive_accuracy()
     if args.copy_probe:
         line["copy_probe"] = copy_probe()
     if not args.no_cpu:

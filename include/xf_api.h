@@ -180,6 +180,19 @@ int xf_kernel_times(xf_ctx *ctx, const char **names, double *avg_ms, int *counts
    carries the fused RK update U1 = U + dt LU).  Overwrites only LU, U1 and
    the primitives, which every stage recomputes. */
 int xf_time_kernel(xf_ctx *ctx, const char *name, int reps, double *avg_ms);
+/* Sweep tile shape per direction (0 = x, 1 = y, 2 = z): x 0 = 64 faces x 4
+   pencils, 1 = 32 x 8; y, z 0 = 16 x 16, 1 = 32 x 8, 2 = 8 x 32.  Every shape
+   computes the same bits.  Replaces the fixed work-group sizes of the
+   settings file (run.blockSize_x/y/z, src/read_ini/settings/read_json.cpp:112-114). */
+int xf_set_tile_shape(xf_ctx *ctx, int dir, int shape);
+int xf_get_tile_shape(const xf_ctx *ctx, int *shape3);
+/* Autotuner: times every tile shape of every active sweep direction on the
+   context's current state (reps back-to-back launches each, stage-1
+   configuration) and keeps the fastest.  shape3[3] receives the choice,
+   ms[9] (optional) the mean ms per launch, ms[3*dir + shape].  Leaves the
+   running GLF maxima unchanged; overwrites LU, U1 and the primitives, which
+   every stage recomputes.  Call after xf_upload_state + xf_initialize. */
+int xf_autotune(xf_ctx *ctx, int reps, int *shape3, double *ms);
 /* Evaluate the device exp and log (the fixed-sequence FP64 functions every
    kernel uses) on n host values; for bit-level checks against the CPU. */
 int xf_debug_math(const double *x, double *exp_out, double *log_out, int n);

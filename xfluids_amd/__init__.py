@@ -124,6 +124,9 @@ _SIGS = {
     "xf_evolve_staged": (C.c_int, [_V, C.c_int, _DP]),
     "xf_debug_math": (C.c_int, [_DP, _DP, _DP, C.c_int]),
     "xf_copy_probe": (C.c_int, [C.c_size_t, C.c_int, _DP]),
+    "xf_set_tile_shape": (C.c_int, [_V, C.c_int, C.c_int]),
+    "xf_get_tile_shape": (C.c_int, [_V, _IP]),
+    "xf_autotune": (C.c_int, [_V, C.c_int, _IP, _DP]),
 }
 
 
@@ -367,6 +370,22 @@ class Context:
         ms = C.c_double()
         self._chk(lib().xf_time_kernel(self._h, name.encode(), reps, C.byref(ms)), "xf_time_kernel")
         return ms.value
+
+    def set_tile_shape(self, direction, shape):
+        self._chk(lib().xf_set_tile_shape(self._h, direction, shape), "xf_set_tile_shape")
+
+    def tile_shape(self):
+        out = np.zeros(3, np.int32)
+        self._chk(lib().xf_get_tile_shape(self._h, _i(out)), "xf_get_tile_shape")
+        return out
+
+    def autotune(self, reps=20):
+        """Pick the fastest sweep tile shape per direction (xf_autotune);
+        returns (shapes[3], ms[3, 3])."""
+        shapes = np.zeros(3, np.int32)
+        ms = np.zeros(9)
+        self._chk(lib().xf_autotune(self._h, reps, _i(shapes), _d(ms)), "xf_autotune")
+        return shapes, ms.reshape(3, 3)
 
     def kernel_times(self):
         cap = 32

@@ -24,6 +24,7 @@ struct Ops {
     void (*sweep)(xf_ctx *, int dir, const double *Ug, int mode, int stage, const double *Ua, const double *Ub,
                   double *Uo);
     void (*chem)(xf_ctx *, int record);
+    void (*prepare)(xf_ctx *);
 };
 
 } // namespace
@@ -43,6 +44,8 @@ struct xf_ctx {
     bool thermo_set = false, mech_set = false;
     Ops ops{};
     bool has_ops = false;
+    int chem_blocks = 0; // resident one-wave blocks of the work-queue chemistry kernel
+    int shape[3] = {0, 0, 0}; // sweep tile shape per direction (SweepShape)
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int last_code = 0, last_id = -1;
@@ -95,11 +98,11 @@ template <int NS, bool COP> struct Launch {
         hipLaunchKernelGGL((k_states<NS, COP>), dim3(nblk(c->N)), dim3(256), 0, c->stream, c->g, c->d_th, U, c->f,
                            c->d.ncop_gamma, full);
     }
-    template <int DIR, int SCH, int PREC>
+    template <int DIR, int SCH, int PREC, int SHAPE>
     static void sweep_t(xf_ctx *c, const double *Ug, int mode, int stage, const double *Ua, const double *Ub, double *Uo)
     {
         constexpr int E = COP ? NS + 4 : 5;
-        using SS = SweepShape<DIR, E>;
+        using SS = SweepShape<DIR, E, SHAPE>;
         const Geom &g = c->g;
         dim3 grid;
         if (DIR == 0)
@@ -110,14 +113,28 @@ template <int NS, bool COP> struct Launch {
             grid = dim3((unsigned)((g.X_inner + SS::TW - 1) / SS::TW), (unsigned)((g.Z_inner + SS::TS - 2) / (SS::TS - 1)), (unsigned)g.Y_inner);
         static const char *names[3] = {"sweep_x", "sweep_y", "sweep_z"};
         ProfScope ps(c, names[DIR]);
-        hipLaunchKernelGGL((k_sweep<NS, COP, DIR, SCH, PREC>), grid, dim3(256), 0, c->stream, c->g, c->d_th, Ug,
-                           c->f, c->d.ncop_gamma, mode, stage, Ua, Ub, Uo);
+        hipLaunchKernelGGL((k_sweep<NS, COP, DIR, SCH, PREC, SHAPE>), grid, dim3(256), 0, c->stream, c->g, c->d_th,
+                           Ug, c->f, c->d.ncop_gamma, mode, stage, Ua, Ub, Uo);
+    }
+    template <int DIR, int SCH, int PREC>
+    static void sweep_s(xf_ctx *c, const double *Ug, int mode, int stage, const double *Ua, const double *Ub, double *Uo)
+    {
+        switch (c->shape[DIR]) {
+        case 1: sweep_t<DIR, SCH, PREC, 1>(c, Ug, mode, stage, Ua, Ub, Uo); break;
+        case 2:
+            if constexpr (SweepShapes<DIR>::count > 2) {
+                sweep_t<DIR, SCH, PREC, 2>(c, Ug, mode, stage, Ua, Ub, Uo);
+                break;
+            }
+            [[fallthrough]];
+        default: sweep_t<DIR, SCH, PREC, 0>(c, Ug, mode, stage, Ua, Ub, Uo);
+        }
     }
     template <int DIR, int PREC>
     static void sweep_p(xf_ctx *c, const double *Ug, int mode, int stage, const double *Ua, const double *Ub, double *Uo)
     {
-        if (c->d.scheme == XF_WENO5Z) sweep_t<DIR, XF_WENO5Z, PREC>(c, Ug, mode, stage, Ua, Ub, Uo);
-        else sweep_t<DIR, XF_WENO5JS, PREC>(c, Ug, mode, stage, Ua, Ub, Uo);
+        if (c->d.scheme == XF_WENO5Z) sweep_s<DIR, XF_WENO5Z, PREC>(c, Ug, mode, stage, Ua, Ub, Uo);
+        else sweep_s<DIR, XF_WENO5JS, PREC>(c, Ug, mode, stage, Ua, Ub, Uo);
     }
     template <int DIR>
     static void sweep_d(xf_ctx *c, const double *Ug, int mode, int stage, const double *Ua, const double *Ub, double *Uo)
@@ -136,11 +153,32 @@ template <int NS, bool COP> struct Launch {
     {
         if constexpr (COP) {
             ProfScope ps(c, "chem");
-            hipLaunchKernelGGL((k_chem<NS>), dim3(nblk(c->Ninner)), dim3(256), 0, c->stream, c->g, c->d_th, c->d_mc,
-                               c->f, record);
+            if (c->d.chem_driver == XF_CHEM_QUEUE) {
+                hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, c->stream, c->f.queue);
+                hipLaunchKernelGGL((k_chem_queue<NS>), dim3(c->chem_blocks), dim3(64), 0, c->stream, c->g, c->d_th,
+                                   c->d_mc, c->f, record);
+            } else {
+                hipLaunchKernelGGL((k_chem<NS>), dim3(nblk(c->Ninner)), dim3(256), 0, c->stream, c->g, c->d_th,
+                                   c->d_mc, c->f, record);
+            }
         }
     }
-    static Ops ops() { return Ops{&states, &sweep, &chem}; }
+    // persistent lanes of the queue driver: as many one-wave blocks as stay
+    // resident, never more than the cells need (queried once, outside any
+    // stream capture)
+    static void prepare(xf_ctx *c)
+    {
+        if constexpr (COP) {
+            int per_cu = 0, ncu = 0, dev = 0;
+            (void)hipGetDevice(&dev);
+            (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_chem_queue<NS>, 64, 0);
+            const long long want = (c->Ninner + 63) / 64;
+            const long long nb = (long long)std::max(per_cu, 1) * std::max(ncu, 1);
+            c->chem_blocks = (int)std::max(1LL, std::min(nb, want));
+        }
+    }
+    static Ops ops() { return Ops{&states, &sweep, &chem, &prepare}; }
 };
 
 bool select_ops(int NS, int cop, Ops &o)
@@ -304,9 +342,6 @@ int xf_create(const xf_case_desc *desc, xf_ctx **out)
         return XF_EINVAL;
     if (d.precision != XF_PLAIN_FP64 && d.precision != XF_MIXED) return XF_EINVAL;
     if (d.chem_driver != XF_CHEM_PER_CELL && d.chem_driver != XF_CHEM_QUEUE) return XF_EINVAL;
-    // the work-queue chemistry driver has no kernel in this library yet:
-    // refuse it rather than run the per-cell driver
-    if (d.chem_driver != XF_CHEM_PER_CELL) return XF_ENOKERNEL;
     Ops ops;
     if (!select_ops(d.NS, d.cop, ops)) return XF_ENOKERNEL;
     xf_ctx *c = new xf_ctx();
@@ -358,6 +393,9 @@ int xf_create(const xf_case_desc *desc, xf_ctx **out)
     if (hipMemcpy(c->d_th, &c->h_th, sizeof(DevThermo), hipMemcpyHostToDevice) ||
         hipMemcpy(c->d_mc, &c->h_mc, sizeof(DevMech), hipMemcpyHostToDevice))
         return fail(XF_EHIP);
+    unsigned qz[4] = {0, 0, 0, 0};
+    if (hipMemcpy(f.queue, qz, sizeof(qz), hipMemcpyHostToDevice)) return fail(XF_EHIP);
+    c->ops.prepare(c);
     *out = c;
     return XF_OK;
 }
@@ -702,26 +740,39 @@ int xf_set_profiling(xf_ctx *c, int on)
     return XF_OK;
 }
 
-int xf_time_kernel(xf_ctx *c, const char *name, int reps, double *avg_ms)
+namespace {
+// the GLF/CFL reduction slots, saved around measurement launches so that a
+// timing or tuning run leaves the running GLF maxima exactly as it found them
+struct RedGuard {
+    xf_ctx *c;
+    double h[RED_SIZE];
+    bool ok;
+    explicit RedGuard(xf_ctx *c_) : c(c_)
+    {
+        ok = hipMemcpyAsync(h, c->f.red, sizeof(h), hipMemcpyDeviceToHost, c->stream) == hipSuccess &&
+             hipStreamSynchronize(c->stream) == hipSuccess;
+    }
+    ~RedGuard()
+    {
+        if (ok) {
+            (void)hipMemcpyAsync(c->f.red, h, sizeof(h), hipMemcpyHostToDevice, c->stream);
+            (void)hipStreamSynchronize(c->stream);
+        }
+    }
+};
+
+// average device ms of `reps` back-to-back launches of one kernel class in
+// its stage-1 configuration (dir < 0: UpdateStates on U)
+int time_launches(xf_ctx *c, int dir, int reps, double *avg_ms)
 {
-    if (!c || !name || reps <= 0 || !avg_ms) return XF_EINVAL;
-    if (c->d.cop && !c->thermo_set) return XF_EINVAL;
-    const std::string n(name);
     int dirs[3], nd = 0;
     if (c->g.DimX) dirs[nd++] = 0;
     if (c->g.DimY) dirs[nd++] = 1;
     if (c->g.DimZ) dirs[nd++] = 2;
-    int dir = -1;
-    if (n == "sweep_x") dir = 0;
-    else if (n == "sweep_y") dir = 1;
-    else if (n == "sweep_z") dir = 2;
-    else if (n != "states") return XF_EINVAL;
     int q = -1;
     for (int i = 0; i < nd; i++)
         if (dirs[i] == dir) q = i;
     if (dir >= 0 && q < 0) return XF_EINVAL;
-    const bool prof = c->prof;
-    c->prof = false;
     hipEvent_t a, b;
     XF_HIP(hipEventCreate(&a));
     XF_HIP(hipEventCreate(&b));
@@ -741,9 +792,90 @@ int xf_time_kernel(xf_ctx *c, const char *name, int reps, double *avg_ms)
     XF_HIP(hipEventElapsedTime(&ms, a, b));
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
-    c->prof = prof;
     *avg_ms = ms / reps;
-    if (int e = check_launch()) return e;
+    return check_launch();
+}
+} // namespace
+
+int xf_set_tile_shape(xf_ctx *c, int dir, int shape)
+{
+    if (!c || dir < 0 || dir > 2 || shape < 0) return XF_EINVAL;
+    const int count = dir == 0 ? SweepShapes<0>::count : SweepShapes<1>::count;
+    if (shape >= count) return XF_EINVAL;
+    if (c->shape[dir] != shape) {
+        c->shape[dir] = shape;
+        // the captured step graph holds the old kernels
+        if (c->gexec) (void)hipGraphExecDestroy(c->gexec), c->gexec = nullptr;
+        if (c->graph) (void)hipGraphDestroy(c->graph), c->graph = nullptr;
+    }
+    return XF_OK;
+}
+
+int xf_get_tile_shape(const xf_ctx *c, int *shape3)
+{
+    if (!c || !shape3) return XF_EINVAL;
+    for (int d = 0; d < 3; d++) shape3[d] = c->shape[d];
+    return XF_OK;
+}
+
+int xf_autotune(xf_ctx *c, int reps, int *shape3, double *ms)
+{
+    if (!c || reps <= 0) return XF_EINVAL;
+    if (c->d.cop && !c->thermo_set) return XF_EINVAL;
+    const bool prof = c->prof;
+    c->prof = false;
+    int rc = XF_OK;
+    {
+        RedGuard guard(c);
+        if (!guard.ok) return XF_EHIP;
+        const int dims[3] = {c->g.DimX, c->g.DimY, c->g.DimZ};
+        for (int dir = 0; dir < 3 && rc == XF_OK; dir++) {
+            const int count = dir == 0 ? SweepShapes<0>::count : SweepShapes<1>::count;
+            for (int k = 0; k < 3; k++)
+                if (ms) ms[3 * dir + k] = 0.0;
+            if (!dims[dir]) continue;
+            int best = 0;
+            double best_ms = 0.0;
+            for (int k = 0; k < count && rc == XF_OK; k++) {
+                c->shape[dir] = k;
+                double t = 0.0;
+                rc = time_launches(c, dir, 2, &t); // warm-up (code load, caches)
+                if (rc == XF_OK) rc = time_launches(c, dir, reps, &t);
+                if (ms) ms[3 * dir + k] = t;
+                if (rc == XF_OK && (k == 0 || t < best_ms)) best = k, best_ms = t;
+            }
+            c->shape[dir] = best;
+        }
+    }
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec), c->gexec = nullptr;
+    if (c->graph) (void)hipGraphDestroy(c->graph), c->graph = nullptr;
+    c->prof = prof;
+    if (shape3)
+        for (int d = 0; d < 3; d++) shape3[d] = c->shape[d];
+    if (rc != XF_OK) return rc;
+    return take_error(c);
+}
+
+int xf_time_kernel(xf_ctx *c, const char *name, int reps, double *avg_ms)
+{
+    if (!c || !name || reps <= 0 || !avg_ms) return XF_EINVAL;
+    if (c->d.cop && !c->thermo_set) return XF_EINVAL;
+    const std::string n(name);
+    int dir = -1;
+    if (n == "sweep_x") dir = 0;
+    else if (n == "sweep_y") dir = 1;
+    else if (n == "sweep_z") dir = 2;
+    else if (n != "states") return XF_EINVAL;
+    const bool prof = c->prof;
+    c->prof = false;
+    int rc;
+    {
+        RedGuard guard(c);
+        if (!guard.ok) return XF_EHIP;
+        rc = time_launches(c, dir, reps, avg_ms);
+    }
+    c->prof = prof;
+    if (rc != XF_OK) return rc;
     return take_error(c);
 }
 

@@ -121,37 +121,51 @@ __device__ __forceinline__ double alpha_of(double r)
     return (180.0 + r * (60.0 + r * (11.0 + r))) / (360.0 + r * (60.0 + r * (12.0 + r)));
 }
 
-// Chemeq2 on one cell.  Returns predictor passes, or -1 at the safety bound.
-template <int NS>
-__device__ int chemeq2_dev(const DevThermo *__restrict__ th, const DevMech *__restrict__ mc, double *y, double dtg,
-                           double rho, double m_p)
-{
-    constexpr int NL = NS + 1;
-    const double ymin = 1.0e-20, epsmin = 1.0e-4, tfd = 1.0 + 1.0e-10, sqreps = 0.05, epscl = 1.0e4;
-    double dtmin = 1.0e-7, scrtch = 1e-25, epsmax = 1.0;
-    double ys[NL], qs[NL], ds[NL], q[NL], d[NL], rtau[NL];
-    double sumy = 0.0;
+// Chemeq2 state of one cell, advanced one predictor/corrector pass at a time
+// so that a lane can also work through a queue of cells (k_chem_queue).
+template <int NS> struct ChemLane {
+    static constexpr int NL = NS + 1;
+    double y[NL], ys[NL], qs[NL], ds[NL], q[NL], d[NL], rtau[NL];
+    double dt, dtg, dtmin, epsmax, ts, tn, rho, m_p;
+    int passes;
+
+    // y = [T, Y_1..Y_NS] on entry: clip and renormalise Y, first rate
+    // evaluation, initial step (ProductionRates.h:488-530)
+    __device__ __forceinline__ void start(const DevThermo *__restrict__ th, const DevMech *__restrict__ mc,
+                                          double dtg_, double rho_, double p_)
+    {
+        const double ymin = 1.0e-20, epsmin = 1.0e-4, sqreps = 0.05;
+        dtg = dtg_, rho = rho_, m_p = p_;
+        dtmin = 1.0e-7, epsmax = 1.0;
+        double scrtch = 1e-25;
+        double sumy = 0.0;
 #pragma unroll
-    for (int i = 1; i < NL; i++) y[i] = dmax(y[i], ymin), sumy += y[i];
-    sumy = 1.0 / sumy;
+        for (int i = 1; i < NL; i++) y[i] = dmax(y[i], ymin), sumy += y[i];
+        sumy = 1.0 / sumy;
 #pragma unroll
-    for (int i = 1; i < NL; i++) y[i] *= sumy;
-    evalcv_dev<NS>(th, mc, y, q, d, rho, m_p);
+        for (int i = 1; i < NL; i++) y[i] *= sumy;
+        evalcv_dev<NS>(th, mc, y, q, d, rho, m_p);
 #pragma unroll
-    for (int i = 0; i < NL; i++) {
-        const double ascr = fabs(q[i]);
-        const double sb = 0.1 * epsmin * ascr - d[i];
-        const double scr2 = (sb >= 0.0 ? 1.0 : -1.0) * fabs(1.0 / y[i]);
-        const double scr1 = scr2 * d[i];
-        scrtch = dmax(-fabs(ascr - d[i]) * scr2, dmax(scr1, scrtch));
+        for (int i = 0; i < NL; i++) {
+            const double ascr = fabs(q[i]);
+            const double sb = 0.1 * epsmin * ascr - d[i];
+            const double scr2 = (sb >= 0.0 ? 1.0 : -1.0) * fabs(1.0 / y[i]);
+            const double scr1 = scr2 * d[i];
+            scrtch = dmax(-fabs(ascr - d[i]) * scr2, dmax(scr1, scrtch));
+        }
+        dt = dmin(sqreps / scrtch, dtg);
+        dtmin *= dt;
+        ts = 0.0, tn = 0.0;
+#pragma unroll
+        for (int i = 0; i < NL; i++) ys[i] = y[i], qs[i] = q[i], ds[i] = d[i];
+        passes = 0;
     }
-    double dt = dmin(sqreps / scrtch, dtg);
-    dtmin *= dt;
-    double ts = 0.0, tn = 0.0;
-#pragma unroll
-    for (int i = 0; i < NL; i++) ys[i] = y[i], qs[i] = q[i], ds[i] = d[i];
-    int passes = 0;
-    for (;;) {
+
+    // one predictor/corrector pass (ProductionRates.h:531-638): 1 = reached
+    // dtg, 0 = continue, -1 = safety bound
+    __device__ __forceinline__ int pass(const DevThermo *__restrict__ th, const DevMech *__restrict__ mc)
+    {
+        const double ymin = 1.0e-20, tfd = 1.0 + 1.0e-10, epscl = 1.0e4;
         if (++passes > CHEM_MAXSUB) return -1;
         // predictor
 #pragma unroll
@@ -182,7 +196,7 @@ __device__ int chemeq2_dev(const DevThermo *__restrict__ th, const DevMech *__re
         }
         eps = eps * epscl;
         if (eps < epsmax) {
-            if (dtg <= (tn * tfd)) return passes;
+            if (dtg <= (tn * tfd)) return 1;
         } else {
             tn = ts;
         }
@@ -194,13 +208,31 @@ __device__ int chemeq2_dev(const DevThermo *__restrict__ th, const DevMech *__re
         if (eps > epsmax) {
             dt = dmin(dt, 0.34 * dto);
             if (dt <= dtmin) epsmax *= 10.0;
-            continue; // back to the predictor from the same saved state
+            return 0; // back to the predictor from the same saved state
         }
         epsmax = 1.0;
         ts = tn;
 #pragma unroll
         for (int i = 0; i < NL; i++) ys[i] = y[i], qs[i] = q[i], ds[i] = d[i];
+        return 0;
     }
+};
+
+// Chemeq2 on one cell.  Returns predictor passes, or -1 at the safety bound.
+template <int NS>
+__device__ int chemeq2_dev(const DevThermo *__restrict__ th, const DevMech *__restrict__ mc, double *y, double dtg,
+                           double rho, double m_p)
+{
+    ChemLane<NS> L;
+#pragma unroll
+    for (int i = 0; i < NS + 1; i++) L.y[i] = y[i];
+    L.start(th, mc, dtg, rho, m_p);
+    int st;
+    while ((st = L.pass(th, mc)) == 0) {
+    }
+#pragma unroll
+    for (int i = 0; i < NS + 1; i++) y[i] = L.y[i];
+    return st < 0 ? -1 : L.passes;
 }
 
 // get_Cope (Mixing_device.h:145-154) at one temperature
@@ -262,6 +294,95 @@ __global__ void __launch_bounds__(256) k_chem(Geom g, const DevThermo *__restric
         atomicOr(&f.err[0], EBIT_CHEM);
         atomicMin(&f.err[1], (int)id);
     }
+}
+
+// Interior cell t (0 <= t < total) -> padded-grid id
+__device__ __forceinline__ long long chem_cell_id(const Geom &g, long long t)
+{
+    const int i = g.Bx + (int)(t % g.X_inner);
+    const int j = g.By + (int)((t / g.X_inner) % g.Y_inner);
+    const int k = g.Bz + (int)(t / ((long long)g.X_inner * g.Y_inner));
+    return (long long)g.Xmax * g.Ymax * k + (long long)g.Xmax * j + i;
+}
+
+// Work-queue driver: persistent lanes draining the interior cells.  Cells
+// of very different stiffness take from 1 to ~10^3 passes; with one lane per
+// cell a wave runs as long as its stiffest cell while the other lanes idle.
+// Here every lane that has finished a cell takes the next one from a device
+// counter (one atomic per wave, handed out by lane rank), so the lanes of a
+// wave stay busy with the same predictor/corrector code until the queue is
+// empty.  Each cell's arithmetic is exactly that of the per-cell driver.
+// Termination: the counter only grows and a wave stops once every lane has
+// drawn an index >= total and holds no cell; a pass is bounded by
+// CHEM_MAXSUB per cell.  The counter queue[0] must be zero at launch.
+template <int NS>
+__global__ void __launch_bounds__(64) k_chem_queue(Geom g, const DevThermo *__restrict__ th,
+                                                   const DevMech *__restrict__ mc, Fields f, int record)
+{
+    const long long total = (long long)g.X_inner * g.Y_inner * g.Z_inner;
+    const long long N = g.N;
+    const double dtc = f.dt[1];
+    const int lane = threadIdx.x & 63;
+    ChemLane<NS> L;
+    long long t = -1, id = 0;
+    double e0 = 0.0;
+    bool active = false, done = false;
+    for (;;) {
+        const bool need = !active && !done;
+        const unsigned long long want = __ballot(need);
+        if (want) {
+            const int cnt = __popcll(want);
+            const int leader = __ffsll((long long)want) - 1;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd((unsigned long long *)f.queue, (unsigned long long)cnt);
+            base = __shfl(base, leader);
+            if (need) {
+                const int rank = __popcll(want & ((1ull << lane) - 1ull));
+                t = (long long)(base + (unsigned long long)rank);
+                if (t < total) {
+                    id = chem_cell_id(g, t);
+                    L.y[0] = f.T[id];
+                    const double rho = f.U[id];
+                    const double rho1 = 1.0 / rho;
+                    L.y[NS] = 1.0;
+#pragma unroll
+                    for (int k = 0; k < NS - 1; k++) {
+                        L.y[1 + k] = f.U[(5 + k) * N + id] * rho1;
+                        L.y[NS] += -L.y[1 + k];
+                    }
+                    e0 = mix_e<NS>(th, L.y + 1, L.y[0]);
+                    L.start(th, mc, dtc, rho, f.p[id]);
+                    active = true;
+                } else {
+                    done = true;
+                }
+            }
+        }
+        if (!__any(active)) break;
+        if (active) {
+            const int st = L.pass(th, mc);
+            if (st != 0) {
+                const double rho = L.rho;
+                const double e1 = mix_e<NS>(th, L.y + 1, L.y[0]);
+#pragma unroll
+                for (int n = 0; n < NS - 1; n++) f.U[(5 + n) * N + id] = L.y[1 + n] * rho;
+                f.T[id] = L.y[0];
+                f.U[4 * N + id] += (e1 - e0) * rho;
+                if (record) f.substeps[t] = st < 0 ? -1 : L.passes;
+                if (st < 0) {
+                    atomicOr(&f.err[0], EBIT_CHEM);
+                    atomicMin(&f.err[1], (int)id);
+                }
+                active = false;
+            }
+        }
+    }
+}
+
+// zero the work-queue counter (a kernel, so the step graph holds no memset)
+__global__ void k_queue_reset(unsigned int *q)
+{
+    if (threadIdx.x < 4) q[threadIdx.x] = 0u;
 }
 
 } // namespace xf

@@ -29,11 +29,18 @@
 
 namespace xf {
 
-template <int DIR, int E> struct SweepShape {
-    // X: 64 faces x 4 pencils; Y/Z: 16 faces x 16 pencils (wave = 16 x 4)
-    static constexpr int TS = DIR == 0 ? 64 : 16;
+// Tile shapes of a 256-lane sweep block: TS faces along the sweep axis by
+// TW = 256 / TS pencils.  Longer tiles stage fewer halo cells per interior
+// cell; wider tiles read longer contiguous row segments (y and z sweeps).
+// x: 0 = 64 x 4, 1 = 32 x 8.  y, z: 0 = 16 x 16, 1 = 32 x 8, 2 = 8 x 32.
+template <int DIR> struct SweepShapes {
+    static constexpr int count = DIR == 0 ? 2 : 3;
+};
+template <int DIR, int E, int SHAPE> struct SweepShape {
+    static constexpr int TS = DIR == 0 ? (SHAPE == 0 ? 64 : 32) : (SHAPE == 0 ? 16 : (SHAPE == 1 ? 32 : 8));
     static constexpr int TW = 256 / TS;
     static constexpr int NC = (TS + 5) * TW; // staged cells
+    static_assert(NC >= 256, "the flux-divergence pass reuses the staging area for 256 face fluxes");
 };
 
 // staged-cell LDS index: lanes adjacent in the wave hit adjacent words
@@ -306,7 +313,7 @@ template <int NS> struct SweepOccupancy {
     static constexpr int waves = NS == 1 ? 4 : (NS <= 3 ? 3 : 2);
 };
 
-template <int NS, bool COP, int DIR, int SCHEME, int PREC>
+template <int NS, bool COP, int DIR, int SCHEME, int PREC, int SHAPE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SweepOccupancy<NS>::waves)))
 k_sweep(Geom g, const DevThermo *__restrict__ th, const double *__restrict__ Ug,
                                                Fields f, double ncop_gamma, int mode, int stage, const double *Ua,
@@ -314,7 +321,7 @@ k_sweep(Geom g, const DevThermo *__restrict__ th, const double *__restrict__ Ug,
 {
     constexpr int E = COP ? NS + 4 : 5;
     constexpr int NCOP = COP ? NS - 1 : 0;
-    using SS = SweepShape<DIR, E>;
+    using SS = SweepShape<DIR, E, SHAPE>;
     constexpr int TS = SS::TS, TW = SS::TW, NC = SS::NC;
     constexpr bool MIX = PREC == XF_MIXED;
     __shared__ double sU[E * NC];

@@ -172,6 +172,16 @@ def reactive_leg(label, case, over, steps, warmup=2, seed=1):
                         chem_driver=xf.XF_CHEM_QUEUE if drv == "queue" else xf.XF_CHEM_PER_CELL)
         ctx.upload(U, T)
         ctx.initialize()
+        # first chemistry call on the initial field: radicals start at the
+        # 1e-20 floor, so every cell takes many passes (the stiff start-up)
+        ctx.update_states(0)
+        dt0 = ctx.get_dt()
+        ctx.synchronize()
+        c0 = time.perf_counter()
+        sub0 = ctx.chemistry(0.5 * dt0, record=True)
+        c1 = time.perf_counter()
+        ctx.upload(U, T)
+        ctx.initialize()
         shapes, _ = ctx.autotune(reps=5)
         ctx.steps(warmup, graph=True)
         ctx.synchronize()
@@ -191,7 +201,10 @@ def reactive_leg(label, case, over, steps, warmup=2, seed=1):
             "tile_shapes": [int(x) for x in shapes],
             "kernels_ms": {k: v[0] for k, v in kt.items()},
             "chem_passes": {"min": int(sub.min()), "median": float(np.median(sub)),
-                            "p99": float(np.percentile(sub, 99)), "max": int(sub.max()), "mean": float(sub.mean())}}
+                            "p99": float(np.percentile(sub, 99)), "max": int(sub.max()), "mean": float(sub.mean())},
+            "first_chem_call": {"ms": (c1 - c0) * 1e3, "passes_min": int(sub0.min()),
+                                "passes_median": float(np.median(sub0)), "passes_max": int(sub0.max()),
+                                "passes_mean": float(sub0.mean())}}
     base = out["forms"]["plain/per-cell"]["value"]
     for k, v in out["forms"].items():
         v["speedup_vs_plain_per_cell"] = v["value"] / base
@@ -223,6 +236,86 @@ def reactive_accuracy(steps=20):
     return out
 
 
+def ensemble_leg(ncases, sides, steps, warmup, seed=7, log=None):
+    """Batched ensemble of independent reactive channel cases (H2/O2/Ar,
+    Ns=9, NR=19), grid side per case drawn by seed from `sides`, initial
+    temperature and equivalence ratio perturbed per case by its own seed.
+    Forms: every case stepped alone, one after another (plain FP64, per-cell
+    chemistry: what running the cases one by one costs), and the ensemble
+    (one hipGraph per step over all cases, one work-queue chemistry launch for
+    all cells) in the plain and the mixed form.  The mixed ensemble's error is
+    measured against the plain single-case runs of the smallest cases."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from parity import field_errors
+    rng = np.random.default_rng(seed)
+    picks = [int(v) for v in rng.choice(sides, size=ncases)]
+    cases = []
+    for k, n in enumerate(picks):
+        s = xf.Setup(xf.case_path("reactive-channel"), nx=n, ny=n)
+        U, T = s.initial("reactive-channel", seed=1000 + k)
+        cases.append((s, U, T))
+    cells = sum(s.desc.ninner for s, _, _ in cases)
+    out = {"workload": "ensemble of %d reactive channel cases, sides from %s by seed %d" % (ncases, sides, seed),
+           "cells": cells, "sides": picks, "steps": steps, "warmup": warmup, "forms": {}}
+
+    def say(msg):
+        if log:
+            log(msg)
+
+    # one after another
+    t_alone = 0.0
+    alone = {}
+    small = sorted(range(ncases), key=lambda k: picks[k])[:4]
+    for k, (s, U, T) in enumerate(cases):
+        ctx = s.context()
+        ctx.upload(U, T)
+        ctx.initialize()
+        ctx.steps(warmup, graph=True)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.steps_async(steps, graph=True)
+        ctx.synchronize()
+        t_alone += time.perf_counter() - t0
+        if k in small:
+            alone[k] = ctx.download(0)
+        ctx.close()
+    say("ensemble leg: cases one by one done")
+    out["forms"]["plain/one-by-one"] = {"value": cells * steps / t_alone / 1e6, "ms_per_step": t_alone * 1e3 / steps}
+    for prec in ("plain", "mixed"):
+        ctxs = []
+        for s, U, T in cases:
+            c = s.context(precision=xf.XF_MIXED if prec == "mixed" else xf.XF_PLAIN_FP64,
+                          chem_driver=xf.XF_CHEM_QUEUE)
+            c.upload(U, T)
+            c.initialize()
+            ctxs.append(c)
+        ens = xf.Ensemble(ctxs)
+        ens.steps(warmup, graph=True)
+        ens.synchronize()
+        t0 = time.perf_counter()
+        ens.steps(steps, graph=True)
+        ens.synchronize()
+        t1 = time.perf_counter()
+        res = {"value": cells * steps / (t1 - t0) / 1e6, "ms_per_step": (t1 - t0) * 1e3 / steps}
+        errs = []
+        for k in small:
+            Ug, Tg = ctxs[k].download(0)
+            s, _, _ = cases[k]
+            e = field_errors(s, alone[k][0], alone[k][1], Ug, Tg)
+            e["side"] = picks[k]
+            errs.append(e)
+        res["error_vs_single_case_fp64"] = errs
+        ens.close()
+        for c in ctxs:
+            c.close()
+        out["forms"]["%s/ensemble" % prec] = res
+        say("ensemble leg: %s ensemble done" % prec)
+    base = out["forms"]["plain/one-by-one"]["value"]
+    for v in out["forms"].values():
+        v["speedup_vs_one_by_one"] = v["value"] / base
+    return out
+
+
 def cpu_baseline(sample_steps):
     """Oracle (plain C, FP64, OpenMP) on the full 1024x512 grid for a few steps."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -244,6 +337,11 @@ def cpu_baseline(sample_steps):
             (sample_steps, NX, NY, dt)}
 
 
+def progress(msg):
+    """Progress to stderr (the JSON line on stdout stays the only output)."""
+    print("[bench] %s" % msg, file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -255,6 +353,11 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true", help="skip the reduced-grid check against the oracle")
     ap.add_argument("--no-reactive", action="store_true", help="skip the reactive workloads")
     ap.add_argument("--no-tune", action="store_true", help="keep the default sweep tile shapes")
+    ap.add_argument("--ensemble-cases", type=int, default=64, help="cases in the ensemble leg (0: skip)")
+    ap.add_argument("--ensemble-sides", default="128,256,512,1024", help="grid sides the ensemble draws from")
+    ap.add_argument("--ensemble-steps", type=int, default=2)
+    ap.add_argument("--reactive-only", type=int, default=-1, metavar="I",
+                    help="profiling aid: run only reactive workload I and print its JSON")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 1 GiB device copy (counter scaling under rocprofv3 --pmc)")
     ap.add_argument("--seed", type=int, default=1)
@@ -264,8 +367,14 @@ def main():
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.reactive_only >= 0:
+        print(json.dumps(reactive_leg(*REACTIVE[args.reactive_only][:3], steps=args.steps, warmup=args.warmup)))
+        return
     forms = ["plain", "mixed"] if args.form == "both" else [args.form]
-    res = {f: run(args, f) for f in forms}
+    res = {}
+    for f in forms:
+        res[f] = run(args, f)
+        progress("%s form timed" % f)
     head = forms[0]
     d, value, ms, kt, finite, _ = res[head]
     line = {
@@ -297,8 +406,12 @@ def main():
             line["forms"][f]["accuracy"] = accuracy_check(f)
     if not args.no_reactive:
         line["reactive"] = [reactive_leg(*r) for r in REACTIVE]
+        progress("reactive legs done")
         if not args.no_accuracy:
             line["reactive_accuracy"] = reactive_accuracy()
+    if args.ensemble_cases > 0:
+        sides = [int(v) for v in args.ensemble_sides.split(",")]
+        line["ensemble"] = ensemble_leg(args.ensemble_cases, sides, args.ensemble_steps, 1, log=progress)
     if args.copy_probe:
         line["copy_probe"] = copy_probe()
     if not args.no_cpu:

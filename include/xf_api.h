@@ -193,6 +193,23 @@ int xf_get_tile_shape(const xf_ctx *ctx, int *shape3);
    running GLF maxima unchanged; overwrites LU, U1 and the primitives, which
    every stage recomputes.  Call after xf_upload_state + xf_initialize. */
 int xf_autotune(xf_ctx *ctx, int reps, int *shape3, double *ms);
+/* ---- ensembles ------------------------------------------------------------
+   Independent cases (any grid sizes; one species set, mechanism and thermo
+   tables) advanced together.  One ensemble step is every case's own step
+   (dt, Strang Reaction, three RK stages, Reaction), with each case's stages on
+   its own stream and the chemistry of all cases in one work-queue launch at
+   each Reaction; xf_ensemble_steps(..., use_graph = 1) captures the whole
+   step as one hipGraph.  Every case's result is bit-identical to stepping
+   it alone.  The contexts stay owned by the caller and must outlive the
+   ensemble; read results with xf_download_state on each.  No counterpart in
+   the reference, which runs one case per process. */
+typedef struct xf_ensemble xf_ensemble;
+int xf_ensemble_create(xf_ctx *const *cases, int ncases, xf_ensemble **out);
+int xf_ensemble_steps(xf_ensemble *ens, int nsteps, int use_graph);
+/* Waits for every case; returns the first case's physical error code. */
+int xf_ensemble_synchronize(xf_ensemble *ens);
+long long xf_ensemble_cells(const xf_ensemble *ens);
+void xf_ensemble_destroy(xf_ensemble *ens);
 /* Evaluate the device exp and log (the fixed-sequence FP64 functions every
    kernel uses) on n host values; for bit-level checks against the CPU. */
 int xf_debug_math(const double *x, double *exp_out, double *log_out, int n);

@@ -127,6 +127,11 @@ _SIGS = {
     "xf_set_tile_shape": (C.c_int, [_V, C.c_int, C.c_int]),
     "xf_get_tile_shape": (C.c_int, [_V, _IP]),
     "xf_autotune": (C.c_int, [_V, C.c_int, _IP, _DP]),
+    "xf_ensemble_create": (C.c_int, [C.POINTER(_V), C.c_int, C.POINTER(_V)]),
+    "xf_ensemble_steps": (C.c_int, [_V, C.c_int, C.c_int]),
+    "xf_ensemble_synchronize": (C.c_int, [_V]),
+    "xf_ensemble_cells": (C.c_longlong, [_V]),
+    "xf_ensemble_destroy": (None, [_V]),
 }
 
 
@@ -413,3 +418,39 @@ def copy_probe(nbytes=1 << 30, reps=5):
     if st != 0:
         raise XfError(st, "xf_copy_probe")
     return ms.value
+
+
+class Ensemble:
+    """Several contexts stepped together (xf_ensemble): each case's stages on
+    its own stream, one chemistry launch for all cases per Reaction."""
+
+    def __init__(self, contexts):
+        self.contexts = list(contexts)
+        arr = (_V * len(self.contexts))(*[c._h for c in self.contexts])
+        h = _V()
+        st = lib().xf_ensemble_create(arr, len(self.contexts), C.byref(h))
+        if st:
+            raise XfError(st, "xf_ensemble_create")
+        self._h = h
+
+    @property
+    def cells(self):
+        return int(lib().xf_ensemble_cells(self._h))
+
+    def steps(self, n, graph=True):
+        st = lib().xf_ensemble_steps(self._h, n, 1 if graph else 0)
+        if st:
+            raise XfError(st, "xf_ensemble_steps")
+
+    def synchronize(self):
+        st = lib().xf_ensemble_synchronize(self._h)
+        if st:
+            raise XfError(st, "xf_ensemble_synchronize")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().xf_ensemble_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()

@@ -25,6 +25,10 @@ struct Ops {
                   double *Uo);
     void (*chem)(xf_ctx *, int record);
     void (*prepare)(xf_ctx *);
+    // shared queue over the cells of several cases (ensembles)
+    void (*chem_multi)(hipStream_t, const ChemCase *, int ncases, long long total, const DevThermo *,
+                       const DevMech *, unsigned long long *queue, int blocks, int record);
+    int (*multi_blocks)(long long total);
 };
 
 } // namespace
@@ -178,7 +182,28 @@ template <int NS, bool COP> struct Launch {
             c->chem_blocks = (int)std::max(1LL, std::min(nb, want));
         }
     }
-    static Ops ops() { return Ops{&states, &sweep, &chem, &prepare}; }
+    static void chem_multi(hipStream_t st, const ChemCase *cases, int ncases, long long total, const DevThermo *th,
+                           const DevMech *mc, unsigned long long *queue, int blocks, int record)
+    {
+        if constexpr (COP) {
+            hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, st, (unsigned int *)queue);
+            hipLaunchKernelGGL((k_chem_queue_multi<NS>), dim3(blocks), dim3(64), 0, st, cases, ncases, total, th, mc,
+                               queue, record);
+        }
+    }
+    static int multi_blocks(long long total)
+    {
+        int per_cu = 0, ncu = 0, dev = 0;
+        if constexpr (COP) {
+            (void)hipGetDevice(&dev);
+            (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_chem_queue_multi<NS>, 64, 0);
+        }
+        const long long want = (total + 63) / 64;
+        const long long nb = (long long)std::max(per_cu, 1) * std::max(ncu, 1);
+        return (int)std::max(1LL, std::min(nb, want));
+    }
+    static Ops ops() { return Ops{&states, &sweep, &chem, &prepare, &chem_multi, &multi_blocks}; }
 };
 
 bool select_ops(int NS, int cop, Ops &o)
@@ -286,18 +311,34 @@ void lu_async(xf_ctx *c, const double *Ug, int rk_stage, const double *Ua, const
     }
 }
 
+void reaction_prep_async(xf_ctx *c);
+
 void reaction_async(xf_ctx *c, int record)
 {
-    states_async(c, c->f.U, 0);
-    hipLaunchKernelGGL(k_dt_chem, dim3(1), dim3(64), 0, c->stream, c->f.dt, c->d.strang);
+    reaction_prep_async(c);
     c->ops.chem(c, record);
 }
 
-void step_async(xf_ctx *c)
+// XFLUIDS::Reaction before its chemistry launch: UpdateStates(U) and the
+// device-side chemistry step (dt/2 for Strang)
+void reaction_prep_async(xf_ctx *c)
+{
+    states_async(c, c->f.U, 0);
+    hipLaunchKernelGGL(k_dt_chem, dim3(1), dim3(64), 0, c->stream, c->f.dt, c->d.strang);
+}
+
+// A step is: dt; [Strang: Reaction]; three RK stages; [Reaction].  The parts
+// between chemistry launches are step_pre and step_mid, so that an ensemble
+// can launch one chemistry kernel for all its cases at each Reaction.
+void step_pre_async(xf_ctx *c)
+{
+    hipLaunchKernelGGL(k_dt, dim3(1), dim3(64), 0, c->stream, c->g, c->f.red, c->f.dt);
+    if (c->d.react && c->d.strang) reaction_prep_async(c);
+}
+
+void step_mid_async(xf_ctx *c)
 {
     Fields &f = c->f;
-    hipLaunchKernelGGL(k_dt, dim3(1), dim3(64), 0, c->stream, c->g, f.red, f.dt);
-    if (c->d.react && c->d.strang) reaction_async(c, 0);
     // stage 1: U -> U1 ; stage 2: (U, U1) -> U2 ; stage 3: (U, U2) -> U
     double *in[3] = {f.U, f.U1, c->U2};
     for (int s = 1; s <= 3; s++) {
@@ -307,7 +348,15 @@ void step_async(xf_ctx *c)
         double *Uo = (s == 1) ? f.U1 : (s == 2 ? c->U2 : f.U);
         lu_async(c, Ug, s, f.U, (s == 1) ? f.U1 : Ug, Uo);
     }
-    if (c->d.react) reaction_async(c, 0);
+    if (c->d.react) reaction_prep_async(c);
+}
+
+void step_async(xf_ctx *c)
+{
+    step_pre_async(c);
+    if (c->d.react && c->d.strang) c->ops.chem(c, 0);
+    step_mid_async(c);
+    if (c->d.react) c->ops.chem(c, 0);
 }
 
 } // namespace
@@ -462,6 +511,20 @@ int xf_set_mechanism(xf_ctx *c, const int *Nu_f, const int *Nu_b, const double *
         }
         m.third_ind[r] = (sum > 0.0) ? 1 : 0; // reaction.cpp:170-172
         for (int q = 0; q < 6; q++) m.Rargus[r * 6 + q] = Rargus[r * 6 + q];
+        DevRxn &x = m.rx[r];
+        x = DevRxn{};
+        x.A = Rargus[r * 6 + 0];
+        x.B = Rargus[r * 6 + 1];
+        x.EaR = Rargus[r * 6 + 2] * 4.184 / Ru;
+        x.third_ind = m.third_ind[r];
+        for (int j = 0; j < NS; j++) {
+            const int nf = m.Nu_f[r * NS + j], nb = m.Nu_b[r * NS + j], nd = m.Nu_d[r * NS + j];
+            if (nf < 0 || nf > 15 || nb < 0 || nb > 15 || nd < -8 || nd > 7) return XF_EINVAL;
+            if (nd != 0) x.md |= 1u << j, x.pd |= (unsigned long long)(nd + 8) << (4 * j);
+            if (m.third[r * NS + j] != 0.0) x.mt |= 1u << j;
+            if (nf != 0) x.mf |= 1u << j, x.pf |= (unsigned long long)nf << (4 * j);
+            if (nb != 0) x.mb |= 1u << j, x.pb |= (unsigned long long)nb << (4 * j);
+        }
     }
     XF_HIP(hipMemcpy(c->d_mc, &c->h_mc, sizeof(DevMech), hipMemcpyHostToDevice));
     c->mech_set = true;
@@ -925,6 +988,171 @@ int xf_debug_math(const double *x, double *exp_out, double *log_out, int n)
     if (e == hipSuccess) e = hipMemcpy(log_out, d + 2 * n, (size_t)n * sizeof(double), hipMemcpyDeviceToHost);
     (void)hipFree(d);
     return e == hipSuccess ? XF_OK : XF_EHIP;
+}
+
+// ---------------------------------------------------------------------------
+// Ensembles: independent cases advanced together, one graph launch per step
+// ---------------------------------------------------------------------------
+} // extern "C"
+
+struct xf_ensemble {
+    std::vector<xf_ctx *> cases;
+    hipStream_t stream = nullptr;
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join;
+    ChemCase *d_cases = nullptr;
+    unsigned long long *d_queue = nullptr;
+    long long total = 0;
+    int blocks = 0;
+    bool react = false, strang = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+};
+
+namespace {
+
+void ens_fork(xf_ensemble *e)
+{
+    (void)hipEventRecord(e->fork, e->stream);
+    for (xf_ctx *c : e->cases) (void)hipStreamWaitEvent(c->stream, e->fork, 0);
+}
+
+void ens_join(xf_ensemble *e)
+{
+    for (size_t k = 0; k < e->cases.size(); k++) {
+        (void)hipEventRecord(e->join[k], e->cases[k]->stream);
+        (void)hipStreamWaitEvent(e->stream, e->join[k], 0);
+    }
+}
+
+void ens_chem(xf_ensemble *e)
+{
+    xf_ctx *c0 = e->cases[0];
+    c0->ops.chem_multi(e->stream, e->d_cases, (int)e->cases.size(), e->total, c0->d_th, c0->d_mc, e->d_queue,
+                       e->blocks, 0);
+}
+
+// one step of every case: each case's stages on its own stream, the
+// chemistry of all cases in one work-queue launch at each Reaction
+void ens_step(xf_ensemble *e)
+{
+    ens_fork(e);
+    for (xf_ctx *c : e->cases) step_pre_async(c);
+    ens_join(e);
+    if (e->react && e->strang) ens_chem(e);
+    ens_fork(e);
+    for (xf_ctx *c : e->cases) step_mid_async(c);
+    ens_join(e);
+    if (e->react) ens_chem(e);
+}
+
+} // namespace
+
+extern "C" {
+
+int xf_ensemble_create(xf_ctx *const *cases, int ncases, xf_ensemble **out)
+{
+    if (!cases || ncases <= 0 || !out) return XF_EINVAL;
+    *out = nullptr;
+    xf_ctx *c0 = cases[0];
+    if (!c0) return XF_EINVAL;
+    for (int k = 0; k < ncases; k++) {
+        xf_ctx *c = cases[k];
+        if (!c || c->d.NS != c0->d.NS || c->d.cop != c0->d.cop || c->d.NR != c0->d.NR ||
+            c->d.react != c0->d.react || c->d.strang != c0->d.strang)
+            return XF_EINVAL;
+        if (c->d.cop && !c->thermo_set) return XF_EINVAL;
+        if (c->d.react && !c->mech_set) return XF_EINVAL;
+        // one chemistry launch serves every case: the tables must agree
+        if (c->d.react && (std::memcmp(&c->h_th, &c0->h_th, sizeof(DevThermo)) != 0 ||
+                           std::memcmp(&c->h_mc, &c0->h_mc, sizeof(DevMech)) != 0))
+            return XF_EINVAL;
+        for (int j = 0; j < k; j++)
+            if (cases[j] == c) return XF_EINVAL;
+    }
+    xf_ensemble *e = new xf_ensemble();
+    e->cases.assign(cases, cases + ncases);
+    e->react = c0->d.react != 0;
+    e->strang = c0->d.strang != 0;
+    auto fail = [&](int code) {
+        xf_ensemble_destroy(e);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) return fail(XF_EHIP);
+    if (hipEventCreateWithFlags(&e->fork, hipEventDisableTiming) != hipSuccess) return fail(XF_EHIP);
+    e->join.assign(ncases, nullptr);
+    for (auto &ev : e->join)
+        if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) return fail(XF_EHIP);
+    std::vector<ChemCase> h(ncases);
+    long long off = 0;
+    for (int k = 0; k < ncases; k++) {
+        xf_ctx *c = cases[k];
+        ChemCase &r = h[k];
+        r.off = off;
+        r.g = c->g;
+        r.U = c->f.U, r.T = c->f.T, r.p = c->f.p, r.dt = c->f.dt;
+        r.substeps = c->f.substeps, r.err = c->f.err;
+        off += c->Ninner;
+    }
+    e->total = off;
+    if (hipMalloc(&e->d_cases, ncases * sizeof(ChemCase)) != hipSuccess ||
+        hipMalloc(&e->d_queue, 2 * sizeof(unsigned long long)) != hipSuccess)
+        return fail(XF_EHIP);
+    if (hipMemcpy(e->d_cases, h.data(), ncases * sizeof(ChemCase), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(XF_EHIP);
+    unsigned long long z[2] = {0, 0};
+    if (hipMemcpy(e->d_queue, z, sizeof(z), hipMemcpyHostToDevice) != hipSuccess) return fail(XF_EHIP);
+    if (e->react) e->blocks = c0->ops.multi_blocks(e->total);
+    *out = e;
+    return XF_OK;
+}
+
+int xf_ensemble_steps(xf_ensemble *e, int nsteps, int use_graph)
+{
+    if (!e || nsteps < 0) return XF_EINVAL;
+    for (xf_ctx *c : e->cases) c->prof = false;
+    if (use_graph) {
+        if (!e->gexec) {
+            XF_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+            ens_step(e);
+            XF_HIP(hipStreamEndCapture(e->stream, &e->graph));
+            XF_HIP(hipGraphInstantiate(&e->gexec, e->graph, nullptr, nullptr, 0));
+        }
+        for (int s = 0; s < nsteps; s++) XF_HIP(hipGraphLaunch(e->gexec, e->stream));
+    } else {
+        for (int s = 0; s < nsteps; s++) ens_step(e);
+    }
+    return check_launch();
+}
+
+int xf_ensemble_synchronize(xf_ensemble *e)
+{
+    if (!e) return XF_EINVAL;
+    XF_HIP(hipStreamSynchronize(e->stream));
+    int first = XF_OK;
+    for (xf_ctx *c : e->cases) {
+        XF_HIP(hipStreamSynchronize(c->stream));
+        const int st = take_error(c);
+        if (st && !first) first = st;
+    }
+    return first;
+}
+
+long long xf_ensemble_cells(const xf_ensemble *e) { return e ? e->total : 0; }
+
+void xf_ensemble_destroy(xf_ensemble *e)
+{
+    if (!e) return;
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    if (e->fork) (void)hipEventDestroy(e->fork);
+    for (auto ev : e->join)
+        if (ev) (void)hipEventDestroy(ev);
+    if (e->d_cases) (void)hipFree(e->d_cases);
+    if (e->d_queue) (void)hipFree(e->d_queue);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
 }
 
 int xf_copy_probe(size_t nbytes, int reps, double *avg_ms)

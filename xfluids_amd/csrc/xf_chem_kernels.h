@@ -61,44 +61,42 @@ __device__ __forceinline__ void evalcv_dev(const DevThermo *__restrict__ th, con
     for (int n = 0; n < NS; n++) qi[n] = 0.0, pi[n] = 0.0;
     const int NR = mc->NR;
     for (int r = 0; r < NR; r++) {
-        const double A = mc->Rargus[r * 6 + 0], B = mc->Rargus[r * 6 + 1], Ea = mc->Rargus[r * 6 + 2];
-        double Kf = A * dexp(B * logT - Ea * 4.184 / Ru / m_tmp);
+        // uniform per-reaction record (scalar loads); species loops run over
+        // compile-time n and test the record's masks, which skips exactly the
+        // zero-coefficient terms (each adds +0 or multiplies by 1 in the
+        // reference, so skipping changes no bit)
+        const DevRxn &R = mc->rx[r];
+        double Kf = R.A * dexp(R.B * logT - R.EaR / m_tmp);
         double DeltaGibbs = 0.0;
         int m_dn = 0;
 #pragma unroll
         for (int n = 0; n < NS; n++) {
-            const int nu = mc->Nu_d[r * NS + n];
-            if (nu != 0) {
+            if ((R.md >> n) & 1u) {
+                const int nu = (int)((R.pd >> (4 * n)) & 15ull) - 8;
                 DeltaGibbs += nu * (logP - gk[n]);
                 m_dn += nu;
             }
         }
         double Kck = dmin(dexp(DeltaGibbs - m_dn * logStandConc), 1.0E+40);
         double tb = 1.0;
-        if (1 == mc->third_ind[r]) {
+        if (1 == R.third_ind) {
             tb = 0.0;
 #pragma unroll
-            for (int it = 0; it < NS; it++) {
-                const double cf = mc->third[r * NS + it];
-                if (cf != 0.0) tb += cf * m_cm[it];
-            }
+            for (int it = 0; it < NS; it++)
+                if ((R.mt >> it) & 1u) tb += mc->third[r * NS + it] * m_cm[it];
         }
         Kf *= tb;
         Kck *= Kf;
 #pragma unroll
-        for (int it = 0; it < NS; it++) {
-            const int nf = mc->Nu_f[r * NS + it];
-            if (nf != 0) Kf *= pown_i(m_cm[it], nf);
-        }
+        for (int it = 0; it < NS; it++)
+            if ((R.mf >> it) & 1u) Kf *= pown_i(m_cm[it], (int)((R.pf >> (4 * it)) & 15ull));
 #pragma unroll
-        for (int it = 0; it < NS; it++) {
-            const int nb = mc->Nu_b[r * NS + it];
-            if (nb != 0) Kck *= pown_i(m_cm[it], nb);
-        }
+        for (int it = 0; it < NS; it++)
+            if ((R.mb >> it) & 1u) Kck *= pown_i(m_cm[it], (int)((R.pb >> (4 * it)) & 15ull));
 #pragma unroll
         for (int n = 0; n < NS; n++) {
-            const int nf = mc->Nu_f[r * NS + n], nb = mc->Nu_b[r * NS + n];
-            if ((nf | nb) != 0) {
+            if (((R.mf | R.mb) >> n) & 1u) {
+                const int nf = (int)((R.pf >> (4 * n)) & 15ull), nb = (int)((R.pb >> (4 * n)) & 15ull);
                 qi[n] += nb * Kf + nf * Kck;
                 pi[n] += nf * Kf + nb * Kck;
             }
@@ -121,51 +119,42 @@ __device__ __forceinline__ double alpha_of(double r)
     return (180.0 + r * (60.0 + r * (11.0 + r))) / (360.0 + r * (60.0 + r * (12.0 + r)));
 }
 
-// Chemeq2 state of one cell, advanced one predictor/corrector pass at a time
-// so that a lane can also work through a queue of cells (k_chem_queue).
+// Chemeq2 state of one cell, advanced one rate evaluation at a time so that
+// a lane can also work through a queue of cells (k_chem_queue).  Every
+// advance is: before() -> evalcv_dev(y) -> after().  The first advance of a
+// cell is the start-up (clip and renormalise Y, initial step from the rates,
+// ProductionRates.h:488-530); each later one is a predictor/corrector pass
+// (ProductionRates.h:531-638).  The sequence of operations per cell is the
+// reference's; only the place of the rate evaluation in the code is shared.
 template <int NS> struct ChemLane {
     static constexpr int NL = NS + 1;
     double y[NL], ys[NL], qs[NL], ds[NL], q[NL], d[NL], rtau[NL];
     double dt, dtg, dtmin, epsmax, ts, tn, rho, m_p;
     int passes;
+    bool started;
 
-    // y = [T, Y_1..Y_NS] on entry: clip and renormalise Y, first rate
-    // evaluation, initial step (ProductionRates.h:488-530)
-    __device__ __forceinline__ void start(const DevThermo *__restrict__ th, const DevMech *__restrict__ mc,
-                                          double dtg_, double rho_, double p_)
+    __device__ __forceinline__ void begin(double dtg_, double rho_, double p_)
     {
-        const double ymin = 1.0e-20, epsmin = 1.0e-4, sqreps = 0.05;
         dtg = dtg_, rho = rho_, m_p = p_;
         dtmin = 1.0e-7, epsmax = 1.0;
-        double scrtch = 1e-25;
-        double sumy = 0.0;
-#pragma unroll
-        for (int i = 1; i < NL; i++) y[i] = dmax(y[i], ymin), sumy += y[i];
-        sumy = 1.0 / sumy;
-#pragma unroll
-        for (int i = 1; i < NL; i++) y[i] *= sumy;
-        evalcv_dev<NS>(th, mc, y, q, d, rho, m_p);
-#pragma unroll
-        for (int i = 0; i < NL; i++) {
-            const double ascr = fabs(q[i]);
-            const double sb = 0.1 * epsmin * ascr - d[i];
-            const double scr2 = (sb >= 0.0 ? 1.0 : -1.0) * fabs(1.0 / y[i]);
-            const double scr1 = scr2 * d[i];
-            scrtch = dmax(-fabs(ascr - d[i]) * scr2, dmax(scr1, scrtch));
-        }
-        dt = dmin(sqreps / scrtch, dtg);
-        dtmin *= dt;
-        ts = 0.0, tn = 0.0;
-#pragma unroll
-        for (int i = 0; i < NL; i++) ys[i] = y[i], qs[i] = q[i], ds[i] = d[i];
+        started = false;
         passes = 0;
     }
 
-    // one predictor/corrector pass (ProductionRates.h:531-638): 1 = reached
-    // dtg, 0 = continue, -1 = safety bound
-    __device__ __forceinline__ int pass(const DevThermo *__restrict__ th, const DevMech *__restrict__ mc)
+    // returns -1 when the pass count reaches the safety bound (no evaluation
+    // is needed then), 0 otherwise
+    __device__ __forceinline__ int before()
     {
-        const double ymin = 1.0e-20, tfd = 1.0 + 1.0e-10, epscl = 1.0e4;
+        const double ymin = 1.0e-20;
+        if (!started) {
+            double sumy = 0.0;
+#pragma unroll
+            for (int i = 1; i < NL; i++) y[i] = dmax(y[i], ymin), sumy += y[i];
+            sumy = 1.0 / sumy;
+#pragma unroll
+            for (int i = 1; i < NL; i++) y[i] *= sumy;
+            return 0;
+        }
         if (++passes > CHEM_MAXSUB) return -1;
         // predictor
 #pragma unroll
@@ -176,7 +165,32 @@ template <int NS> struct ChemLane {
             y[i] = dmax(ys[i] + scr, ymin);
         }
         tn = ts + dt;
-        evalcv_dev<NS>(th, mc, y, q, d, rho, m_p);
+        return 0;
+    }
+
+    // after the rate evaluation at y: 1 = reached dtg, 0 = continue
+    __device__ __forceinline__ int after()
+    {
+        const double ymin = 1.0e-20, tfd = 1.0 + 1.0e-10, epscl = 1.0e4;
+        if (!started) {
+            const double epsmin = 1.0e-4, sqreps = 0.05;
+            double scrtch = 1e-25;
+#pragma unroll
+            for (int i = 0; i < NL; i++) {
+                const double ascr = fabs(q[i]);
+                const double sb = 0.1 * epsmin * ascr - d[i];
+                const double scr2 = (sb >= 0.0 ? 1.0 : -1.0) * fabs(1.0 / y[i]);
+                const double scr1 = scr2 * d[i];
+                scrtch = dmax(-fabs(ascr - d[i]) * scr2, dmax(scr1, scrtch));
+            }
+            dt = dmin(sqreps / scrtch, dtg);
+            dtmin *= dt;
+            ts = 0.0, tn = 0.0;
+#pragma unroll
+            for (int i = 0; i < NL; i++) ys[i] = y[i], qs[i] = q[i], ds[i] = d[i];
+            started = true;
+            return 0;
+        }
         // corrector fused with the convergence test
         double eps = 1e-10;
 #pragma unroll
@@ -216,6 +230,14 @@ template <int NS> struct ChemLane {
         for (int i = 0; i < NL; i++) ys[i] = y[i], qs[i] = q[i], ds[i] = d[i];
         return 0;
     }
+
+    // one advance: 1 = done, 0 = continue, -1 = safety bound
+    __device__ __forceinline__ int advance(const DevThermo *__restrict__ th, const DevMech *__restrict__ mc)
+    {
+        if (before() < 0) return -1;
+        evalcv_dev<NS>(th, mc, y, q, d, rho, m_p);
+        return after();
+    }
 };
 
 // Chemeq2 on one cell.  Returns predictor passes, or -1 at the safety bound.
@@ -226,9 +248,9 @@ __device__ int chemeq2_dev(const DevThermo *__restrict__ th, const DevMech *__re
     ChemLane<NS> L;
 #pragma unroll
     for (int i = 0; i < NS + 1; i++) L.y[i] = y[i];
-    L.start(th, mc, dtg, rho, m_p);
+    L.begin(dtg, rho, m_p);
     int st;
-    while ((st = L.pass(th, mc)) == 0) {
+    while ((st = L.advance(th, mc)) == 0) {
     }
 #pragma unroll
     for (int i = 0; i < NS + 1; i++) y[i] = L.y[i];
@@ -305,26 +327,35 @@ __device__ __forceinline__ long long chem_cell_id(const Geom &g, long long t)
     return (long long)g.Xmax * g.Ymax * k + (long long)g.Xmax * j + i;
 }
 
-// Work-queue driver: persistent lanes draining the interior cells.  Cells
-// of very different stiffness take from 1 to ~10^3 passes; with one lane per
-// cell a wave runs as long as its stiffest cell while the other lanes idle.
-// Here every lane that has finished a cell takes the next one from a device
-// counter (one atomic per wave, handed out by lane rank), so the lanes of a
-// wave stay busy with the same predictor/corrector code until the queue is
-// empty.  Each cell's arithmetic is exactly that of the per-cell driver.
-// Termination: the counter only grows and a wave stops once every lane has
-// drawn an index >= total and holds no cell; a pass is bounded by
-// CHEM_MAXSUB per cell.  The counter queue[0] must be zero at launch.
-template <int NS>
-__global__ void __launch_bounds__(64) k_chem_queue(Geom g, const DevThermo *__restrict__ th,
-                                                   const DevMech *__restrict__ mc, Fields f, int record)
+// The part of one case the chemistry step touches.  `off` is the index of
+// the case's first interior cell in the concatenated cell list of an
+// ensemble (0 for a single case).
+struct ChemCase {
+    long long off;
+    Geom g;
+    double *U, *T, *p, *dt;
+    int *substeps, *err;
+};
+
+// Work-queue driver: persistent lanes draining the interior cells of one or
+// several cases.  Cells of very different stiffness take from 1 to ~10^3
+// passes; with one lane per cell a wave runs as long as its stiffest cell
+// while the other lanes idle.  Here every lane that has finished a cell takes
+// the next one from a device counter (one atomic per wave, handed out by lane
+// rank), so the lanes of a wave stay busy with the same code until the queue
+// is empty; a cell that starts and the cells that continue share one rate
+// evaluation per iteration.  Each cell's arithmetic is exactly that of the
+// per-cell driver.  Termination: the counter only grows, a wave stops once
+// every lane has drawn an index >= total and holds no cell, and a cell is
+// bounded by CHEM_MAXSUB passes.  The counter must be zero at launch.
+template <int NS, class Cases>
+__device__ __forceinline__ void chem_queue_run(const Cases &cases, long long total, const DevThermo *__restrict__ th,
+                                               const DevMech *__restrict__ mc, unsigned long long *queue, int record)
 {
-    const long long total = (long long)g.X_inner * g.Y_inner * g.Z_inner;
-    const long long N = g.N;
-    const double dtc = f.dt[1];
     const int lane = threadIdx.x & 63;
     ChemLane<NS> L;
-    long long t = -1, id = 0;
+    ChemCase cc{};
+    long long tl = -1, id = 0;
     double e0 = 0.0;
     bool active = false, done = false;
     for (;;) {
@@ -334,24 +365,27 @@ __global__ void __launch_bounds__(64) k_chem_queue(Geom g, const DevThermo *__re
             const int cnt = __popcll(want);
             const int leader = __ffsll((long long)want) - 1;
             unsigned long long base = 0;
-            if (lane == leader) base = atomicAdd((unsigned long long *)f.queue, (unsigned long long)cnt);
+            if (lane == leader) base = atomicAdd(queue, (unsigned long long)cnt);
             base = __shfl(base, leader);
             if (need) {
                 const int rank = __popcll(want & ((1ull << lane) - 1ull));
-                t = (long long)(base + (unsigned long long)rank);
+                const long long t = (long long)(base + (unsigned long long)rank);
                 if (t < total) {
-                    id = chem_cell_id(g, t);
-                    L.y[0] = f.T[id];
-                    const double rho = f.U[id];
+                    cc = cases.find(t);
+                    tl = t - cc.off;
+                    const long long N = cc.g.N;
+                    id = chem_cell_id(cc.g, tl);
+                    L.y[0] = cc.T[id];
+                    const double rho = cc.U[id];
                     const double rho1 = 1.0 / rho;
                     L.y[NS] = 1.0;
 #pragma unroll
                     for (int k = 0; k < NS - 1; k++) {
-                        L.y[1 + k] = f.U[(5 + k) * N + id] * rho1;
+                        L.y[1 + k] = cc.U[(5 + k) * N + id] * rho1;
                         L.y[NS] += -L.y[1 + k];
                     }
                     e0 = mix_e<NS>(th, L.y + 1, L.y[0]);
-                    L.start(th, mc, dtc, rho, f.p[id]);
+                    L.begin(cc.dt[1], rho, cc.p[id]);
                     active = true;
                 } else {
                     done = true;
@@ -360,23 +394,74 @@ __global__ void __launch_bounds__(64) k_chem_queue(Geom g, const DevThermo *__re
         }
         if (!__any(active)) break;
         if (active) {
-            const int st = L.pass(th, mc);
+            // starting cells and running cells share this one evaluation
+            const int st = L.advance(th, mc);
             if (st != 0) {
+                const long long N = cc.g.N;
                 const double rho = L.rho;
                 const double e1 = mix_e<NS>(th, L.y + 1, L.y[0]);
 #pragma unroll
-                for (int n = 0; n < NS - 1; n++) f.U[(5 + n) * N + id] = L.y[1 + n] * rho;
-                f.T[id] = L.y[0];
-                f.U[4 * N + id] += (e1 - e0) * rho;
-                if (record) f.substeps[t] = st < 0 ? -1 : L.passes;
+                for (int n = 0; n < NS - 1; n++) cc.U[(5 + n) * N + id] = L.y[1 + n] * rho;
+                cc.T[id] = L.y[0];
+                cc.U[4 * N + id] += (e1 - e0) * rho;
+                if (record) cc.substeps[tl] = st < 0 ? -1 : L.passes;
                 if (st < 0) {
-                    atomicOr(&f.err[0], EBIT_CHEM);
-                    atomicMin(&f.err[1], (int)id);
+                    atomicOr(&cc.err[0], EBIT_CHEM);
+                    atomicMin(&cc.err[1], (int)id);
                 }
                 active = false;
             }
         }
     }
+}
+
+struct OneCase {
+    ChemCase c;
+    __device__ __forceinline__ ChemCase find(long long) const { return c; }
+};
+
+// cases[k].off ascending; the case holding cell t by binary search
+struct ManyCases {
+    const ChemCase *cases;
+    int n;
+    __device__ __forceinline__ ChemCase find(long long t) const
+    {
+        int lo = 0, hi = n - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (cases[mid].off <= t) lo = mid;
+            else hi = mid - 1;
+        }
+        return cases[lo];
+    }
+};
+
+// two waves per SIMD for the 9-species state (<= 256 registers per lane)
+template <int NS> struct ChemOccupancy {
+    static constexpr int waves = NS <= 3 ? 3 : 2;
+};
+
+template <int NS>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ChemOccupancy<NS>::waves)))
+k_chem_queue(Geom g, const DevThermo *__restrict__ th, const DevMech *__restrict__ mc, Fields f, int record)
+{
+    OneCase one;
+    one.c.off = 0;
+    one.c.g = g;
+    one.c.U = f.U, one.c.T = f.T, one.c.p = f.p, one.c.dt = f.dt;
+    one.c.substeps = f.substeps, one.c.err = f.err;
+    chem_queue_run<NS>(one, (long long)g.X_inner * g.Y_inner * g.Z_inner, th, mc, (unsigned long long *)f.queue,
+                       record);
+}
+
+// one queue over the cells of every case of an ensemble (shared mechanism)
+template <int NS>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ChemOccupancy<NS>::waves)))
+k_chem_queue_multi(const ChemCase *__restrict__ cases, int ncases, long long total, const DevThermo *__restrict__ th,
+                   const DevMech *__restrict__ mc, unsigned long long *queue, int record)
+{
+    ManyCases many{cases, ncases};
+    chem_queue_run<NS>(many, total, th, mc, queue, record);
 }
 
 // zero the work-queue counter (a kernel, so the step graph holds no memset)

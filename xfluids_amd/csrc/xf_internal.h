@@ -20,13 +20,27 @@ struct DevThermo {
     int thermo; // 1 NASA-9, 0 JANAF
 };
 
+// One reaction as the rate kernel reads it: Arrhenius A, B and
+// Ea * 4.184 / Ru (the reference's left-to-right product, hoisted), bit masks
+// of the species with non-zero Nu_d / third-body efficiency / Nu_f / Nu_b,
+// and those coefficients packed in 4-bit fields (Nu_d stored + 8).  One
+// scalar load of 64 bytes per reaction replaces NS table reads per list.
+struct DevRxn {
+    double A, B, EaR;
+    unsigned md, mt, mf, mb;
+    unsigned long long pd, pf, pb;
+    int third_ind, pad;
+};
+
 // Reaction tables (reference layout, src/include/global_setup.h:210-215)
+// plus the per-reaction records built from them
 struct DevMech {
     double Rargus[MAXNR * 6];
     double third[MAXNR * MAXNS];
     int Nu_f[MAXNR * MAXNS], Nu_b[MAXNR * MAXNS], Nu_d[MAXNR * MAXNS];
     int third_ind[MAXNR];
     int NR;
+    DevRxn rx[MAXNR];
 };
 
 // Geometry passed by value to every kernel

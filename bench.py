@@ -236,6 +236,44 @@ def reactive_accuracy(steps=20):
     return out
 
 
+def mixing_layer_leg(n, steps, warmup, log=None):
+    """3-D reacting mixing layer, H2/N2 (600 K) under air (1200 K), velocity
+    ratio 3, Ns=9 NR=19 (H2O-N2 mechanism), n^3 cells, periodic in x and z.
+    Plain FP64 with per-cell chemistry vs the mixed form with the work-queue
+    chemistry; sweep tile shapes autotuned per direction, steps replayed as
+    hipGraphs.  Inviscid: the mixture-averaged viscous and diffusive fluxes of
+    the full physics are not part of this build."""
+    s = xf.Setup(xf.case_path("mixing-layer"), nx=n, ny=n, nz=n)
+    d = s.desc
+    U, T = s.initial("mixing-layer", seed=11)
+    cells = d.X_inner * d.Y_inner * d.Z_inner
+    out = {"workload": "3-D reacting mixing layer %d^3, Ns=9 NR=19, inviscid" % n, "cells": cells,
+           "steps": steps, "warmup": warmup, "forms": {}}
+    for prec, drv in (("plain", "per-cell"), ("mixed", "queue")):
+        ctx = s.context(precision=xf.XF_MIXED if prec == "mixed" else xf.XF_PLAIN_FP64,
+                        chem_driver=xf.XF_CHEM_QUEUE if drv == "queue" else xf.XF_CHEM_PER_CELL)
+        ctx.upload(U, T)
+        ctx.initialize()
+        shapes, ms = ctx.autotune(reps=3)
+        ctx.steps(warmup, graph=True)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.steps_async(steps, graph=True)
+        ctx.synchronize()
+        t1 = time.perf_counter()
+        kt = {k: ctx.time_kernel(k, 5) for k in ("states", "sweep_x", "sweep_y", "sweep_z")}
+        ctx.close()
+        out["forms"]["%s/%s" % (prec, drv)] = {
+            "value": cells * steps / (t1 - t0) / 1e6, "ms_per_step": (t1 - t0) * 1e3 / steps,
+            "tile_shapes": [int(x) for x in shapes], "autotune_ms": ms.tolist(), "kernels_ms": kt}
+        if log:
+            log("mixing layer: %s/%s done" % (prec, drv))
+    base = out["forms"]["plain/per-cell"]["value"]
+    for v in out["forms"].values():
+        v["speedup_vs_plain_per_cell"] = v["value"] / base
+    return out
+
+
 def ensemble_leg(ncases, sides, steps, warmup, seed=7, log=None):
     """Batched ensemble of independent reactive channel cases (H2/O2/Ar,
     Ns=9, NR=19), grid side per case drawn by seed from `sides`, initial
@@ -353,6 +391,7 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true", help="skip the reduced-grid check against the oracle")
     ap.add_argument("--no-reactive", action="store_true", help="skip the reactive workloads")
     ap.add_argument("--no-tune", action="store_true", help="keep the default sweep tile shapes")
+    ap.add_argument("--mixing-layer", type=int, default=256, metavar="N", help="3-D mixing layer N^3 (0: skip)")
     ap.add_argument("--ensemble-cases", type=int, default=64, help="cases in the ensemble leg (0: skip)")
     ap.add_argument("--ensemble-sides", default="128,256,512,1024", help="grid sides the ensemble draws from")
     ap.add_argument("--ensemble-steps", type=int, default=2)
@@ -409,6 +448,8 @@ def main():
         progress("reactive legs done")
         if not args.no_accuracy:
             line["reactive_accuracy"] = reactive_accuracy()
+    if args.mixing_layer > 0:
+        line["mixing_layer"] = mixing_layer_leg(args.mixing_layer, 2, 1, log=progress)
     if args.ensemble_cases > 0:
         sides = [int(v) for v in args.ensemble_sides.split(",")]
         line["ensemble"] = ensemble_leg(args.ensemble_cases, sides, args.ensemble_steps, 1, log=progress)

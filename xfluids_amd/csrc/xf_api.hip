@@ -424,17 +424,16 @@ int xf_create(const xf_case_desc *desc, xf_ctx **out)
         hipMalloc(&f.substeps, (size_t)c->Ninner * sizeof(int)) || hipMalloc(&f.queue, 4 * sizeof(unsigned)) ||
         hipMalloc(&c->d_th, sizeof(DevThermo)) || hipMalloc(&c->d_mc, sizeof(DevMech)))
         return fail(XF_EHIP);
-    // zero everything once (eigen_block starts at 0, src/Fluids.cpp:427)
-    std::vector<char> zeros(std::max(nE, (size_t)d.NS * n1), 0);
-    if (hipMemcpy(f.U, zeros.data(), nE, hipMemcpyHostToDevice) || hipMemcpy(f.U1, zeros.data(), nE, hipMemcpyHostToDevice) ||
-        hipMemcpy(f.LU, zeros.data(), nE, hipMemcpyHostToDevice) || hipMemcpy(c->U2, zeros.data(), nE, hipMemcpyHostToDevice))
+    // zero everything once (eigen_block starts at 0, src/Fluids.cpp:427);
+    // plain memsets at creation, never inside a captured step
+    if (hipMemset(f.U, 0, nE) || hipMemset(f.U1, 0, nE) || hipMemset(f.LU, 0, nE) || hipMemset(c->U2, 0, nE))
         return fail(XF_EHIP);
     for (auto pp : dbl)
-        if (hipMemcpy(*pp, zeros.data(), n1, hipMemcpyHostToDevice)) return fail(XF_EHIP);
-    if (hipMemcpy(f.hi, zeros.data(), (size_t)d.NS * n1, hipMemcpyHostToDevice) ||
-        hipMemcpy(f.red, zeros.data(), RED_SIZE * sizeof(double), hipMemcpyHostToDevice) ||
-        hipMemcpy(f.dt, zeros.data(), 4 * sizeof(double), hipMemcpyHostToDevice))
+        if (hipMemset(*pp, 0, n1)) return fail(XF_EHIP);
+    if (hipMemset(f.hi, 0, (size_t)d.NS * n1) || hipMemset(f.red, 0, RED_SIZE * sizeof(double)) ||
+        hipMemset(f.dt, 0, 4 * sizeof(double)) || hipMemset(f.substeps, 0, (size_t)c->Ninner * sizeof(int)))
         return fail(XF_EHIP);
+    if (hipDeviceSynchronize()) return fail(XF_EHIP);
     int e0[2] = {0, 0x7fffffff};
     if (hipMemcpy(f.err, e0, sizeof(e0), hipMemcpyHostToDevice)) return fail(XF_EHIP);
     c->h_th.thermo = d.thermo;

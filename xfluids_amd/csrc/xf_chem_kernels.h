@@ -39,17 +39,28 @@ __device__ __forceinline__ void evalcv_dev(const DevThermo *__restrict__ th, con
 {
     const double m_tmp = y[0];
     const double _dens = 1.0 / m_dens;
-    // species Gibbs energy and internal-energy ingredients at T (Ri = 1)
+    // species Gibbs energy and internal-energy ingredients at T (Ri = 1).
+    // When every lane of the wave uses the same fit range (the usual case),
+    // the coefficient reads take that range as a wave-uniform value: scalar
+    // loads, shared by the wave, instead of one vector load per lane.
     const TPre t = th->thermo ? tpre<true>(m_tmp, 1) : tpre<true>(m_tmp, 0);
+    const int r0 = __builtin_amdgcn_readfirstlane(t.r);
+    const bool uni = __all(t.r == r0);
+    TPre tu = t;
+    tu.r = r0;
     double h1[NS], gk[NS];
+    auto species = [&](const TPre &tt) {
 #pragma unroll
-    for (int n = 0; n < NS; n++) {
-        h1[n] = h_k(th, t, 1.0, n);
-        double temp = 0.0;
-        temp += s_k(th, t, 1.0, n);
-        temp += (-h1[n] / m_tmp);
-        gk[n] = temp;
-    }
+        for (int n = 0; n < NS; n++) {
+            h1[n] = h_k(th, tt, 1.0, n);
+            double temp = 0.0;
+            temp += s_k(th, tt, 1.0, n);
+            temp += (-h1[n] / m_tmp);
+            gk[n] = temp;
+        }
+    };
+    if (uni) species(tu);
+    else species(t);
     const double logStandConc = dlog(m_p / (R0 * m_tmp));
     const double logT = dlog(m_tmp);
     const double logP = dlog(m_p * (1.0 / 101325.0));
@@ -110,7 +121,10 @@ __device__ __forceinline__ void evalcv_dev(const DevThermo *__restrict__ th, con
         const double ei = (h1[n] - m_tmp) * th->Ri[n];
         q0 -= qi[n] * ei, d0 -= pi[n] * ei;
     }
-    const double _Cv = 1.0 / mix_cv<NS>(th, y + 1, t);
+    double cv;
+    if (uni) cv = mix_cv<NS>(th, y + 1, tu);
+    else cv = mix_cv<NS>(th, y + 1, t);
+    const double _Cv = 1.0 / cv;
     q[0] = q0 * _Cv, d[0] = d0 * _Cv;
 }
 

@@ -46,6 +46,25 @@ int xf_setup_context(const xf_setup *s, int precision, int chem_driver, xf_ctx *
    [Reaction].  Call xf_initialize first (main.cpp:44-47). */
 int xf_evolve_staged(xf_ctx *ctx, int nsteps, double *dt_out);
 
+/* ---- checkpoints ----------------------------------------------------------
+   The reference's checkpoint file (XFLUIDS::Output_Ubak / Read_Ubak,
+   src/XFLUIDS.cpp:658-724): int Step, double Time, float Time_consumption
+   (4 + 8 + 4 bytes, no padding), then the conserved state with ghost cells
+   as the device holds it in the reference, AoS U[Emax*id + n] over
+   Xmax*Ymax*Zmax cells (Emax*Xmax*Ymax*Zmax doubles, iniset.cpp:345).  The
+   temperature is not part of the format: a restart starts the Newton
+   inversion from the caller's T.  Host-only functions; the file size must
+   match the descriptor exactly. */
+int xf_checkpoint_write(const char *path, const xf_case_desc *desc, const double *U_aos, int step, double time,
+                        float time_consumption);
+int xf_checkpoint_read(const char *path, const xf_case_desc *desc, double *U_aos, int *step, double *time,
+                       float *time_consumption);
+/* CopyToUbak + Output_Ubak / Read_Ubak + CopyToU for a context: download U and
+   write it; read and upload it (which = 0, so U1 = U as after
+   initialisation), keeping the context's current T as the Newton guess. */
+int xf_save_checkpoint(xf_ctx *ctx, const char *path, int step, double time, float time_consumption);
+int xf_load_checkpoint(xf_ctx *ctx, const char *path, int *step, double *time, float *time_consumption);
+
 #ifdef __cplusplus
 }
 #endif

@@ -130,3 +130,31 @@ def test_runtime_subset_mechanisms_match_upstream(mech):
     a = open(os.path.join(xf.RUNTIME_DAT, "Reaction", mech, "reaction_list.dat")).read()
     b = open(os.path.join(UPSTREAM, "runtime.dat", "Reaction", mech, "reaction_list.dat")).read()
     assert a == b
+
+
+def test_checkpoint_format_round_trip(tmp_path):
+    """The reference's checkpoint layout: int Step, double Time, float
+    Time_consumption, then Emax*Xmax*Ymax*Zmax doubles (AoS, ghosts included)."""
+    import struct
+    for case, over in (("shock-bubble", dict(nx=24, ny=16)), ("isentropic-vortex", dict(nx=12, ny=10))):
+        s = xf.Setup(xf.case_path(case), **over)
+        d = s.desc
+        E = d.NS + 4 if d.cop else 5
+        n = E * d.Xmax * d.Ymax * d.Zmax
+        U = np.random.default_rng(5).normal(size=n)
+        p = tmp_path / ("ck_" + case)
+        xf.checkpoint_write(p, d, U, 1234, 3.25e-4, 17.5)
+        raw = p.read_bytes()
+        assert len(raw) == 4 + 8 + 4 + 8 * n
+        step, tm, tc = struct.unpack("<idf", raw[:16])
+        assert (step, tm, tc) == (1234, 3.25e-4, 17.5)
+        assert np.array_equal(np.frombuffer(raw[16:], dtype="<f8"), U)
+        U2, step2, tm2, tc2 = xf.checkpoint_read(p, d)
+        assert np.array_equal(U2.view(np.uint64), U.view(np.uint64)) and (step2, tm2, tc2) == (1234, 3.25e-4, 17.5)
+        # a file for another geometry is refused
+        p.write_bytes(raw + b"\0")
+        with pytest.raises(xf.XfError):
+            xf.checkpoint_read(p, d)
+        p.write_bytes(raw[:-8])
+        with pytest.raises(xf.XfError):
+            xf.checkpoint_read(p, d)

@@ -122,6 +122,10 @@ _SIGS = {
     "xf_setup_initial": (C.c_int, [_V, C.c_char_p, C.c_ulonglong, _DP, _DP]),
     "xf_setup_context": (C.c_int, [_V, C.c_int, C.c_int, C.POINTER(_V)]),
     "xf_evolve_staged": (C.c_int, [_V, C.c_int, _DP]),
+    "xf_checkpoint_write": (C.c_int, [C.c_char_p, _V, _DP, C.c_int, C.c_double, C.c_float]),
+    "xf_checkpoint_read": (C.c_int, [C.c_char_p, _V, _DP, _IP, _DP, C.POINTER(C.c_float)]),
+    "xf_save_checkpoint": (C.c_int, [_V, C.c_char_p, C.c_int, C.c_double, C.c_float]),
+    "xf_load_checkpoint": (C.c_int, [_V, C.c_char_p, _IP, _DP, C.POINTER(C.c_float)]),
     "xf_debug_math": (C.c_int, [_DP, _DP, _DP, C.c_int]),
     "xf_copy_probe": (C.c_int, [C.c_size_t, C.c_int, _DP]),
     "xf_set_tile_shape": (C.c_int, [_V, C.c_int, C.c_int]),
@@ -392,6 +396,18 @@ class Context:
         self._chk(lib().xf_autotune(self._h, reps, _i(shapes), _d(ms)), "xf_autotune")
         return shapes, ms.reshape(3, 3)
 
+    def save_checkpoint(self, path, step, time, time_consumption=0.0):
+        """Write U in the reference's checkpoint format (xf_save_checkpoint)."""
+        self._chk(lib().xf_save_checkpoint(self._h, str(path).encode(), step, time, time_consumption),
+                  "xf_save_checkpoint")
+
+    def load_checkpoint(self, path):
+        """Read a checkpoint into U (T is kept); returns (step, time, time_consumption)."""
+        st, tm, tc = C.c_int(), C.c_double(), C.c_float()
+        self._chk(lib().xf_load_checkpoint(self._h, str(path).encode(), C.byref(st), C.byref(tm), C.byref(tc)),
+                  "xf_load_checkpoint")
+        return st.value, tm.value, tc.value
+
     def kernel_times(self):
         cap = 32
         names = (C.c_char_p * cap)()
@@ -454,3 +470,22 @@ class Ensemble:
 
     def __del__(self):
         self.close()
+
+
+def checkpoint_write(path, desc, U_aos, step, time, time_consumption=0.0):
+    """Write an AoS state in the reference's checkpoint format (host only)."""
+    U = np.ascontiguousarray(U_aos, dtype=np.float64)
+    st = lib().xf_checkpoint_write(str(path).encode(), C.byref(desc), _d(U), step, time, time_consumption)
+    if st:
+        raise XfError(st, "xf_checkpoint_write")
+
+
+def checkpoint_read(path, desc):
+    """Read a checkpoint written for `desc`: (U_aos, step, time, time_consumption)."""
+    E = desc.NS + 4 if desc.cop else 5
+    U = np.zeros(E * desc.Xmax * desc.Ymax * desc.Zmax)
+    st, tm, tc = C.c_int(), C.c_double(), C.c_float()
+    r = lib().xf_checkpoint_read(str(path).encode(), C.byref(desc), _d(U), C.byref(st), C.byref(tm), C.byref(tc))
+    if r:
+        raise XfError(r, "xf_checkpoint_read")
+    return U, st.value, tm.value, tc.value

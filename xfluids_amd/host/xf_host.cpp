@@ -205,4 +205,65 @@ int xf_evolve_staged(xf_ctx *ctx, int nsteps, double *dt_out)
     return XF_OK;
 }
 
+namespace {
+size_t state_doubles(const xf_case_desc &d)
+{
+    const size_t E = d.cop ? (size_t)d.NS + 4 : 5;
+    return E * (size_t)d.Xmax * (size_t)d.Ymax * (size_t)d.Zmax;
+}
+} // namespace
+
+int xf_checkpoint_write(const char *path, const xf_case_desc *d, const double *U_aos, int step, double time,
+                        float time_consumption)
+{
+    if (!path || !d || !U_aos) return XF_EINVAL;
+    std::FILE *f = std::fopen(path, "wb");
+    if (!f) return XF_EINVAL;
+    const size_t n = state_doubles(*d);
+    bool ok = std::fwrite(&step, sizeof(int), 1, f) == 1 && std::fwrite(&time, sizeof(double), 1, f) == 1 &&
+              std::fwrite(&time_consumption, sizeof(float), 1, f) == 1 && std::fwrite(U_aos, sizeof(double), n, f) == n;
+    ok = (std::fclose(f) == 0) && ok;
+    return ok ? XF_OK : XF_EINVAL;
+}
+
+int xf_checkpoint_read(const char *path, const xf_case_desc *d, double *U_aos, int *step, double *time,
+                       float *time_consumption)
+{
+    if (!path || !d || !U_aos) return XF_EINVAL;
+    std::FILE *f = std::fopen(path, "rb");
+    if (!f) return XF_EINVAL;
+    const size_t n = state_doubles(*d);
+    int st = 0;
+    double tm = 0.0;
+    float tc = 0.0f;
+    bool ok = std::fread(&st, sizeof(int), 1, f) == 1 && std::fread(&tm, sizeof(double), 1, f) == 1 &&
+              std::fread(&tc, sizeof(float), 1, f) == 1 && std::fread(U_aos, sizeof(double), n, f) == n;
+    char extra;
+    ok = ok && std::fread(&extra, 1, 1, f) == 0; // nothing may follow the state
+    std::fclose(f);
+    if (!ok) return XF_EINVAL;
+    if (step) *step = st;
+    if (time) *time = tm;
+    if (time_consumption) *time_consumption = tc;
+    return XF_OK;
+}
+
+int xf_save_checkpoint(xf_ctx *ctx, const char *path, int step, double time, float time_consumption)
+{
+    xf_case_desc d;
+    if (!ctx || xf_get_desc(ctx, &d)) return XF_EINVAL;
+    std::vector<double> U(state_doubles(d));
+    if (int e = xf_download_state(ctx, 0, U.data(), nullptr)) return e;
+    return xf_checkpoint_write(path, &d, U.data(), step, time, time_consumption);
+}
+
+int xf_load_checkpoint(xf_ctx *ctx, const char *path, int *step, double *time, float *time_consumption)
+{
+    xf_case_desc d;
+    if (!ctx || xf_get_desc(ctx, &d)) return XF_EINVAL;
+    std::vector<double> U(state_doubles(d));
+    if (int e = xf_checkpoint_read(path, &d, U.data(), step, time, time_consumption)) return e;
+    return xf_upload_state(ctx, 0, U.data(), nullptr);
+}
+
 } // extern "C"

@@ -49,6 +49,7 @@ extern "C" {
 /* face-flux schemes */
 #define XF_WENO5JS 0 /* weno5old_GPU, schemes/WENO5s_schemes.hpp:71-95 */
 #define XF_WENO5Z 1  /* weno5Z_P + weno5Z_M, schemes/WENO5s_schemes.hpp:246-316 */
+#define XF_WENOCU6 2 /* WENOCU6_GPU, schemes/WENO6s_schemes.hpp:5-80 (SCHEME_ORDER 6) */
 
 /* precision forms */
 #define XF_PLAIN_FP64 0 /* everything FP64, bitwise reproducible vs the CPU restatement */
@@ -74,7 +75,7 @@ typedef struct xf_case_desc {
     int NR;             /* reactions */
     int cop;            /* multi-component (COP) */
     int thermo;         /* 1 = NASA-9 (Thermo=1), 0 = JANAF/NASA-7 (Thermo=0) */
-    int scheme;         /* XF_WENO5JS or XF_WENO5Z */
+    int scheme;         /* XF_WENO5JS, XF_WENO5Z or XF_WENOCU6 */
     int react;          /* Sources_React */
     int strang;         /* 1 = Strang splitting, 0 = Lie */
     double ncop_gamma;  /* NCOP_Gamma */

@@ -406,6 +406,67 @@ double ora_weno5z(const double *pp, const double *mm)
     return weno5z_body(pp[1], pp[2], pp[3], pp[4], pp[5]) + weno5z_body(mm[6], mm[5], mm[4], mm[3], mm[2]);
 }
 
+/* WENOCU6_BODYGPU: src/solver_Reconstruction/schemes/WENO6s_schemes.hpp:5-51,
+   constants Utils_schemes.hpp:5-15 and src/include/global_setup.h:36-37 */
+static double wenocu6_body(double v1, double v2, double v3, double v4, double v5, double v6, double epsilon)
+{
+    const double ohtz = 1.0 / 120.0, ohff = 1.0 / 144.0, ftss = 1.0 / 5760.0, sxtn = 1.0 / 16.0, twfr = 1.0 / 24.0;
+    const double six = 1.0 / 6.0;
+    const double wa2a2 = 13.0 / 3.0, wa3a3 = 3129.0 / 80.0, wa4a4 = 87617.0 / 140.0, wa3a5 = 14127.0 / 224.0,
+                 wa5a5 = 252337135.0 / 16128.0;
+    double s11 = v1 - 2.0 * v2 + v3;
+    double s12 = v1 - 4.0 * v2 + 3.0 * v3;
+    double s1 = 13.0 * s11 * s11 + 3.0 * s12 * s12;
+    double s21 = v2 - 2.0 * v3 + v4;
+    double s22 = v2 - v4;
+    double s2 = 13.0 * s21 * s21 + 3.0 * s22 * s22;
+    double s31 = v3 - 2.0 * v4 + v5;
+    double s32 = 3.0 * v3 - 4.0 * v4 + v5;
+    double s3 = 13.0 * s31 * s31 + 3.0 * s32 * s32;
+    double tau61 = (259.0 * v6 - 1895.0 * v5 + 6670.0 * v4 - 2590.0 * v3 - 2785.0 * v2 + 341.0 * v1) * ftss;
+    double tau62 = -(v5 - 12.0 * v4 + 22.0 * v3 - 12.0 * v2 + v1) * sxtn;
+    double tau63 = -(7.0 * v6 - 47.0 * v5 + 94.0 * v4 - 70.0 * v3 + 11.0 * v2 + 5.0 * v1) * ohff;
+    double tau64 = (v5 - 4.0 * v4 + 6.0 * v3 - 4.0 * v2 + v1) * twfr;
+    double tau65 = -(-v6 + 5.0 * v5 - 10.0 * v4 + 10.0 * v3 - 5.0 * v2 + v1) * ohtz;
+    double a1a1 = 1.0, a2a2 = wa2a2, a1a3 = 0.5, a3a3 = wa3a3, a2a4 = 4.2;
+    double a1a5 = 0.125, a4a4 = wa4a4, a3a5 = wa3a5, a5a5 = wa5a5;
+    double s6 = (tau61 * tau61 * a1a1 + tau62 * tau62 * a2a2 + tau61 * tau63 * a1a3 + tau63 * tau63 * a3a3 +
+                 tau62 * tau64 * a2a4 + tau61 * tau65 * a1a5 + tau64 * tau64 * a4a4 + tau63 * tau65 * a3a5 +
+                 tau65 * tau65 * a5a5) * 12.0;
+    double s55 = (s1 + s3 + 4.0 * s2) * six;
+    double s5 = fabs(s6 - s55);
+    double r0 = 20.0;
+    double r1 = r0 + s5 / (s1 + epsilon);
+    double r2 = r0 + s5 / (s2 + epsilon);
+    double r3 = r0 + s5 / (s3 + epsilon);
+    double r4 = r0 + s5 / (s6 + epsilon);
+    double a1 = 0.05 * r1;
+    double a2 = 0.45 * r2;
+    double a3 = 0.45 * r3;
+    double a4 = 0.05 * r4;
+    double tw1 = 1.0 / (a1 + a2 + a3 + a4);
+    double w1 = a1 * tw1;
+    double w2 = a2 * tw1;
+    double w3 = a3 * tw1;
+    double w4 = a4 * tw1;
+    double temp = 0.0;
+    temp += w1 * (2.0 * v1 - 7.0 * v2 + 11.0 * v3);
+    temp += w2 * (-v2 + 5.0 * v3 + 2.0 * v4);
+    temp += w3 * (2.0 * v3 + 5.0 * v4 - v5);
+    temp += w4 * (11.0 * v4 - 7.0 * v5 + 2.0 * v6);
+    return temp;
+}
+
+/* WENOCU6_GPU(&pp[3], &mm[3], dl): WENO6s_schemes.hpp:54-80; epsilon = 1e-8 dl^2 */
+double ora_wenocu6(const double *pp, const double *mm, double delta)
+{
+    const double six = 1.0 / 6.0;
+    const double epsilon = 1.e-8 * delta * delta;
+    double temf = wenocu6_body(pp[1], pp[2], pp[3], pp[4], pp[5], pp[6], epsilon);
+    double temm = wenocu6_body(mm[6], mm[5], mm[4], mm[3], mm[2], mm[1], epsilon);
+    return (temf + temm) * six;
+}
+
 /* RoeAverageLeft_{x,y,z} / RoeAverageRight_{x,y,z} (EIGEN_ALLOC == 0):
    positive-definite_eigen/Eigen_matrix.hpp:7-455, with MARCO_PREEIGEN
    (Eigen_callback.h:102-106) evaluated per call as in the reference. */
@@ -646,7 +707,10 @@ static void face_flux(ora_fluid *f, int dir, size_t id_l, size_t id_r, long stri
             pp[m + 3] = 0.5 * (ff[m + 3] + artificial_viscosity * uf[m + 3]);
             mm[m + 3] = 0.5 * (ff[m + 3] - artificial_viscosity * uf[m + 3]);
         }
-        f_flux = (c->scheme == ORA_WENO5Z) ? ora_weno5z(pp, mm) : ora_weno5js(pp, mm);
+        if (c->scheme == ORA_WENOCU6)
+            f_flux = ora_wenocu6(pp, mm, dir == 0 ? c->dx : (dir == 1 ? c->dy : c->dz));
+        else
+            f_flux = (c->scheme == ORA_WENO5Z) ? ora_weno5z(pp, mm) : ora_weno5js(pp, mm);
         ora_eigen_lr(c, dir, n, NULL, eig, &eigen_value, z, _yi, c2, _rho, _u, _v, _w, _H, b1, b3);
         for (int n1 = 0; n1 < E; n1++)
             _p[n][n1] = f_flux * eig[n1];

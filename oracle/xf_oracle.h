@@ -34,7 +34,7 @@ enum { ORA_INFLOW = 0, ORA_OUTFLOW = 1, ORA_SYMMETRY = 2, ORA_PERIODIC = 3,
        ORA_NSLIPWALL = 4, ORA_VISCWALL = 5, ORA_SLIPWALL = 6, ORA_INNERBLOCK = 7 };
 
 /* Scheme codes for the characteristic face flux */
-enum { ORA_WENO5JS = 0, ORA_WENO5Z = 1 };
+enum { ORA_WENO5JS = 0, ORA_WENO5Z = 1, ORA_WENOCU6 = 2 };
 
 typedef struct {
     /* geometry: Block / MeshSize (src/include/global_setup.h:162-200),
@@ -50,7 +50,7 @@ typedef struct {
     int NR;          /* NUM_REA */
     int cop;         /* COP defined (multi-component) */
     int thermo;      /* Thermo: 1 = NASA-9, 0 = JANAF/NASA-7 */
-    int scheme;      /* ORA_WENO5JS (weno5old_GPU) or ORA_WENO5Z */
+    int scheme;      /* ORA_WENO5JS (weno5old_GPU), ORA_WENO5Z or ORA_WENOCU6 */
     int react;       /* ReactSources */
     int strang;      /* SlipOrder == "Strang" */
     double ncop_gamma; /* NCOP_Gamma */
@@ -111,6 +111,7 @@ double ora_mix_R(const ora_config *c, const double *yi);
 double ora_get_T(const ora_config *c, const double *yi, double e, double T0);
 double ora_weno5js(const double *pp, const double *mm);     /* weno5old_GPU(&pp[3],&mm[3]) */
 double ora_weno5z(const double *pp, const double *mm);      /* weno5Z_P + weno5Z_M        */
+double ora_wenocu6(const double *pp, const double *mm, double delta); /* WENOCU6_GPU(&pp[3],&mm[3],dl) */
 /* left row n / right column n of the Roe eigen-system in direction dir
    (0,1,2); z, yi of length NS-1 */
 void ora_eigen_lr(const ora_config *c, int dir, int n, double *eig_l, double *eig_r,

@@ -61,6 +61,7 @@ def load(path):
         "ora_mix_h": (C.c_double, [P, _DP, C.c_double]), "ora_mix_e": (C.c_double, [P, _DP, C.c_double]),
         "ora_mix_R": (C.c_double, [P, _DP]), "ora_get_T": (C.c_double, [P, _DP, C.c_double, C.c_double]),
         "ora_weno5js": (C.c_double, [_DP, _DP]), "ora_weno5z": (C.c_double, [_DP, _DP]),
+        "ora_wenocu6": (C.c_double, [_DP, _DP, C.c_double]),
         "ora_eigen_lr": (None, [P, C.c_int, C.c_int, _DP, _DP, _DP, _DP, _DP] + [C.c_double] * 8),
         "ora_evalcv": (None, [P, _DP, _DP, _DP, C.c_double, C.c_double]),
         "ora_chemeq2": (C.c_int, [P, _DP, C.c_double, C.c_double, C.c_double]),

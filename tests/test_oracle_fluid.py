@@ -167,7 +167,7 @@ def _sod_exact(x, t, x0=0.5, g=1.4, L=(1.0, 0.0, 1.0), R=(0.125, 0.0, 0.1)):
     return out
 
 
-@pytest.mark.parametrize("scheme", [0, 1])
+@pytest.mark.parametrize("scheme", [0, 1, 2])
 def test_sod_shock_tube_matches_exact_riemann(scheme):
     n = 400
     s = xf.Setup(xf.case_path("reactive-shock-tube"), nx=n)

@@ -119,3 +119,41 @@ def test_eigenvectors_are_inverse(cop, NS, d):
         scale = np.abs(Lm).max() * np.abs(Rm).max()
         assert np.abs(Lm @ Rm - np.eye(E)).max() <= 1e-13 * scale
         assert np.abs(Rm @ Lm - np.eye(E)).max() <= 1e-13 * scale
+
+
+def cu6(pp, mm, delta=1.0):
+    a = np.ascontiguousarray(pp, dtype=np.float64)
+    b = np.ascontiguousarray(mm, dtype=np.float64)
+    return lib().ora_wenocu6(a.ctypes.data_as(DP), b.ctypes.data_as(DP), delta)
+
+
+def test_wenocu6_linear_data_is_exact():
+    """All four candidate stencils of WENO-CU6 (three upwind, one downwind)
+    reproduce linear data, so any convex weighting returns the exact face
+    value; the minus side reads mm[6..1] mirrored."""
+    rng = np.random.default_rng(7)
+    for _ in range(200):
+        a, b = rng.normal() * 10, rng.normal() * 3
+        v = a + b * np.arange(-2, 6, dtype=np.float64)     # points i-3 .. i+4 in pp[0..7]
+        exact = a + b * 1.5                                 # the face i + 1/2
+        got = cu6(v, np.zeros(8), delta=rng.uniform(1e-4, 1.0))
+        assert abs(got - exact) <= 1e-12 * max(1.0, abs(exact))
+        w = v[::-1].copy()
+        got_m = cu6(np.zeros(8), w)
+        exact_m = a + b * 1.5                               # mirrored stencil, same face
+        assert abs(got_m - exact_m) <= 1e-12 * max(1.0, abs(exact_m))
+
+
+def test_wenocu6_smooth_data_converges_at_high_order():
+    """Cell averages of sin(x): as h -> 0 the weights tend to the ideal
+    (0.05, 0.45, 0.45, 0.05), the sixth-order central reconstruction
+    (1 -8 37 37 -8 1)/60, so the face error falls at order > 5."""
+    errs = []
+    for h in (0.2, 0.1, 0.05):
+        x0 = 0.3
+        centers = x0 + h * (np.arange(8) - 3)
+        avg = (np.cos(centers - h / 2) - np.cos(centers + h / 2)) / h
+        face = np.sin(x0 + h / 2)
+        errs.append(abs(cu6(avg, np.zeros(8), delta=h) - face))
+    orders = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    assert np.all(orders > 5.0), (errs, orders)

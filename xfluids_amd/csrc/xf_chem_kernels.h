@@ -479,7 +479,7 @@ k_chem_queue_multi(const ChemCase *__restrict__ cases, int ncases, long long tot
 }
 
 // zero the work-queue counter (a kernel, so the step graph holds no memset)
-__global__ void k_queue_reset(unsigned int *q)
+static __global__ void k_queue_reset(unsigned int *q)
 {
     if (threadIdx.x < 4) q[threadIdx.x] = 0u;
 }

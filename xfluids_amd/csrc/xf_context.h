@@ -1,0 +1,218 @@
+// Context of one case and the per-species-count kernel dispatch table,
+// shared by the C-ABI translation unit (xf_api.hip) and the translation units
+// that instantiate the kernels for each compile-time species count
+// (xf_ops_ns*.hip), so that those compile in parallel.
+#pragma once
+#include "xf_sweep_kernels.h"
+#include "xf_chem_kernels.h"
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+using namespace xf;
+
+namespace xf {
+
+struct EvPair {
+    const char *name;
+    hipEvent_t a, b;
+};
+
+struct Ops {
+    void (*states)(xf_ctx *, const double *U, int full);
+    void (*sweep)(xf_ctx *, int dir, const double *Ug, int mode, int stage, const double *Ua, const double *Ub,
+                  double *Uo);
+    void (*chem)(xf_ctx *, int record);
+    void (*prepare)(xf_ctx *);
+    // shared queue over the cells of several cases (ensembles)
+    void (*chem_multi)(hipStream_t, const ChemCase *, int ncases, long long total, const DevThermo *,
+                       const DevMech *, unsigned long long *queue, int blocks, int record);
+    int (*multi_blocks)(long long total);
+};
+
+// dispatch tables, one translation unit each
+Ops ops_ns1();
+Ops ops_ns2();
+Ops ops_ns3();
+Ops ops_ns9();
+
+} // namespace xf
+
+struct xf_ctx {
+    xf_case_desc d;
+    Geom g;
+    int E, NCOP;
+    long long N, Ninner;
+    hipStream_t stream = nullptr;
+    Fields f{};
+    double *U2 = nullptr; // RK stage-2 output of the fused path
+    DevThermo h_th{};
+    DevThermo *d_th = nullptr;
+    DevMech h_mc{};
+    DevMech *d_mc = nullptr;
+    bool thermo_set = false, mech_set = false;
+    Ops ops{};
+    bool has_ops = false;
+    int chem_blocks = 0; // resident one-wave blocks of the work-queue chemistry kernel
+    int shape[3] = {0, 0, 0}; // sweep tile shape per direction (SweepShape)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int last_code = 0, last_id = -1;
+    bool prof = false;
+    std::string prof_filter; // ",name,name," or empty for all
+    std::vector<EvPair> evs;
+    std::map<std::string, std::pair<double, int>> times;
+};
+
+#define XF_HIP(call)                                                                                                   \
+    do {                                                                                                               \
+        hipError_t e_ = (call);                                                                                        \
+        if (e_ != hipSuccess) {                                                                                        \
+            fprintf(stderr, "xfluids_amd: HIP error %s at %s:%d\n", hipGetErrorString(e_), __FILE__, __LINE__);       \
+            return XF_EHIP;                                                                                            \
+        }                                                                                                              \
+    } while (0)
+
+namespace xf {
+
+inline unsigned nblk(long long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+struct ProfScope {
+    xf_ctx *c;
+    const char *name;
+    hipEvent_t a{}, b{};
+    bool on;
+    ProfScope(xf_ctx *c_, const char *n) : c(c_), name(n)
+    {
+        on = c->prof && (c->prof_filter.empty() || c->prof_filter.find("," + std::string(n) + ",") != std::string::npos);
+        if (on) {
+            (void)hipEventCreate(&a);
+            (void)hipEventCreate(&b);
+            (void)hipEventRecord(a, c->stream);
+        }
+    }
+    ~ProfScope()
+    {
+        if (on) {
+            (void)hipEventRecord(b, c->stream);
+            c->evs.push_back({name, a, b});
+        }
+    }
+};
+
+template <int NS, bool COP> struct Launch {
+    static void states(xf_ctx *c, const double *U, int full)
+    {
+        ProfScope ps(c, "states");
+        hipLaunchKernelGGL((k_states<NS, COP>), dim3(nblk(c->N)), dim3(256), 0, c->stream, c->g, c->d_th, U, c->f,
+                           c->d.ncop_gamma, full);
+    }
+    template <int DIR, int SCH, int PREC, int SHAPE>
+    static void sweep_t(xf_ctx *c, const double *Ug, int mode, int stage, const double *Ua, const double *Ub, double *Uo)
+    {
+        constexpr int E = COP ? NS + 4 : 5;
+        using SS = SweepShape<DIR, E, SHAPE>;
+        const Geom &g = c->g;
+        dim3 grid;
+        if (DIR == 0)
+            grid = dim3((unsigned)((g.X_inner + SS::TS - 2) / (SS::TS - 1)), (unsigned)((g.Y_inner * g.Z_inner + SS::TW - 1) / SS::TW), 1);
+        else if (DIR == 1)
+            grid = dim3((unsigned)((g.X_inner + SS::TW - 1) / SS::TW), (unsigned)((g.Y_inner + SS::TS - 2) / (SS::TS - 1)), (unsigned)g.Z_inner);
+        else
+            grid = dim3((unsigned)((g.X_inner + SS::TW - 1) / SS::TW), (unsigned)((g.Z_inner + SS::TS - 2) / (SS::TS - 1)), (unsigned)g.Y_inner);
+        static const char *names[3] = {"sweep_x", "sweep_y", "sweep_z"};
+        ProfScope ps(c, names[DIR]);
+        hipLaunchKernelGGL((k_sweep<NS, COP, DIR, SCH, PREC, SHAPE>), grid, dim3(256), 0, c->stream, c->g, c->d_th,
+                           Ug, c->f, c->d.ncop_gamma, mode, stage, Ua, Ub, Uo);
+    }
+    template <int DIR, int SCH, int PREC>
+    static void sweep_s_(xf_ctx *c, const double *Ug, int mode, int stage, const double *Ua, const double *Ub, double *Uo)
+    {
+        switch (c->shape[DIR]) {
+        case 1: sweep_t<DIR, SCH, PREC, 1>(c, Ug, mode, stage, Ua, Ub, Uo); break;
+        case 2:
+            if constexpr (SweepShapes<DIR>::count > 2) {
+                sweep_t<DIR, SCH, PREC, 2>(c, Ug, mode, stage, Ua, Ub, Uo);
+                break;
+            }
+            [[fallthrough]];
+        default: sweep_t<DIR, SCH, PREC, 0>(c, Ug, mode, stage, Ua, Ub, Uo);
+        }
+    }
+    template <int DIR, int PREC>
+    static void sweep_p(xf_ctx *c, const double *Ug, int mode, int stage, const double *Ua, const double *Ub, double *Uo)
+    {
+        if (c->d.scheme == XF_WENO5Z) sweep_s_<DIR, XF_WENO5Z, PREC>(c, Ug, mode, stage, Ua, Ub, Uo);
+        else if (c->d.scheme == XF_WENOCU6) sweep_s_<DIR, XF_WENOCU6, PREC>(c, Ug, mode, stage, Ua, Ub, Uo);
+        else sweep_s_<DIR, XF_WENO5JS, PREC>(c, Ug, mode, stage, Ua, Ub, Uo);
+    }
+    template <int DIR>
+    static void sweep_d(xf_ctx *c, const double *Ug, int mode, int stage, const double *Ua, const double *Ub, double *Uo)
+    {
+        if (c->d.precision == XF_MIXED) sweep_p<DIR, XF_MIXED>(c, Ug, mode, stage, Ua, Ub, Uo);
+        else sweep_p<DIR, XF_PLAIN_FP64>(c, Ug, mode, stage, Ua, Ub, Uo);
+    }
+    static void sweep(xf_ctx *c, int dir, const double *Ug, int mode, int stage, const double *Ua, const double *Ub,
+                      double *Uo)
+    {
+        if (dir == 0) sweep_d<0>(c, Ug, mode, stage, Ua, Ub, Uo);
+        if (dir == 1) sweep_d<1>(c, Ug, mode, stage, Ua, Ub, Uo);
+        if (dir == 2) sweep_d<2>(c, Ug, mode, stage, Ua, Ub, Uo);
+    }
+    static void chem(xf_ctx *c, int record)
+    {
+        if constexpr (COP) {
+            ProfScope ps(c, "chem");
+            if (c->d.chem_driver == XF_CHEM_QUEUE) {
+                hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, c->stream, c->f.queue);
+                hipLaunchKernelGGL((k_chem_queue<NS>), dim3(c->chem_blocks), dim3(64), 0, c->stream, c->g, c->d_th,
+                                   c->d_mc, c->f, record);
+            } else {
+                hipLaunchKernelGGL((k_chem<NS>), dim3(nblk(c->Ninner)), dim3(256), 0, c->stream, c->g, c->d_th,
+                                   c->d_mc, c->f, record);
+            }
+        }
+    }
+    // persistent lanes of the queue driver: as many one-wave blocks as stay
+    // resident, never more than the cells need (queried once, outside any
+    // stream capture)
+    static void prepare(xf_ctx *c)
+    {
+        if constexpr (COP) {
+            int per_cu = 0, ncu = 0, dev = 0;
+            (void)hipGetDevice(&dev);
+            (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_chem_queue<NS>, 64, 0);
+            const long long want = (c->Ninner + 63) / 64;
+            const long long nb = (long long)std::max(per_cu, 1) * std::max(ncu, 1);
+            c->chem_blocks = (int)std::max(1LL, std::min(nb, want));
+        }
+    }
+    static void chem_multi(hipStream_t st, const ChemCase *cases, int ncases, long long total, const DevThermo *th,
+                           const DevMech *mc, unsigned long long *queue, int blocks, int record)
+    {
+        if constexpr (COP) {
+            hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, st, (unsigned int *)queue);
+            hipLaunchKernelGGL((k_chem_queue_multi<NS>), dim3(blocks), dim3(64), 0, st, cases, ncases, total, th, mc,
+                               queue, record);
+        }
+    }
+    static int multi_blocks(long long total)
+    {
+        int per_cu = 0, ncu = 0, dev = 0;
+        if constexpr (COP) {
+            (void)hipGetDevice(&dev);
+            (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_chem_queue_multi<NS>, 64, 0);
+        }
+        const long long want = (total + 63) / 64;
+        const long long nb = (long long)std::max(per_cu, 1) * std::max(ncu, 1);
+        return (int)std::max(1LL, std::min(nb, want));
+    }
+    static Ops ops() { return Ops{&states, &sweep, &chem, &prepare, &chem_multi, &multi_blocks}; }
+};
+
+} // namespace xf

@@ -102,7 +102,7 @@ __device__ __forceinline__ void bc_axis(int bc, int &s, int B, int inner, int Lm
     s = t;
 }
 
-__global__ void __launch_bounds__(256) k_boundary_all(Geom g, double *__restrict__ U, int E, int bx0, int bx1, int by0,
+static __global__ void __launch_bounds__(256) k_boundary_all(Geom g, double *__restrict__ U, int E, int bx0, int bx1, int by0,
                                                       int by1, int bz0, int bz1, double *red, int reset_red)
 {
     if (reset_red && blockIdx.x == 0 && threadIdx.x < 12) {
@@ -300,23 +300,14 @@ __global__ void __launch_bounds__(256) k_states(Geom g, const DevThermo *__restr
 // small control kernels
 // ---------------------------------------------------------------------------
 // zero the per-pass maxima (pending GLF, CFL) before an UpdateStates pass
-__global__ void k_reset_pass_red(double *red)
+static __global__ void k_reset_pass_red(double *red)
 {
     if (threadIdx.x < 9) red[RED_GLFP + threadIdx.x] = 0.0;
     if (threadIdx.x < 3) red[RED_DT + threadIdx.x] = 0.0;
 }
 
-// GetLU's GLF reduction: fold the last pass's maxima into the running maxima
-__global__ void k_fold_glf(double *red)
-{
-    if (threadIdx.x < 9) {
-        const double a = red[RED_GLF + threadIdx.x], b = red[RED_GLFP + threadIdx.x];
-        red[RED_GLF + threadIdx.x] = b > a ? b : a;
-    }
-}
-
 // GetDt: dt = CFL / sum_d max_d / Delta_d, written to device memory
-__global__ void k_dt(Geom g, const double *red, double *dt)
+static __global__ void k_dt(Geom g, const double *red, double *dt)
 {
     if (threadIdx.x == 0) {
         const double mx = g.DimX ? red[RED_DT + 0] : 0.0;
@@ -328,7 +319,7 @@ __global__ void k_dt(Geom g, const double *red, double *dt)
 }
 
 // Strang half step on device: dt[1] = dt[0] * 0.5 (or dt[0] for Lie)
-__global__ void k_dt_chem(double *dt, int strang)
+static __global__ void k_dt_chem(double *dt, int strang)
 {
     if (threadIdx.x == 0) dt[1] = strang ? dt[0] * 0.5 : dt[0];
 }
@@ -346,7 +337,7 @@ __device__ __forceinline__ void rk3_point(int stage, double dt, double &U, doubl
         U = (U + 2.0 * U1 + 2.0 * dt * LU) * OT;
 }
 
-__global__ void __launch_bounds__(256) k_rk3(Geom g, int E, double *__restrict__ U, double *__restrict__ U1,
+static __global__ void __launch_bounds__(256) k_rk3(Geom g, int E, double *__restrict__ U, double *__restrict__ U1,
                                              const double *__restrict__ LU, const double *__restrict__ dtp, int stage)
 {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -366,7 +357,7 @@ __global__ void __launch_bounds__(256) k_rk3(Geom g, int E, double *__restrict__
 }
 
 // EstimateFluidNANKernel (src/Fluids.cpp:47-87) on interior cells
-__global__ void __launch_bounds__(256) k_estimate_nan(Geom g, int E, const double *__restrict__ UI,
+static __global__ void __launch_bounds__(256) k_estimate_nan(Geom g, int E, const double *__restrict__ UI,
                                                       const double *__restrict__ LU, int *err)
 {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;

@@ -51,6 +51,7 @@ struct Geom {
     int DimX, DimY, DimZ;
     double _dx, _dy, _dz, CFL;
     long long N; // padded cells
+    double dx, dy, dz; // cell sizes (WENO-CU6 epsilon = 1e-8 dl^2)
 };
 
 // Device pointers of one case

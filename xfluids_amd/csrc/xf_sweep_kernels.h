@@ -126,9 +126,61 @@ __device__ __forceinline__ double weno5z_body(double v1, double v2, double v3, d
     return q1 + q2 + q3;
 }
 
-template <int SCHEME>
-__device__ __forceinline__ double weno_face(const double *pp, const double *mm)
+// WENOCU6_BODYGPU (schemes/WENO6s_schemes.hpp:5-51): three upwind and one
+// downwind candidate, the sixth-order indicator s6 from tau61..tau65, and
+// weights 0.05 / 0.45 / 0.45 / 0.05 times (20 + s5 / (s_k + epsilon))
+__device__ __forceinline__ double wenocu6_body(double v1, double v2, double v3, double v4, double v5, double v6,
+                                               double epsilon)
 {
+    const double ohtz = 1.0 / 120.0, ohff = 1.0 / 144.0, ftss = 1.0 / 5760.0, sxtn = 1.0 / 16.0, twfr = 1.0 / 24.0;
+    const double six = 1.0 / 6.0;
+    const double wa2a2 = 13.0 / 3.0, wa3a3 = 3129.0 / 80.0, wa4a4 = 87617.0 / 140.0, wa3a5 = 14127.0 / 224.0,
+                 wa5a5 = 252337135.0 / 16128.0;
+    const double s11 = v1 - 2.0 * v2 + v3;
+    const double s12 = v1 - 4.0 * v2 + 3.0 * v3;
+    const double s1 = 13.0 * s11 * s11 + 3.0 * s12 * s12;
+    const double s21 = v2 - 2.0 * v3 + v4;
+    const double s22 = v2 - v4;
+    const double s2 = 13.0 * s21 * s21 + 3.0 * s22 * s22;
+    const double s31 = v3 - 2.0 * v4 + v5;
+    const double s32 = 3.0 * v3 - 4.0 * v4 + v5;
+    const double s3 = 13.0 * s31 * s31 + 3.0 * s32 * s32;
+    const double tau61 = (259.0 * v6 - 1895.0 * v5 + 6670.0 * v4 - 2590.0 * v3 - 2785.0 * v2 + 341.0 * v1) * ftss;
+    const double tau62 = -(v5 - 12.0 * v4 + 22.0 * v3 - 12.0 * v2 + v1) * sxtn;
+    const double tau63 = -(7.0 * v6 - 47.0 * v5 + 94.0 * v4 - 70.0 * v3 + 11.0 * v2 + 5.0 * v1) * ohff;
+    const double tau64 = (v5 - 4.0 * v4 + 6.0 * v3 - 4.0 * v2 + v1) * twfr;
+    const double tau65 = -(-v6 + 5.0 * v5 - 10.0 * v4 + 10.0 * v3 - 5.0 * v2 + v1) * ohtz;
+    const double s6 = (tau61 * tau61 * 1.0 + tau62 * tau62 * wa2a2 + tau61 * tau63 * 0.5 + tau63 * tau63 * wa3a3 +
+                       tau62 * tau64 * 4.2 + tau61 * tau65 * 0.125 + tau64 * tau64 * wa4a4 + tau63 * tau65 * wa3a5 +
+                       tau65 * tau65 * wa5a5) * 12.0;
+    const double s55 = (s1 + s3 + 4.0 * s2) * six;
+    const double s5 = fabs(s6 - s55);
+    const double r0 = 20.0;
+    const double a1 = 0.05 * (r0 + s5 / (s1 + epsilon));
+    const double a2 = 0.45 * (r0 + s5 / (s2 + epsilon));
+    const double a3 = 0.45 * (r0 + s5 / (s3 + epsilon));
+    const double a4 = 0.05 * (r0 + s5 / (s6 + epsilon));
+    const double tw1 = 1.0 / (a1 + a2 + a3 + a4);
+    const double w1 = a1 * tw1, w2 = a2 * tw1, w3 = a3 * tw1, w4 = a4 * tw1;
+    double temp = 0.0;
+    temp += w1 * (2.0 * v1 - 7.0 * v2 + 11.0 * v3);
+    temp += w2 * (-v2 + 5.0 * v3 + 2.0 * v4);
+    temp += w3 * (2.0 * v3 + 5.0 * v4 - v5);
+    temp += w4 * (11.0 * v4 - 7.0 * v5 + 2.0 * v6);
+    return temp;
+}
+
+// pp, mm hold the split values of points i-2..i+3 in [1..6] (WENO5 reads
+// pp[1..5] and mm[2..6]; WENO-CU6 reads all six on both sides)
+template <int SCHEME>
+__device__ __forceinline__ double weno_face(const double *pp, const double *mm, double delta)
+{
+    if (SCHEME == XF_WENOCU6) {
+        const double epsilon = 1.e-8 * delta * delta;
+        const double temf = wenocu6_body(pp[1], pp[2], pp[3], pp[4], pp[5], pp[6], epsilon);
+        const double temm = wenocu6_body(mm[6], mm[5], mm[4], mm[3], mm[2], mm[1], epsilon);
+        return (temf + temm) * (1.0 / 6.0);
+    }
     if (SCHEME == XF_WENO5Z)
         return weno5z_body(pp[1], pp[2], pp[3], pp[4], pp[5]) + weno5z_body(mm[6], mm[5], mm[4], mm[3], mm[2]);
     const double six = 1.0 / 6.0;
@@ -181,9 +233,51 @@ __device__ __forceinline__ float weno5_body_f32(float v1, float v2, float v3, fl
     return (a1 * q1 + a2 * q2 + a3 * q3) * rcp_f32(a1 + a2 + a3) * (1.0f / 6.0f);
 }
 
-template <int SCHEME>
-__device__ __forceinline__ float weno_face_f32(const float *pp, const float *mm)
+// WENO-CU6 in FP32 with the same rescaling: every indicator (s1..s3, s6)
+// is quadratic in the data, so epsilon = 1e-8 dl^2 becomes 1e-8 dl^2 / m^2
+__device__ __forceinline__ float wenocu6_body_f32(float v1, float v2, float v3, float v4, float v5, float v6,
+                                                  float eps_abs)
 {
+    const float c1 = 2.0f * v1 - 7.0f * v2 + 11.0f * v3;
+    const float c2 = -v2 + 5.0f * v3 + 2.0f * v4;
+    const float c3 = 2.0f * v3 + 5.0f * v4 - v5;
+    const float c4 = 11.0f * v4 - 7.0f * v5 + 2.0f * v6;
+    const float m = fmaxf(fmaxf(fmaxf(fabsf(v1), fabsf(v2)), fmaxf(fabsf(v3), fabsf(v4))), fmaxf(fabsf(v5), fabsf(v6)));
+    if (!(m > 1e-30f)) return (0.05f * c1 + 0.45f * c2 + 0.45f * c3 + 0.05f * c4) * (1.0f / 6.0f);
+    const float inv = rcp_f32(m);
+    const float u1 = v1 * inv, u2 = v2 * inv, u3 = v3 * inv, u4 = v4 * inv, u5 = v5 * inv, u6 = v6 * inv;
+    float t = u1 - 2.0f * u2 + u3, r = u1 - 4.0f * u2 + 3.0f * u3;
+    const float s1 = 13.0f * t * t + 3.0f * r * r;
+    t = u2 - 2.0f * u3 + u4, r = u2 - u4;
+    const float s2 = 13.0f * t * t + 3.0f * r * r;
+    t = u3 - 2.0f * u4 + u5, r = 3.0f * u3 - 4.0f * u4 + u5;
+    const float s3 = 13.0f * t * t + 3.0f * r * r;
+    const float t1 = (259.0f * u6 - 1895.0f * u5 + 6670.0f * u4 - 2590.0f * u3 - 2785.0f * u2 + 341.0f * u1) *
+                     (1.0f / 5760.0f);
+    const float t2 = -(u5 - 12.0f * u4 + 22.0f * u3 - 12.0f * u2 + u1) * (1.0f / 16.0f);
+    const float t3 = -(7.0f * u6 - 47.0f * u5 + 94.0f * u4 - 70.0f * u3 + 11.0f * u2 + 5.0f * u1) * (1.0f / 144.0f);
+    const float t4 = (u5 - 4.0f * u4 + 6.0f * u3 - 4.0f * u2 + u1) * (1.0f / 24.0f);
+    const float t5 = -(-u6 + 5.0f * u5 - 10.0f * u4 + 10.0f * u3 - 5.0f * u2 + u1) * (1.0f / 120.0f);
+    const float s6 = (t1 * t1 + t2 * t2 * (13.0f / 3.0f) + t1 * t3 * 0.5f + t3 * t3 * (3129.0f / 80.0f) +
+                      t2 * t4 * 4.2f + t1 * t5 * 0.125f + t4 * t4 * (87617.0f / 140.0f) +
+                      t3 * t5 * (14127.0f / 224.0f) + t5 * t5 * (252337135.0f / 16128.0f)) * 12.0f;
+    const float s5 = fabsf(s6 - (s1 + s3 + 4.0f * s2) * (1.0f / 6.0f));
+    const float eps = fmaxf(fminf(eps_abs * inv * inv, 1e30f), 1e-30f);
+    const float a1 = 0.05f * (20.0f + s5 * rcp_f32(s1 + eps));
+    const float a2 = 0.45f * (20.0f + s5 * rcp_f32(s2 + eps));
+    const float a3 = 0.45f * (20.0f + s5 * rcp_f32(s3 + eps));
+    const float a4 = 0.05f * (20.0f + s5 * rcp_f32(s6 + eps));
+    return (a1 * c1 + a2 * c2 + a3 * c3 + a4 * c4) * rcp_f32(a1 + a2 + a3 + a4) * (1.0f / 6.0f);
+}
+
+template <int SCHEME>
+__device__ __forceinline__ float weno_face_f32(const float *pp, const float *mm, float delta)
+{
+    if (SCHEME == XF_WENOCU6) {
+        const float eps = 1.e-8f * delta * delta;
+        return wenocu6_body_f32(pp[1], pp[2], pp[3], pp[4], pp[5], pp[6], eps) +
+               wenocu6_body_f32(mm[6], mm[5], mm[4], mm[3], mm[2], mm[1], eps);
+    }
     return weno5_body_f32<SCHEME>(pp[1], pp[2], pp[3], pp[4], pp[5]) +
            weno5_body_f32<SCHEME>(mm[6], mm[5], mm[4], mm[3], mm[2]);
 }
@@ -475,22 +569,25 @@ k_sweep(Geom g, const DevThermo *__restrict__ th, const double *__restrict__ Ug,
         const double un = vr3[DIR];
         const double un_c = un * _c1;
 
-        // GLF split of the six projected points and the WENO5 face value
+        // GLF split of the six projected points and the face value (WENO5
+        // uses pp[1..5], mm[2..6]; WENO-CU6 all six on both sides)
+        constexpr int PLO = SCHEME == XF_WENOCU6 ? 1 : 2, PHI = SCHEME == XF_WENOCU6 ? 6 : 5;
+        const double dl = DIR == 0 ? g.dx : (DIR == 1 ? g.dy : g.dz);
         auto split_weno = [&](double av, const double *uf, const double *ff) -> double {
             double pp[7], mm[7];
 #pragma unroll
-            for (int m = 1; m <= 5; m++) pp[m] = 0.5 * (ff[m] + av * uf[m]);
+            for (int m = 1; m <= PHI; m++) pp[m] = 0.5 * (ff[m] + av * uf[m]);
 #pragma unroll
-            for (int m = 2; m <= 6; m++) mm[m] = 0.5 * (ff[m] - av * uf[m]);
-            return weno_face<SCHEME>(pp, mm);
+            for (int m = PLO; m <= 6; m++) mm[m] = 0.5 * (ff[m] - av * uf[m]);
+            return weno_face<SCHEME>(pp, mm, dl);
         };
         auto split_weno_f32 = [&](float av, const float *uf, const float *ff) -> double {
             float pp[7], mm[7];
 #pragma unroll
-            for (int m = 1; m <= 5; m++) pp[m] = 0.5f * fmaf(av, uf[m], ff[m]);
+            for (int m = 1; m <= PHI; m++) pp[m] = 0.5f * fmaf(av, uf[m], ff[m]);
 #pragma unroll
-            for (int m = 2; m <= 6; m++) mm[m] = 0.5f * fmaf(-av, uf[m], ff[m]);
-            return (double)weno_face_f32<SCHEME>(pp, mm);
+            for (int m = PLO; m <= 6; m++) mm[m] = 0.5f * fmaf(-av, uf[m], ff[m]);
+            return (double)weno_face_f32<SCHEME>(pp, mm, (float)dl);
         };
         // dense rows: n = 0 (u-c), n = 1 + DIR (normal), n = E - 1 (u+c);
         // called with literal n, so every branch on n folds at compile time

@@ -194,7 +194,7 @@ bool setup_load(const std::string &settings, const std::string &runtime_root, Se
     const std::string scheme = eq->value("Scheme", std::string("WENO5"));
     d.thermo = (thermo == "JANAF") ? 0 : 1;
     if (thermo_override >= 0) d.thermo = thermo_override ? 1 : 0;
-    d.scheme = (scheme == "WENO5Z") ? XF_WENO5Z : XF_WENO5JS;
+    d.scheme = (scheme == "WENO5Z") ? XF_WENO5Z : (scheme == "WENOCU6" ? XF_WENOCU6 : XF_WENO5JS);
     s.slip_order = eq->value("SlipOrder", std::string("Strang"));
     d.strang = s.slip_order == "Strang";
     d.precision = XF_PLAIN_FP64;

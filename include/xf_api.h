@@ -81,6 +81,8 @@ typedef struct xf_case_desc {
     double ncop_gamma;  /* NCOP_Gamma */
     int precision;      /* XF_PLAIN_FP64 or XF_MIXED */
     int chem_driver;    /* XF_CHEM_PER_CELL or XF_CHEM_QUEUE */
+    int positivity;     /* PositivityPreserving flux limiter on the face fluxes
+                           (PositivityPreserving_kernels.hpp:5-76) */
 } xf_case_desc;
 
 typedef struct xf_ctx xf_ctx;

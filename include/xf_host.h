@@ -24,6 +24,9 @@ void xf_setup_free(xf_setup *s);
 /* Override mesh resolution / CFL / scheme / thermo before use (recomputes
    the geometry as Setup::init); pass 0 (or < 0 for cfl) to keep a value. */
 int xf_setup_override(xf_setup *s, int nx, int ny, int nz, double cfl, int scheme, int thermo);
+/* Switch the positivity-preserving flux limiter on (1) or off (0), as the
+   settings key equations.PositivityPreserving does (src/read_ini/settings/read_json.cpp:68). */
+int xf_setup_set_positivity(xf_setup *s, int on);
 int xf_setup_desc(const xf_setup *s, xf_case_desc *out);
 /* Thermal tables Hia[NS*21], Hib[NS*6], Wi[NS] (kg/mol). */
 int xf_setup_thermo(const xf_setup *s, double *Hia, double *Hib, double *Wi);

@@ -723,8 +723,66 @@ static void face_flux(ora_fluid *f, int dir, size_t id_l, size_t id_r, long stri
     }
 }
 
-/* GetLU: FDM_Method/ConVenction_block.hpp:10-617 (no positivity limiter,
-   no viscous terms).  eigen_block keeps its running maximum across calls:
+/* PositivityPreservingKernel: positive-definite_eigen/PositivityPreserving_kernels.hpp:5-76,
+   face (id_l, id_r).  Blends the high-order face flux toward the
+   Lax-Friedrichs flux until density and each of the Ns-1 mass fractions of
+   the two adjacent cells stay above epsilon after a forward-Euler step of
+   size lambda.  Kept as written, including the species loop's update of
+   FF[n] (indexed by the species counter), whose values are never read
+   again. */
+static void positivity_face(ora_fluid *f, size_t id_l, size_t id_r, const double *UI, const double *Fl, double *Fwall,
+                            double lambda_0, double lambda, const double *epsilon)
+{
+    const int E = f->E, NCOP = f->NCOP;
+    double theta, theta_u, theta_p, rho_min, F_LF[ORA_MAXE], FF_LF[ORA_MAXE], FF[ORA_MAXE];
+    const double *UU = UI + (size_t)E * id_l, *UP = UI + (size_t)E * id_r;
+    double *Fw = Fwall + (size_t)E * id_l;
+    for (int n = 0; n < E; n++) {
+        F_LF[n] = 0.5 * (Fl[n + (size_t)E * id_l] + Fl[n + (size_t)E * id_r] + lambda_0 * (UU[n] - UP[n]));
+        FF_LF[n] = 2.0 * lambda * F_LF[n];
+        FF[n] = 2.0 * lambda * Fw[n];
+    }
+    theta_u = 1.0, theta_p = 1.0;
+    rho_min = ora_min(UU[0], epsilon[0]);
+    if (UU[0] - FF[0] < rho_min)
+        theta_u = (UU[0] - FF_LF[0] - rho_min + 1.0e-40) / (FF[0] - FF_LF[0] + 1.0e-40);
+    rho_min = ora_min(UP[0], epsilon[0]);
+    if (UP[0] + FF[0] < rho_min)
+        theta_p = (UP[0] + FF_LF[0] - rho_min + 1.0e-40) / (FF_LF[0] - FF[0] + 1.0e-40);
+    theta = ora_min(ora_max(ora_min(theta_u, theta_p), 0.0), 1.0);
+    for (int n = 0; n < E; n++) {
+        FF[n] = (1.0 - theta) * FF_LF[n] + theta * FF[n];
+        Fw[n] = (1.0 - theta) * F_LF[n] + theta * Fw[n];
+    }
+    double yi_q[ORA_MAXNS], yi_u[ORA_MAXNS], yi_qp[ORA_MAXNS], yi_up[ORA_MAXNS];
+    const double _rhoq = 1.0 / (UU[0] - FF[0]), _rhou = 1.0 / (UU[0] - FF_LF[0]);
+    const double _rhoqp = 1.0 / (UP[0] + FF[0]), _rhoup = 1.0 / (UP[0] + FF_LF[0]);
+    yi_q[NCOP] = 1.0, yi_u[NCOP] = 1.0, yi_qp[NCOP] = 1.0, yi_up[NCOP] = 1.0;
+    for (int n = 0; n < NCOP; n++) {
+        const int tid = n + 5;
+        yi_q[n] = (UU[tid] - FF[tid]) * _rhoq, yi_q[NCOP] -= yi_q[n];
+        yi_u[n] = (UU[tid] - FF_LF[tid]) * _rhou, yi_u[NCOP] -= yi_u[n];
+        yi_qp[n] = (UP[tid] + FF[tid]) * _rhoqp, yi_qp[NCOP] -= yi_qp[n];
+        yi_up[n] = (UP[tid] + FF_LF[tid]) * _rhoup, yi_up[NCOP] -= yi_up[n];
+        theta_u = 1.0, theta_p = 1.0;
+        const double temp = epsilon[n + 2];
+        if (yi_q[n] < temp) {
+            const double yi_min = ora_min(yi_u[n], temp);
+            theta_u = (yi_u[n] - yi_min + 1.0e-40) / (yi_u[n] - yi_q[n] + 1.0e-40);
+        }
+        if (yi_qp[n] < temp) {
+            const double yi_min = ora_min(yi_up[n], temp);
+            theta_p = (yi_up[n] - yi_min + 1.0e-40) / (yi_up[n] - yi_qp[n] + 1.0e-40);
+        }
+        theta = ora_min(ora_max(ora_min(theta_u, theta_p), 0.0), 1.0);
+        for (int nn = 0; nn < E; nn++) {
+            FF[n] = (1.0 - theta) * FF_LF[n] + theta * FF[n];
+            Fw[nn] = (1.0 - theta) * F_LF[nn] + theta * Fw[nn];
+        }
+    }
+}
+
+/* GetLU: FDM_Method/ConVenction_block.hpp:10-617 (no viscous terms).  eigen_block keeps its running maximum across calls:
    it is zeroed once (src/Fluids.cpp:427) and each sycl::reduction without
    initialize_to_identity combines with the stored value. */
 void ora_compute_lu(ora_fluid *f, int which)
@@ -779,6 +837,26 @@ void ora_compute_lu(ora_fluid *f, int which)
                 }
     }
     /* UpdateFluidLU: Reconstruction_kernels.hpp:201-234 */
+    /* positivity-preserving limiter, x then y then z, on the faces to the
+       right of each interior cell (ConVenction_block.hpp:328-413); lambda_0 is
+       uvw_c_max of the last GetDt */
+    if (c->positivity) {
+        double epsilon[ORA_MAXNS + 2];
+        epsilon[0] = 1.0e-13, epsilon[1] = 1.0e-13;
+        for (int ii = 2; ii < c->NS + 2; ii++) epsilon[ii] = 0.0;
+        for (int d = 0; d < 3; d++) {
+            if (!dims[d]) continue;
+            const double lambda_0 = f->uvw_c_max[d];
+            const double lambda = c->CFLnumber / lambda_0;
+#pragma omp parallel for collapse(2) schedule(static)
+            for (int k = c->Bwidth_Z; k < c->Bwidth_Z + c->Z_inner; k++)
+                for (int j = c->Bwidth_Y; j < c->Bwidth_Y + c->Y_inner; j++)
+                    for (int i = c->Bwidth_X; i < c->Bwidth_X + c->X_inner; i++) {
+                        size_t id_l = IDX(f, i, j, k);
+                        positivity_face(f, id_l, id_l + strides[d], UI, Fl[d], Fw[d], lambda_0, lambda, epsilon);
+                    }
+        }
+    }
 #pragma omp parallel for collapse(2) schedule(static)
     for (int k = c->Bwidth_Z; k < c->Bwidth_Z + c->Z_inner; k++)
         for (int j = c->Bwidth_Y; j < c->Bwidth_Y + c->Y_inner; j++)

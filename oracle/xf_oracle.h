@@ -68,6 +68,9 @@ typedef struct {
     /* 1: transcendental functions from libm (the reference CPU build's
        behaviour); 0: the deterministic exp/log shared by the HIP kernels */
     int use_libm;
+    /* PositivityPreserving (settings equations block): the flux limiter of
+       PositivityPreserving_kernels.hpp:5-76 on the face fluxes */
+    int positivity;
 } ora_config;
 
 typedef struct ora_fluid ora_fluid;

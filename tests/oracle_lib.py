@@ -31,7 +31,7 @@ class OraConfig(C.Structure):
                 ("Nu_f", C.c_int * (MAXNR * MAXNS)), ("Nu_b", C.c_int * (MAXNR * MAXNS)),
                 ("Nu_d", C.c_int * (MAXNR * MAXNS)), ("third_ind", C.c_int * MAXNR),
                 ("React_ThirdCoef", C.c_double * (MAXNR * MAXNS)), ("Rargus", C.c_double * (MAXNR * 6)),
-                ("use_libm", C.c_int)]
+                ("use_libm", C.c_int), ("positivity", C.c_int)]
 
 
 _lib = None

@@ -14,7 +14,9 @@ CASES = {
     "shock_bubble_js_nasa": ("shock-bubble", "shock-bubble", dict(nx=96, ny=48, scheme=0, thermo=1), 6),
     "shock_bubble_cu6": ("shock-bubble", "shock-bubble", dict(nx=96, ny=48, scheme=2), 6),
     "vortex_cu6": ("isentropic-vortex", "isentropic-vortex", dict(nx=48, ny=48, scheme=2), 8),
+    "shock_bubble_pp": ("shock-bubble", "shock-bubble", dict(nx=96, ny=48, positivity=1), 6),
     "reactive_shock_tube": ("reactive-shock-tube", "reactive-shock-tube", dict(nx=512), 8),
+    "reactive_shock_tube_pp": ("reactive-shock-tube", "reactive-shock-tube", dict(nx=512, positivity=1), 8),
     "reactive_channel": ("reactive-channel", "reactive-channel", dict(nx=24, ny=24), 3),
     "mixing_layer_3d": ("mixing-layer", "mixing-layer", dict(nx=16, ny=16, nz=16), 2),
 }

@@ -13,7 +13,15 @@ RK3, time step) against closed forms and library routines.
 * the GLF running maxima never decrease over a run;
 * the time step is CFL / sum_d max(|u_d| + c) / dx_d over every cell, ghosts
   included, with c = sqrt(1.4 p / rho) (GlobalDt_block.hpp:6-114; for the
-  gamma = 1.4 vortex that is the acoustic CFL limit itself).
+  gamma = 1.4 vortex that is the acoustic CFL limit itself);
+* the positivity-preserving limiter (PositivityPreserving_kernels.hpp:5-76):
+  on a state with near-vacuum cells the forward-Euler stage rho + dt LU stays
+  non-negative in every interior cell (the limited face fluxes make it a
+  convex combination of half-states with rho - 2 lambda F >= min(rho, 1e-13),
+  the weights lambda0_d / dx_d / sum_e lambda0_e / dx_e summing to one),
+  while without the limiter it goes negative; on a state with vanishing mass
+  fractions the negative excursion of rho Y shrinks by orders of magnitude;
+  on Sod's tube (no half-state below the floors) it changes nothing.
 """
 import numpy as np
 import pytest
@@ -235,3 +243,145 @@ def test_time_step_is_acoustic_cfl_limit(case, wl, over):
             inv += (np.abs(o.field(name)) + c).max() / h
     want = cfg.CFLnumber / inv
     assert abs(o.get_dt() - want) <= 1e-14 * want
+
+
+def _limiter_stage(pos, vacuum, seed=3):
+    """One GetDt + GetLU on a perturbed shock-bubble state; returns the
+    interior forward-Euler stage U + dt LU.  vacuum: 30 % of the cells scaled
+    down by 1e-6 (velocity, mass fractions and T kept); otherwise the first
+    species' mass fraction is set to 1e-12 in half of the cells."""
+    s = xf.Setup(xf.case_path("shock-bubble"), nx=48, ny=32, positivity=pos)
+    d = s.desc
+    U, T = s.initial("shock-bubble", seed=seed)
+    A = U.reshape(d.Zmax, d.Ymax, d.Xmax, d.E).copy()
+    rng = np.random.default_rng(seed)
+    rho = A[..., 0].copy()
+    m0 = A[..., 1].copy()
+    A[..., 1] += rho * 300.0 * rng.standard_normal(rho.shape)
+    A[..., 4] += 0.5 * (A[..., 1] ** 2 - m0 ** 2) / rho
+    if vacuum:
+        A[rng.random(rho.shape) < 0.3] *= 1e-6
+    else:
+        Y = A[..., 5:] / rho[..., None]
+        Y[..., 0] = np.where(rng.random(rho.shape) < 0.5, 1e-12, Y[..., 0])
+        A[..., 5:] = Y * rho[..., None]
+    o = Oracle(config_from_setup(s))
+    assert o.cfg.positivity == pos
+    o.set_state(A.ravel(), T)
+    assert o.initialize() == 0
+    dt = o.get_dt()
+    o.compute_lu(0)
+    Un = o.field("U").reshape(A.shape) + dt * o.field("LU").reshape(A.shape)
+    return Un[:, d.Bwidth_Y:d.Bwidth_Y + d.Y_inner, d.Bwidth_X:d.Bwidth_X + d.X_inner]
+
+
+def test_positivity_limiter_keeps_density_non_negative():
+    off = _limiter_stage(0, vacuum=True)[..., 0]
+    on = _limiter_stage(1, vacuum=True)[..., 0]
+    assert off.min() < -1e-3          # measured -3.1e-2
+    assert on.min() >= -1e-12, on.min()  # measured +1.1e-6
+
+
+@pytest.mark.parametrize("seed", [1, 3, 4])
+def test_positivity_limiter_shrinks_negative_mass_fractions(seed):
+    # the species part limits in mass-fraction space with the half-state
+    # densities of the density-limited and Lax-Friedrichs fluxes, so rho Y
+    # is not bounded exactly; the most negative value shrinks by 7.7x, 24x
+    # and 450x for seeds 4, 1 and 3
+    off = _limiter_stage(0, vacuum=False, seed=seed)[..., 5:].min()
+    on = _limiter_stage(1, vacuum=False, seed=seed)[..., 5:].min()
+    assert off < -5e-4, off
+    assert on > off / 5.0, (off, on)
+
+
+def test_positivity_limiter_inactive_on_sod():
+    n = 200
+    out = []
+    for pos in (0, 1):
+        s = xf.Setup(xf.case_path("reactive-shock-tube"), nx=n)
+        cfg = config_from_setup(s)
+        cfg.cop, cfg.NS, cfg.NR, cfg.react, cfg.strang = 0, 1, 0, 0, 0
+        cfg.ncop_gamma, cfg.scheme, cfg.positivity = 1.4, 1, pos
+        cfg.dx, cfg._dx = 1.0 / n, float(n)
+        for q in range(6):
+            cfg.BC[q] = BC["outflow"]
+        o = Oracle(cfg)
+        x = (np.arange(cfg.Xmax) - cfg.Bwidth_X + 0.5) / n
+        U = np.zeros((cfg.Xmax, 5))
+        U[:, 0] = np.where(x < 0.5, 1.0, 0.125)
+        U[:, 4] = np.where(x < 0.5, 1.0, 0.1) / 0.4
+        o.set_state(U.ravel(), np.zeros(cfg.Xmax))
+        assert o.initialize() == 0
+        st, _ = o.steps(40)
+        assert st == 0
+        out.append(o.field("U").copy())
+    assert np.array_equal(out[0], out[1])
+
+
+def test_positivity_limiter_binds_density_half_states():
+    """NO-COP tube with near-vacuum cells: every face the limiter changes ends
+    with one of its half-states rho -+ 2 lambda F at the floor min(rho, 1e-13)
+    (to rounding); every face it leaves alone already had both above it."""
+    n = 200
+    out = []
+    for pos in (0, 1):
+        s = xf.Setup(xf.case_path("reactive-shock-tube"), nx=n)
+        cfg = config_from_setup(s)
+        cfg.cop, cfg.NS, cfg.NR, cfg.react, cfg.strang = 0, 1, 0, 0, 0
+        cfg.ncop_gamma, cfg.scheme, cfg.positivity = 1.4, 1, pos
+        cfg.dx, cfg._dx = 1.0 / n, float(n)
+        for q in range(6):
+            cfg.BC[q] = BC["outflow"]
+        o = Oracle(cfg)
+        if not out:
+            r = np.random.default_rng(11)
+            vac = r.random(cfg.Xmax) < 0.3
+            rho = np.where(vac, 1e-6, 1.0)
+            p = np.where(vac, 1e-6, 1.0)
+            u = 2.0 * r.standard_normal(cfg.Xmax)
+            U = np.zeros((cfg.Xmax, 5))
+            U[:, 0], U[:, 1], U[:, 4] = rho, rho * u, p / 0.4 + 0.5 * rho * u * u
+        o.set_state(U.ravel(), np.zeros(cfg.Xmax))
+        assert o.initialize() == 0
+        o.get_dt()
+        o.compute_lu(0)
+        lam = cfg.CFLnumber / o.field("uvw_c_max")[0]
+        out.append((o.field("U").reshape(-1, 5)[:, 0].copy(), o.field("FluxFw").reshape(-1, 5)[:, 0].copy(), lam))
+    (R, F0, lam), (_, F1, _) = out
+    B = cfg.Bwidth_X
+    i = np.arange(B, B + n)
+    ul, ur = R[i], R[i + 1]
+    fl0, fl1 = 2 * lam * F0[i], 2 * lam * F1[i]
+    mn_l, mn_r = np.minimum(ul, 1e-13), np.minimum(ur, 1e-13)
+    tol = 1e-14 * (ul + ur + np.abs(fl0) + np.abs(fl1))
+    changed = F0[i] != F1[i]
+    assert changed.sum() > 10
+    # untouched faces: both half-states already at or above the floor
+    assert np.all(ul[~changed] - fl0[~changed] >= mn_l[~changed] - tol[~changed])
+    assert np.all(ur[~changed] + fl0[~changed] >= mn_r[~changed] - tol[~changed])
+    # limited faces: both at or above the floor, and the binding one on it
+    hl, hr = ul - fl1 - mn_l, ur + fl1 - mn_r
+    c = changed
+    assert np.all(hl[c] >= -tol[c]) and np.all(hr[c] >= -tol[c])
+    assert np.all(np.minimum(hl[c], hr[c]) <= tol[c]), np.minimum(hl[c], hr[c]).max()
+
+
+def test_positivity_limiter_inactive_on_mixed_gas():
+    """Shock-bubble with every mass fraction inside [0.05, 0.95]: no half-state
+    reaches a floor, so the run is the same with and without the limiter."""
+    out = []
+    for pos in (0, 1):
+        s = xf.Setup(xf.case_path("shock-bubble"), nx=48, ny=24, positivity=pos)
+        d = s.desc
+        U, T = s.initial("shock-bubble", seed=5)
+        A = U.reshape(-1, d.E).copy()
+        rng = np.random.default_rng(5)
+        Y = np.stack([0.3 + 0.1 * rng.random(len(A)), 0.2 + 0.1 * rng.random(len(A))], axis=1)
+        A[:, 5:7] = Y * A[:, :1]
+        o = Oracle(config_from_setup(s))
+        o.set_state(A.ravel(), T)
+        assert o.initialize() == 0
+        st, _ = o.steps(4)
+        assert st == 0
+        out.append(o.field("U").copy())
+    assert np.array_equal(out[0], out[1])

@@ -45,7 +45,8 @@ class CaseDesc(C.Structure):
                 ("CFLnumber", C.c_double), ("BC", C.c_int * 6),
                 ("NS", C.c_int), ("NR", C.c_int), ("cop", C.c_int), ("thermo", C.c_int),
                 ("scheme", C.c_int), ("react", C.c_int), ("strang", C.c_int),
-                ("ncop_gamma", C.c_double), ("precision", C.c_int), ("chem_driver", C.c_int)]
+                ("ncop_gamma", C.c_double), ("precision", C.c_int), ("chem_driver", C.c_int),
+                ("positivity", C.c_int)]
 
     @property
     def E(self):
@@ -113,6 +114,7 @@ _SIGS = {
     "xf_setup_load": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(_V), C.c_char_p, C.c_size_t]),
     "xf_setup_free": (None, [_V]),
     "xf_setup_override": (C.c_int, [_V, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int]),
+    "xf_setup_set_positivity": (C.c_int, [_V, C.c_int]),
     "xf_setup_desc": (C.c_int, [_V, C.POINTER(CaseDesc)]),
     "xf_setup_thermo": (C.c_int, [_V, _DP, _DP, _DP]),
     "xf_setup_mechanism": (C.c_int, [_V, _IP, _IP, _DP, _DP]),
@@ -177,7 +179,8 @@ def case_path(name):
 class Setup:
     """Case setup from a settings JSON (reference schema) + runtime.dat."""
 
-    def __init__(self, settings, runtime_root=RUNTIME_DAT, nx=0, ny=0, nz=0, cfl=-1.0, scheme=-1, thermo=-1):
+    def __init__(self, settings, runtime_root=RUNTIME_DAT, nx=0, ny=0, nz=0, cfl=-1.0, scheme=-1, thermo=-1,
+                 positivity=-1):
         L = lib()
         h = C.c_void_p()
         err = C.create_string_buffer(512)
@@ -189,6 +192,10 @@ class Setup:
             st = L.xf_setup_override(h, nx, ny, nz, cfl, scheme, thermo)
             if st:
                 raise XfError(st, "xf_setup_override")
+        if positivity >= 0:
+            st = L.xf_setup_set_positivity(h, int(positivity))
+            if st:
+                raise XfError(st, "xf_setup_set_positivity")
 
     def __del__(self):
         if getattr(self, "_h", None) and _lib is not None:

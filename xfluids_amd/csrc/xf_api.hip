@@ -216,6 +216,7 @@ int xf_create(const xf_case_desc *desc, xf_ctx **out)
     g.DimX = d.DimX, g.DimY = d.DimY, g.DimZ = d.DimZ;
     g._dx = d._dx, g._dy = d._dy, g._dz = d._dz, g.CFL = d.CFLnumber;
     g.dx = d.dx, g.dy = d.dy, g.dz = d.dz;
+    g.positivity = d.positivity ? 1 : 0;
     g.N = c->N;
     auto fail = [&](int code) {
         xf_destroy(c);

@@ -307,7 +307,7 @@ static __global__ void k_reset_pass_red(double *red)
 }
 
 // GetDt: dt = CFL / sum_d max_d / Delta_d, written to device memory
-static __global__ void k_dt(Geom g, const double *red, double *dt)
+static __global__ void k_dt(Geom g, double *red, double *dt)
 {
     if (threadIdx.x == 0) {
         const double mx = g.DimX ? red[RED_DT + 0] : 0.0;
@@ -315,6 +315,8 @@ static __global__ void k_dt(Geom g, const double *red, double *dt)
         const double mz = g.DimZ ? red[RED_DT + 2] : 0.0;
         const double dtref = mx * g._dx + my * g._dy + mz * g._dz;
         dt[0] = g.CFL / dtref;
+        // uvw_c_max as GetDt leaves it (the positivity limiter's lambda_0)
+        red[RED_LAM + 0] = mx, red[RED_LAM + 1] = my, red[RED_LAM + 2] = mz;
     }
 }
 

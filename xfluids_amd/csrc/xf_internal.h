@@ -52,6 +52,7 @@ struct Geom {
     double _dx, _dy, _dz, CFL;
     long long N; // padded cells
     double dx, dy, dz; // cell sizes (WENO-CU6 epsilon = 1e-8 dl^2)
+    int positivity;    // positivity-preserving flux limiter on
 };
 
 // Device pointers of one case
@@ -78,6 +79,9 @@ struct Fields {
 constexpr int RED_GLF = 0;
 constexpr int RED_GLFP = 9;
 constexpr int RED_DT = 18;
+// uvw_c_max of the last GetDt (the CFL maxima k_dt used), read by the
+// positivity-preserving limiter: RED_LAM + d
+constexpr int RED_LAM = 21;
 constexpr int RED_SIZE = 32;
 
 // error code bits in err[0]

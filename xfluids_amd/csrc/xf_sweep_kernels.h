@@ -401,6 +401,58 @@ __device__ __forceinline__ double glf_alpha(const double *red, int n)
     return b > a ? b : a;
 }
 
+// PositivityPreservingKernel (positive-definite_eigen/PositivityPreserving_kernels.hpp:5-76)
+// on the face flux Fw of one lane: UL/UR, FL/FR are the conservative state
+// and physical flux of the left and right cells.  Same operations as the
+// reference; the species loop's FF[n] updates (indexed by the species
+// counter) feed nothing and are left out.
+template <int E, int NCOP>
+__device__ __forceinline__ void positivity_face(double *Fw, const double *UL, const double *UR, const double *FL,
+                                                const double *FR, double lambda_0, double lambda)
+{
+    const double eps_rho = 1.0e-13, eps_y = 0.0;
+    double F_LF[E], FF_LF[E], FF[E];
+#pragma unroll
+    for (int n = 0; n < E; n++) {
+        F_LF[n] = 0.5 * (FL[n] + FR[n] + lambda_0 * (UL[n] - UR[n]));
+        FF_LF[n] = 2.0 * lambda * F_LF[n];
+        FF[n] = 2.0 * lambda * Fw[n];
+    }
+    double theta_u = 1.0, theta_p = 1.0;
+    double rho_min = dmin(UL[0], eps_rho);
+    if (UL[0] - FF[0] < rho_min) theta_u = (UL[0] - FF_LF[0] - rho_min + 1.0e-40) / (FF[0] - FF_LF[0] + 1.0e-40);
+    rho_min = dmin(UR[0], eps_rho);
+    if (UR[0] + FF[0] < rho_min) theta_p = (UR[0] + FF_LF[0] - rho_min + 1.0e-40) / (FF_LF[0] - FF[0] + 1.0e-40);
+    double theta = dmin(dmax(dmin(theta_u, theta_p), 0.0), 1.0);
+#pragma unroll
+    for (int n = 0; n < E; n++) {
+        FF[n] = (1.0 - theta) * FF_LF[n] + theta * FF[n];
+        Fw[n] = (1.0 - theta) * F_LF[n] + theta * Fw[n];
+    }
+    if (NCOP > 0) {
+        const double _rhoq = 1.0 / (UL[0] - FF[0]), _rhou = 1.0 / (UL[0] - FF_LF[0]);
+        const double _rhoqp = 1.0 / (UR[0] + FF[0]), _rhoup = 1.0 / (UR[0] + FF_LF[0]);
+#pragma unroll
+        for (int n = 0; n < NCOP; n++) {
+            const int tid = n + 5;
+            const double yq = (UL[tid] - FF[tid]) * _rhoq, yu = (UL[tid] - FF_LF[tid]) * _rhou;
+            const double yqp = (UR[tid] + FF[tid]) * _rhoqp, yup = (UR[tid] + FF_LF[tid]) * _rhoup;
+            theta_u = 1.0, theta_p = 1.0;
+            if (yq < eps_y) {
+                const double yi_min = dmin(yu, eps_y);
+                theta_u = (yu - yi_min + 1.0e-40) / (yu - yq + 1.0e-40);
+            }
+            if (yqp < eps_y) {
+                const double yi_min = dmin(yup, eps_y);
+                theta_p = (yup - yi_min + 1.0e-40) / (yup - yqp + 1.0e-40);
+            }
+            theta = dmin(dmax(dmin(theta_u, theta_p), 0.0), 1.0);
+#pragma unroll
+            for (int nn = 0; nn < E; nn++) Fw[nn] = (1.0 - theta) * F_LF[nn] + theta * Fw[nn];
+        }
+    }
+}
+
 // Waves per SIMD the sweep is compiled for (the register budget per lane is
 // 512 / waves): the species count sets how much state a face lane carries
 template <int NS> struct SweepOccupancy {
@@ -743,6 +795,34 @@ k_sweep(Geom g, const DevThermo *__restrict__ th, const double *__restrict__ Ug,
             flux[4] += fl * R4;
         }
         dense_row(E - 1);
+        // positivity-preserving limiter on the faces right of interior cells
+        if (g.positivity && sl >= B) {
+            const double lambda_0 = f.red[RED_LAM + DIR];
+            const double lambda = g.CFL / lambda_0;
+            double UL[E], UR[E], FL[E], FR[E];
+            auto cell_flux = [&](int li, double *Uc, double *Fc) {
+#pragma unroll
+                for (int n = 0; n < E; n++) Uc[n] = sU[n * NC + li];
+                if (!MIX) {
+#pragma unroll
+                    for (int n = 0; n < E; n++) Fc[n] = sF[n * NC + li];
+                } else {
+                    // the physical flux exactly as staged for the plain form
+                    const double rho1 = sR1[li], p = sP[li];
+                    const double vel[3] = {Uc[1] * rho1, Uc[2] * rho1, Uc[3] * rho1};
+                    const double Um = Uc[1 + DIR];
+                    Fc[0] = Um;
+#pragma unroll
+                    for (int k = 0; k < 3; k++) Fc[1 + k] = (k == DIR) ? Um * vel[k] + p : Um * vel[k];
+                    Fc[4] = (Uc[4] + p) * vel[DIR];
+#pragma unroll
+                    for (int ii = 5; ii < E; ii++) Fc[ii] = Um * (Uc[ii] * rho1);
+                }
+            };
+            cell_flux(li_l, UL, FL);
+            cell_flux(li_r, UR, FR);
+            positivity_face<E, NCOP>(flux, UL, UR, FL, FR, lambda_0, lambda);
+        }
     }
 
     // ---- flux divergence on the TS-1 cells of the tile ----------------------

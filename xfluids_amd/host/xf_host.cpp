@@ -101,6 +101,13 @@ int xf_setup_override(xf_setup *h, int nx, int ny, int nz, double cfl, int schem
     return XF_OK;
 }
 
+int xf_setup_set_positivity(xf_setup *h, int on)
+{
+    if (!h || (on != 0 && on != 1)) return XF_EINVAL;
+    h->s.desc.positivity = on;
+    return XF_OK;
+}
+
 int xf_setup_desc(const xf_setup *h, xf_case_desc *out)
 {
     if (!h || !out) return XF_EINVAL;

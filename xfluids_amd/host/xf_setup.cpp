@@ -218,6 +218,8 @@ bool setup_load(const std::string &settings, const std::string &runtime_root, Se
     d.NR = s.NR;
     // Sources_React defaults to COP_CHEME: on for reacting mixtures
     d.react = (int)eq->value("Sources_React", (d.cop && s.NR > 0) ? 1.0 : 0.0) && s.NR > 0;
+    // equations.PositivityPreserving (settings/*.json): the flux limiter
+    d.positivity = eq->value("PositivityPreserving", 0.0) != 0.0 ? 1 : 0;
     s.Y_out.assign(NS, 1.0);
     s.Y_in.assign(NS, 1.0);
     if (d.cop) {

@@ -83,6 +83,11 @@ typedef struct xf_case_desc {
     int chem_driver;    /* XF_CHEM_PER_CELL or XF_CHEM_QUEUE */
     int positivity;     /* PositivityPreserving flux limiter on the face fluxes
                            (PositivityPreserving_kernels.hpp:5-76) */
+    int visc;           /* Visc: viscous stresses in the wall fluxes (CMakeLists.txt:55) */
+    int visc_heat;      /* Visc_Heat: Fourier heat conduction (CMakeLists.txt:56) */
+    int visc_diffu;     /* Visc_Diffu: species diffusion (CMakeLists.txt:57) */
+    double Yil_limiter; /* BlSz.Yil_limiter and BlSz.Dim_limiter */
+    double Dim_limiter; /*   (src/read_ini/src/iniset.cpp:358-359) */
 } xf_case_desc;
 
 typedef struct xf_ctx xf_ctx;
@@ -110,6 +115,13 @@ int xf_set_thermo(xf_ctx *ctx, const double *Hia, const double *Hib, const doubl
    Setup::ReadReactions (src/read_ini/src/reaction.cpp:6-138). */
 int xf_set_mechanism(xf_ctx *ctx, const int *Nu_f, const int *Nu_b, const double *React_ThirdCoef,
                      const double *Rargus);
+
+/* Transport fits in the reference's layout, each a cubic in ln T:
+   fitted_coefficients_visc[NS][4] (ln mu_k, Pa s), fitted_coefficients_therm[NS][4]
+   (ln lambda_k, W/(m K)) and Dkj_matrix[NS*NS][4] (ln(p D_kj), D in cm^2/s,
+   p in Pa).  Replaces the device copies of Setup::GetFitCoefficient
+   (src/read_ini/src/viscfit.cpp:148-193).  Needs xf_set_thermo first (Wi). */
+int xf_set_transport(xf_ctx *ctx, const double *fcv, const double *fct, const double *Dkj);
 
 /* ---- state exchange (AoS host <-> SoA device) ---------------------------- */
 /* which: 0 = U, 1 = U1.  T is the per-cell temperature (Newton initial guess).

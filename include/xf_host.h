@@ -27,6 +27,19 @@ int xf_setup_override(xf_setup *s, int nx, int ny, int nz, double cfl, int schem
 /* Switch the positivity-preserving flux limiter on (1) or off (0), as the
    settings key equations.PositivityPreserving does (src/read_ini/settings/read_json.cpp:68). */
 int xf_setup_set_positivity(xf_setup *s, int on);
+/* Switch the viscous terms (Visc, Visc_Heat, Visc_Diffu of CMakeLists.txt:55-57,
+   settings keys equations.Visc / Visc_Heat / Visc_Diffu here); fits the
+   transport properties (Setup::GetFitCoefficient, viscfit.cpp:148-193) when
+   visc is switched on.  Needs a multi-component mixture. */
+int xf_setup_set_viscous(xf_setup *s, int visc, int heat, int diffu);
+/* The fitted transport tables of xf_set_transport: fcv[NS*4], fct[NS*4],
+   Dkj[NS*NS*4] (zeros when the case has no viscous terms). */
+int xf_setup_transport(const xf_setup *s, double *fcv, double *fct, double *Dkj);
+/* The kinetic-theory values the fits are made from (Setup::viscosity,
+   thermal_conductivities and Dkj, viscfit.cpp:444-555): what = 0 mu_k [Pa s],
+   1 lambda_k [W/(m K)] at pressure p, 2 D_kj [cm^2/s] at pressure p [Pa].
+   NaN on error. */
+double xf_setup_transport_exact(const xf_setup *s, int what, int k, int j, double T, double p);
 int xf_setup_desc(const xf_setup *s, xf_case_desc *out);
 /* Thermal tables Hia[NS*21], Hib[NS*6], Wi[NS] (kg/mol). */
 int xf_setup_thermo(const xf_setup *s, double *Hia, double *Hib, double *Wi);

@@ -60,6 +60,7 @@ ora_fluid *ora_create(const ora_config *cfg)
     /* eigen_block zeroed once at allocation: src/Fluids.cpp:421-429 */
     memset(f->eigen_block, 0, sizeof(f->eigen_block));
     memset(f->uvw_c_max, 0, sizeof(f->uvw_c_max));
+    ora_visc_alloc(f);
     return f;
 }
 
@@ -70,6 +71,7 @@ void ora_destroy(ora_fluid *f)
                       f->eigen_local_x, f->eigen_local_y, f->eigen_local_z, f->rho, f->p, f->u, f->v, f->w,
                       f->e, f->gamma, f->T, f->H, f->c, f->Cp, f->Ri, f->y};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
+    ora_visc_free(f);
     free(f);
 }
 
@@ -82,6 +84,7 @@ double *ora_field(ora_fluid *f, const char *name)
     F(U) F(U1) F(LU) F(rho) F(p) F(u) F(v) F(w) F(e) F(y) F(gamma) F(T) F(H) F(c) F(Cp) F(Ri)
     F(FluxF) F(FluxG) F(FluxH) F(FluxFw) F(FluxGw) F(FluxHw)
     F(eigen_local_x) F(eigen_local_y) F(eigen_local_z) F(eigen_block) F(uvw_c_max)
+    F(Vde) F(va) F(tca) F(Dkm) F(hi)
 #undef F
     return NULL;
 }
@@ -857,6 +860,9 @@ void ora_compute_lu(ora_fluid *f, int which)
                     }
         }
     }
+    /* viscous, heat-conduction and diffusion wall fluxes
+       (ConVenction_block.hpp:424-575), after the limiter */
+    if (c->visc) ora_viscous_fluxes(f);
 #pragma omp parallel for collapse(2) schedule(static)
     for (int k = c->Bwidth_Z; k < c->Bwidth_Z + c->Z_inner; k++)
         for (int j = c->Bwidth_Y; j < c->Bwidth_Y + c->Y_inner; j++)

@@ -18,7 +18,15 @@ struct ora_fluid {
     double *eigen_local_x, *eigen_local_y, *eigen_local_z;
     double eigen_block[3 * ORA_MAXE];
     double uvw_c_max[6];
+    /* viscous terms (cfg.visc): velocity gradients Vde[9][n], Wilke viscosity
+       and conductivity, mixture-averaged D_km (species-major), species
+       enthalpies (AoS, NS * id + k) */
+    double *Vde, *va, *tca, *Dkm, *hi;
 };
+
+void ora_visc_alloc(ora_fluid *f);
+void ora_visc_free(ora_fluid *f);
+void ora_viscous_fluxes(ora_fluid *f);
 
 /* transcendental dispatch: libm or the deterministic restatement */
 double ora_xexp(double x);

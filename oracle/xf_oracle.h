@@ -71,6 +71,15 @@ typedef struct {
     /* PositivityPreserving (settings equations block): the flux limiter of
        PositivityPreserving_kernels.hpp:5-76 on the face fluxes */
     int positivity;
+    /* Visc, Visc_Heat, Visc_Diffu (CMakeLists.txt:55-57, compile-time in the
+       reference): viscous stresses, heat conduction and species diffusion */
+    int visc, visc_heat, visc_diffu;
+    /* BlSz.Yil_limiter and BlSz.Dim_limiter as Setup::init forms them
+       (src/read_ini/src/iniset.cpp:358-359) */
+    double Yil_limiter, Dim_limiter;
+    /* transport fits of Setup::GetFitCoefficient (src/read_ini/src/viscfit.cpp:148-193):
+       ln(mu_k), ln(lambda_k) and ln(p D_kj) as cubics in ln T */
+    double fcv[ORA_MAXNS * 4], fct[ORA_MAXNS * 4], Dkj[ORA_MAXNS * ORA_MAXNS * 4];
 } ora_config;
 
 typedef struct ora_fluid ora_fluid;

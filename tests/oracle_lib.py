@@ -31,7 +31,11 @@ class OraConfig(C.Structure):
                 ("Nu_f", C.c_int * (MAXNR * MAXNS)), ("Nu_b", C.c_int * (MAXNR * MAXNS)),
                 ("Nu_d", C.c_int * (MAXNR * MAXNS)), ("third_ind", C.c_int * MAXNR),
                 ("React_ThirdCoef", C.c_double * (MAXNR * MAXNS)), ("Rargus", C.c_double * (MAXNR * 6)),
-                ("use_libm", C.c_int), ("positivity", C.c_int)]
+                ("use_libm", C.c_int), ("positivity", C.c_int),
+                ("visc", C.c_int), ("visc_heat", C.c_int), ("visc_diffu", C.c_int),
+                ("Yil_limiter", C.c_double), ("Dim_limiter", C.c_double),
+                ("fcv", C.c_double * (MAXNS * 4)), ("fct", C.c_double * (MAXNS * 4)),
+                ("Dkj", C.c_double * (MAXNS * MAXNS * 4))]
 
 
 _lib = None
@@ -120,6 +124,13 @@ def config_from_setup(setup, use_libm=0, **over):
                 c.third_ind[r] = 1 if s > 0.0 else 0
                 for q in range(6):
                     c.Rargus[r * 6 + q] = Rargus[r * 6 + q]
+        if d.visc:
+            fcv, fct, Dkj = setup.transport_tables()
+            for q in range(NS * 4):
+                c.fcv[q] = fcv[q]
+                c.fct[q] = fct[q]
+            for q in range(NS * NS * 4):
+                c.Dkj[q] = Dkj[q]
     for k, v in over.items():
         setattr(c, k, v)
     return c
@@ -143,7 +154,8 @@ class Oracle:
     def field(self, name, n=None):
         sizes = {"U": self.E, "U1": self.E, "LU": self.E, "FluxF": self.E, "FluxG": self.E, "FluxH": self.E,
                  "FluxFw": self.E, "FluxGw": self.E, "FluxHw": self.E, "eigen_local_x": self.E,
-                 "eigen_local_y": self.E, "eigen_local_z": self.E, "y": self.cfg.NS}
+                 "eigen_local_y": self.E, "eigen_local_z": self.E, "y": self.cfg.NS, "Vde": 9,
+                 "Dkm": self.cfg.NS, "hi": self.cfg.NS}
         if name == "eigen_block":
             cnt = 3 * MAXE
         elif name == "uvw_c_max":

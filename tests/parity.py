@@ -19,6 +19,8 @@ CASES = {
     "reactive_shock_tube_pp": ("reactive-shock-tube", "reactive-shock-tube", dict(nx=512, positivity=1), 8),
     "reactive_channel": ("reactive-channel", "reactive-channel", dict(nx=24, ny=24), 3),
     "mixing_layer_3d": ("mixing-layer", "mixing-layer", dict(nx=16, ny=16, nz=16), 2),
+    "shock_bubble_visc": ("shock-bubble", "shock-bubble", dict(nx=96, ny=48, viscous=(1, 1, 1)), 6),
+    "mixing_layer_3d_visc": ("mixing-layer", "mixing-layer", dict(nx=16, ny=16, nz=16, viscous=(1, 1, 1)), 2),
 }
 
 

@@ -46,7 +46,8 @@ class CaseDesc(C.Structure):
                 ("NS", C.c_int), ("NR", C.c_int), ("cop", C.c_int), ("thermo", C.c_int),
                 ("scheme", C.c_int), ("react", C.c_int), ("strang", C.c_int),
                 ("ncop_gamma", C.c_double), ("precision", C.c_int), ("chem_driver", C.c_int),
-                ("positivity", C.c_int)]
+                ("positivity", C.c_int), ("visc", C.c_int), ("visc_heat", C.c_int), ("visc_diffu", C.c_int),
+                ("Yil_limiter", C.c_double), ("Dim_limiter", C.c_double)]
 
     @property
     def E(self):
@@ -88,6 +89,7 @@ _SIGS = {
     "xf_error_string": (C.c_char_p, [C.c_int]),
     "xf_set_thermo": (C.c_int, [_V, _DP, _DP, _DP]),
     "xf_set_mechanism": (C.c_int, [_V, _IP, _IP, _DP, _DP]),
+    "xf_set_transport": (C.c_int, [_V, _DP, _DP, _DP]),
     "xf_upload_state": (C.c_int, [_V, C.c_int, _DP, _DP]),
     "xf_download_state": (C.c_int, [_V, C.c_int, _DP, _DP]),
     "xf_download_prims": (C.c_int, [_V] + [_DP] * 10),
@@ -115,6 +117,9 @@ _SIGS = {
     "xf_setup_free": (None, [_V]),
     "xf_setup_override": (C.c_int, [_V, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int]),
     "xf_setup_set_positivity": (C.c_int, [_V, C.c_int]),
+    "xf_setup_set_viscous": (C.c_int, [_V, C.c_int, C.c_int, C.c_int]),
+    "xf_setup_transport": (C.c_int, [_V, _DP, _DP, _DP]),
+    "xf_setup_transport_exact": (C.c_double, [_V, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double]),
     "xf_setup_desc": (C.c_int, [_V, C.POINTER(CaseDesc)]),
     "xf_setup_thermo": (C.c_int, [_V, _DP, _DP, _DP]),
     "xf_setup_mechanism": (C.c_int, [_V, _IP, _IP, _DP, _DP]),
@@ -180,7 +185,7 @@ class Setup:
     """Case setup from a settings JSON (reference schema) + runtime.dat."""
 
     def __init__(self, settings, runtime_root=RUNTIME_DAT, nx=0, ny=0, nz=0, cfl=-1.0, scheme=-1, thermo=-1,
-                 positivity=-1):
+                 positivity=-1, viscous=None):
         L = lib()
         h = C.c_void_p()
         err = C.create_string_buffer(512)
@@ -196,6 +201,12 @@ class Setup:
             st = L.xf_setup_set_positivity(h, int(positivity))
             if st:
                 raise XfError(st, "xf_setup_set_positivity")
+        if viscous is not None:
+            # (visc, heat, diffu) switches, as equations.Visc / Visc_Heat / Visc_Diffu
+            v, h_, df = (list(viscous) + [0, 0, 0])[:3]
+            st = L.xf_setup_set_viscous(h, int(v), int(h_), int(df))
+            if st:
+                raise XfError(st, "xf_setup_set_viscous")
 
     def __del__(self):
         if getattr(self, "_h", None) and _lib is not None:
@@ -218,6 +229,19 @@ class Setup:
         Hia, Hib, Wi = np.zeros(NS * 21), np.zeros(NS * 6), np.zeros(NS)
         lib().xf_setup_thermo(self._h, _d(Hia), _d(Hib), _d(Wi))
         return Hia, Hib, Wi
+
+    def transport_tables(self):
+        """Fits of ln(mu_k), ln(lambda_k), ln(p D_kj) in ln T: fcv[NS*4], fct[NS*4], Dkj[NS*NS*4]."""
+        NS = self.desc.NS
+        fcv, fct, Dkj = np.zeros(NS * 4), np.zeros(NS * 4), np.zeros(NS * NS * 4)
+        st = lib().xf_setup_transport(self._h, _d(fcv), _d(fct), _d(Dkj))
+        if st:
+            raise XfError(st, "xf_setup_transport")
+        return fcv, fct, Dkj
+
+    def transport_exact(self, what, k, j, T, p=101325.0):
+        """Kinetic-theory value behind a fit: what 0 mu_k [Pa s], 1 lambda_k [W/(m K)], 2 D_kj [cm^2/s]."""
+        return lib().xf_setup_transport_exact(self._h, int(what), int(k), int(j), float(T), float(p))
 
     def mechanism(self):
         d = self.desc

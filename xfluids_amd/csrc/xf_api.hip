@@ -3,6 +3,7 @@
 // fused device-resident time-step loop (optionally one hipGraph per step).
 #include "xf_context.h"
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -111,9 +112,14 @@ void lu_async(xf_ctx *c, const double *Ug, int rk_stage, const double *Ua, const
     if (c->g.DimX) dirs[nd++] = 0;
     if (c->g.DimY) dirs[nd++] = 1;
     if (c->g.DimZ) dirs[nd++] = 2;
+    // viscous terms (ConVenction_block.hpp:424-575): transport coefficients
+    // and centre gradients once, then each direction's wall flux right before
+    // its sweep, which subtracts it from the face flux
+    if (c->d.visc) c->ops.visc_prep(c, Ug);
     for (int q = 0; q < nd; q++) {
         int mode = (q > 0) ? 1 : 0;
         if (rk_stage > 0 && q == nd - 1) mode |= 2;
+        if (c->d.visc) c->ops.visc_flux(c, dirs[q], Ug);
         c->ops.sweep(c, dirs[q], Ug, mode, rk_stage, Ua, Ub, Uo);
     }
 }
@@ -199,6 +205,11 @@ int xf_create(const xf_case_desc *desc, xf_ctx **out)
     if (d.precision != XF_PLAIN_FP64 && d.precision != XF_MIXED) return XF_EINVAL;
     if (d.chem_driver != XF_CHEM_PER_CELL && d.chem_driver != XF_CHEM_QUEUE) return XF_EINVAL;
     if (d.scheme != XF_WENO5JS && d.scheme != XF_WENO5Z && d.scheme != XF_WENOCU6) return XF_EINVAL;
+    // the viscous terms need the species transport data of a mixture, and
+    // their centre derivatives reach 4 cells into the ghost layers
+    if (d.visc && !d.cop) return XF_EINVAL;
+    if (d.visc && ((d.DimX && d.Bwidth_X < 4) || (d.DimY && d.Bwidth_Y < 4) || (d.DimZ && d.Bwidth_Z < 4)))
+        return XF_EINVAL;
     Ops ops;
     if (!select_ops(d.NS, d.cop, ops)) return XF_ENOKERNEL;
     xf_ctx *c = new xf_ctx();
@@ -217,6 +228,7 @@ int xf_create(const xf_case_desc *desc, xf_ctx **out)
     g._dx = d._dx, g._dy = d._dy, g._dz = d._dz, g.CFL = d.CFLnumber;
     g.dx = d.dx, g.dy = d.dy, g.dz = d.dz;
     g.positivity = d.positivity ? 1 : 0;
+    g.visc = d.visc ? 1 : 0;
     g.N = c->N;
     auto fail = [&](int code) {
         xf_destroy(c);
@@ -253,6 +265,15 @@ int xf_create(const xf_case_desc *desc, xf_ctx **out)
         return fail(XF_EHIP);
     unsigned qz[4] = {0, 0, 0, 0};
     if (hipMemcpy(f.queue, qz, sizeof(qz), hipMemcpyHostToDevice)) return fail(XF_EHIP);
+    if (d.visc) {
+        const size_t nv[6] = {n1, n1, (size_t)d.NS * n1, 9 * n1, nE,
+                              (size_t)nblk(c->N) * 2 * d.NS * sizeof(double)};
+        double **pv[6] = {&f.mu, &f.kap, &f.Dkm, &f.Vde, &f.Fv, &f.ypart};
+        for (int q = 0; q < 6; q++)
+            if (hipMalloc(pv[q], nv[q]) || hipMemset(*pv[q], 0, nv[q])) return fail(XF_EHIP);
+        if (hipMalloc(&c->d_tr, sizeof(DevTransport))) return fail(XF_EHIP);
+        if (hipDeviceSynchronize()) return fail(XF_EHIP);
+    }
     c->ops.prepare(c);
     *out = c;
     return XF_OK;
@@ -266,8 +287,10 @@ void xf_destroy(xf_ctx *c)
     if (c->graph) (void)hipGraphDestroy(c->graph);
     for (auto &e : c->evs) (void)hipEventDestroy(e.a), (void)hipEventDestroy(e.b);
     Fields &f = c->f;
-    void *ptrs[] = {f.U, f.U1, f.LU, c->U2, f.rho, f.u, f.v, f.w, f.p, f.H, f.T, f.c, f.gamma, f.e, f.Cp, f.gcv, f.R,
-                    f.hi, f.red, f.dt, f.err, f.substeps, f.queue, c->d_th, c->d_mc};
+    void *ptrs[] = {f.U,   f.U1,    f.LU,       c->U2,   f.rho,   f.u,     f.v,   f.w,   f.p,   f.H,
+                    f.T,   f.c,     f.gamma,    f.e,     f.Cp,    f.gcv,   f.R,   f.hi,  f.red, f.dt,
+                    f.err, f.substeps, f.queue, c->d_th, c->d_mc, c->d_tr, f.mu,  f.kap, f.Dkm, f.Vde,
+                    f.Fv,  f.ypart};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -299,6 +322,34 @@ int xf_set_thermo(xf_ctx *c, const double *Hia, const double *Hib, const double 
     c->h_th.thermo = c->d.thermo;
     XF_HIP(hipMemcpy(c->d_th, &c->h_th, sizeof(DevThermo), hipMemcpyHostToDevice));
     c->thermo_set = true;
+    return XF_OK;
+}
+
+int xf_set_transport(xf_ctx *c, const double *fcv, const double *fct, const double *Dkj)
+{
+    if (!c || !fcv || !fct || !Dkj || !c->d.visc || !c->d_tr || !c->thermo_set) return XF_EINVAL;
+    const int NS = c->d.NS;
+    DevTransport h{};
+    for (int k = 0; k < NS; k++)
+        for (int q = 0; q < 4; q++) h.fcv[4 * k + q] = fcv[4 * k + q], h.fct[4 * k + q] = fct[4 * k + q];
+    for (int k = 0; k < NS; k++)
+        for (int j = 0; j < NS; j++)
+            for (int q = 0; q < 4; q++) {
+                // the kernel evaluates each species pair once for both
+                // orders, as the symmetric table Setup builds allows
+                if (Dkj[(k * NS + j) * 4 + q] != Dkj[(j * NS + k) * 4 + q]) return XF_EINVAL;
+                h.Dkj[(k * MAXNS + j) * 4 + q] = Dkj[(k * NS + j) * 4 + q];
+            }
+    // PHI's species-pair factors (Visc_device.h:27-35): pow(W_j/W_k, 1/4) and
+    // pow(1 + W_k/W_j, -1/2) as IEEE square roots
+    for (int k = 0; k < NS; k++)
+        for (int j = 0; j < NS; j++) {
+            const double Wk = c->h_th.Wi[k], Wj = c->h_th.Wi[j];
+            h.phiA[k * MAXNS + j] = std::sqrt(std::sqrt(Wj / Wk));
+            h.phiB[k * MAXNS + j] = 1.0 / std::sqrt(1.0 + Wk / Wj);
+        }
+    XF_HIP(hipMemcpy(c->d_tr, &h, sizeof(DevTransport), hipMemcpyHostToDevice));
+    c->transport_set = true;
     return XF_OK;
 }
 
@@ -465,6 +516,7 @@ int xf_update_states(xf_ctx *c, int which)
 int xf_compute_lu(xf_ctx *c, int which)
 {
     if (!c || which < 0 || which > 2) return XF_EINVAL;
+    if (c->d.visc && !c->transport_set) return XF_EINVAL;
     lu_async(c, which_ptr(c, which), 0, nullptr, nullptr, nullptr);
     if (int e = check_launch()) return e;
     return take_error(c);
@@ -539,6 +591,7 @@ int xf_initialize(xf_ctx *c)
 static int steps_enqueue(xf_ctx *c, int nsteps, int use_graph)
 {
     if (c->d.cop && !c->thermo_set) return XF_EINVAL;
+    if (c->d.visc && !c->transport_set) return XF_EINVAL;
     if (c->d.react && (!c->mech_set || c->d.NR <= 0)) return XF_EINVAL;
     if (use_graph) {
         if (!c->gexec) {
@@ -872,6 +925,7 @@ int xf_ensemble_create(xf_ctx *const *cases, int ncases, xf_ensemble **out)
             return XF_EINVAL;
         if (c->d.cop && !c->thermo_set) return XF_EINVAL;
         if (c->d.react && !c->mech_set) return XF_EINVAL;
+        if (c->d.visc && !c->transport_set) return XF_EINVAL;
         // one chemistry launch serves every case: the tables must agree
         if (c->d.react && (std::memcmp(&c->h_th, &c0->h_th, sizeof(DevThermo)) != 0 ||
                            std::memcmp(&c->h_mc, &c0->h_mc, sizeof(DevMech)) != 0))

@@ -31,6 +31,10 @@ struct Ops {
     void (*chem_multi)(hipStream_t, const ChemCase *, int ncases, long long total, const DevThermo *,
                        const DevMech *, unsigned long long *queue, int blocks, int record);
     int (*multi_blocks)(long long total);
+    // viscous terms of GetLU: transport coefficients and centre gradients of
+    // the state Ug, then the wall flux of one direction (before its sweep)
+    void (*visc_prep)(xf_ctx *, const double *Ug);
+    void (*visc_flux)(xf_ctx *, int dir, const double *Ug);
 };
 
 // dispatch tables, one translation unit each
@@ -53,7 +57,8 @@ struct xf_ctx {
     DevThermo *d_th = nullptr;
     DevMech h_mc{};
     DevMech *d_mc = nullptr;
-    bool thermo_set = false, mech_set = false;
+    DevTransport *d_tr = nullptr;
+    bool thermo_set = false, mech_set = false, transport_set = false;
     Ops ops{};
     bool has_ops = false;
     int chem_blocks = 0; // resident one-wave blocks of the work-queue chemistry kernel
@@ -212,7 +217,53 @@ template <int NS, bool COP> struct Launch {
         const long long nb = (long long)std::max(per_cu, 1) * std::max(ncu, 1);
         return (int)std::max(1LL, std::min(nb, want));
     }
-    static Ops ops() { return Ops{&states, &sweep, &chem, &prepare, &chem_multi, &multi_blocks}; }
+    static void visc_prep(xf_ctx *c, const double *Ug)
+    {
+        if constexpr (COP) {
+            ProfScope ps(c, "visc_prep");
+            const Geom &g = c->g;
+            hipLaunchKernelGGL((k_transport<NS>), dim3(nblk(c->N)), dim3(256), 0, c->stream, g, c->d_th, c->d_tr, Ug,
+                               c->f, c->d.visc_heat, c->d.visc_diffu);
+            if (c->d.visc_diffu)
+                hipLaunchKernelGGL((k_yreduce<NS>), dim3(1), dim3(256), 0, c->stream, c->f.ypart, (int)nblk(c->N),
+                                   c->f.red);
+            const int i0 = g.DimX ? g.Bx - 2 : 0, nx = g.DimX ? g.X_inner + 4 : 1;
+            const int j0 = g.DimY ? g.By - 2 : 0, ny = g.DimY ? g.Y_inner + 4 : 1;
+            const int k0 = g.DimZ ? g.Bz - 2 : 0, nz = g.DimZ ? g.Z_inner + 4 : 1;
+            hipLaunchKernelGGL(k_vel_deriv, dim3(nblk((long long)nx * ny * nz)), dim3(256), 0, c->stream, g, c->f, i0,
+                               nx, j0, ny, k0, nz);
+            const int dims[3] = {g.DimX, g.DimY, g.DimZ};
+            for (int d = 0; d < 3; d++) {
+                if (!dims[d]) continue;
+                const long long B = d == 0 ? g.Bx : (d == 1 ? g.By : g.Bz);
+                const long long lines = d == 0 ? (long long)g.Ymax * g.Zmax
+                                               : (d == 1 ? (long long)g.Xmax * g.Zmax : (long long)g.Xmax * g.Ymax);
+                hipLaunchKernelGGL(k_vde_bc, dim3(nblk(2 * B * lines)), dim3(256), 0, c->stream, g, c->f, d,
+                                   c->d.BC[2 * d], c->d.BC[2 * d + 1]);
+            }
+        }
+    }
+    template <int DIR> static void visc_flux_d(xf_ctx *c, const double *Ug)
+    {
+        const Geom &g = c->g;
+        const long long n = (long long)(g.X_inner + (DIR == 0)) * (g.Y_inner + (DIR == 1)) * (g.Z_inner + (DIR == 2));
+        static const char *names[3] = {"visc_x", "visc_y", "visc_z"};
+        ProfScope ps(c, names[DIR]);
+        hipLaunchKernelGGL((k_visc_flux<NS, DIR>), dim3(nblk(n)), dim3(256), 0, c->stream, g, Ug, c->f,
+                           c->d.visc_heat, c->d.visc_diffu, c->d.Yil_limiter, c->d.Dim_limiter);
+    }
+    static void visc_flux(xf_ctx *c, int dir, const double *Ug)
+    {
+        if constexpr (COP) {
+            if (dir == 0) visc_flux_d<0>(c, Ug);
+            if (dir == 1) visc_flux_d<1>(c, Ug);
+            if (dir == 2) visc_flux_d<2>(c, Ug);
+        }
+    }
+    static Ops ops()
+    {
+        return Ops{&states, &sweep, &chem, &prepare, &chem_multi, &multi_blocks, &visc_prep, &visc_flux};
+    }
 };
 
 } // namespace xf

@@ -53,6 +53,7 @@ struct Geom {
     long long N; // padded cells
     double dx, dy, dz; // cell sizes (WENO-CU6 epsilon = 1e-8 dl^2)
     int positivity;    // positivity-preserving flux limiter on
+    int visc;          // viscous wall fluxes (Fv) subtracted from the face fluxes
 };
 
 // Device pointers of one case
@@ -67,6 +68,11 @@ struct Fields {
     int *err;                          // [0] code bits, [1] min cell id
     int *substeps;                     // chemistry predictor passes per interior cell
     unsigned int *queue;               // chemistry work-queue counter
+    // viscous terms: Wilke viscosity and conductivity, mixture-averaged
+    // D_km (NS planes), centre velocity gradients (9 planes), and the
+    // viscous face flux of the direction being swept (E planes)
+    double *mu, *kap, *Dkm, *Vde, *Fv;
+    double *ypart; // per-block interior min / max of each mass fraction (2 NS per block)
 };
 
 // reduction slot layout (doubles, compared as uint64 bit patterns)
@@ -82,7 +88,11 @@ constexpr int RED_DT = 18;
 // uvw_c_max of the last GetDt (the CFL maxima k_dt used), read by the
 // positivity-preserving limiter: RED_LAM + d
 constexpr int RED_LAM = 21;
-constexpr int RED_SIZE = 32;
+// interior min / max of each mass fraction for the diffusion limiter:
+// RED_YMIN + k, RED_YMAX + k
+constexpr int RED_YMIN = 32;
+constexpr int RED_YMAX = RED_YMIN + MAXNS;
+constexpr int RED_SIZE = RED_YMAX + MAXNS;
 
 // error code bits in err[0]
 constexpr int EBIT_RHO_Y = 1;

@@ -26,6 +26,7 @@
 //    order (x, then y, then z), optionally fused with the SSP-RK3 stage.
 #pragma once
 #include "xf_fluid_kernels.h"
+#include "xf_visc_kernels.h"
 
 namespace xf {
 
@@ -822,6 +823,11 @@ k_sweep(Geom g, const DevThermo *__restrict__ th, const double *__restrict__ Ug,
             cell_flux(li_l, UL, FL);
             cell_flux(li_r, UR, FR);
             positivity_face<E, NCOP>(flux, UL, UR, FL, FR, lambda_0, lambda);
+        }
+        // viscous wall flux of this face (after the limiter, as GetLU orders them)
+        if (g.visc) {
+#pragma unroll
+            for (int n = 0; n < E; n++) flux[n] = flux[n] - f.Fv[n * N + id_l];
         }
     }
 

@@ -1,0 +1,415 @@
+// Viscous wall fluxes: velocity gradients at cell centres, mixture-averaged
+// transport coefficients and the fourth-order viscous, heat-conduction and
+// species-diffusion face fluxes that GetLU subtracts from the convective face
+// fluxes (src/solver_Reconstruction/FDM_Method/ConVenction_block.hpp:424-575).
+//
+// Kernels, in launch order for one GetLU:
+//   k_transport<NS>   every padded cell: u, v, w, Wilke mu and lambda, D_km;
+//                     per-block min/max of each interior mass fraction
+//   k_yreduce<NS>     one block: those partials into the reduction slots
+//   k_vel_deriv       fourth-order centre derivatives of u, v, w
+//   k_vde_bc          the ghost-cell derivatives each wall flux reads
+//   k_visc_flux<NS,DIR> per face of one direction; the sweep of that
+//                     direction subtracts the result from its face flux
+// All arithmetic is FP64 in the reference's operation order, with the same
+// exp/log as the oracle, so the plain form stays bit-identical.
+#pragma once
+#include "xf_fluid_kernels.h"
+
+namespace xf {
+
+// Transport fits (Setup::GetFitCoefficient, src/read_ini/src/viscfit.cpp:148-193)
+// and the species-pair factors of PHI (Visc_device.h:27-35):
+// phiA[k][j] = (W_j/W_k)^(1/4) = sqrt(sqrt(W_j/W_k)), phiB[k][j] = (1 + W_k/W_j)^(-1/2)
+struct DevTransport {
+    double fcv[MAXNS * 4], fct[MAXNS * 4], Dkj[MAXNS * MAXNS * 4];
+    double phiA[MAXNS * MAXNS], phiB[MAXNS * MAXNS];
+};
+
+// Vde component order (global_setup.h:122-133)
+enum { DUCX = 0, DVCX = 1, DWCX = 2, DUCY = 3, DVCY = 4, DWCY = 5, DUCZ = 6, DVCZ = 7, DWCZ = 8 };
+
+// sycl::min / sycl::max on doubles
+__device__ __forceinline__ double smin(double x, double y) { return y < x ? y : x; }
+__device__ __forceinline__ double smax(double x, double y) { return x < y ? y : x; }
+
+// cubic in ln T (Viscosity / Thermal_conductivity / GetDkj, Visc_device.h:9-90)
+__device__ __forceinline__ double fit4(const double *a, double lT)
+{
+    double v = a[3];
+#pragma unroll
+    for (int i = 2; i >= 0; i--) v = v * lT + a[i];
+    return v;
+}
+
+// Gettransport_coeff_aver + Get_transport_coeff_aver (Visc_kernels.hpp:205-235,
+// Visc_device.h:99-177) with get_xi (Mixing_device.h:31-44) on every padded
+// cell; also GetStates' u, v, w (Update_device.hpp:36) and, with diffusion,
+// each block's min / max of the interior mass fractions
+// (ConVenction_block.hpp:469-490) into ypart[block][2 NS]
+template <int NS>
+__global__ void __launch_bounds__(256) k_transport(Geom g, const DevThermo *__restrict__ th,
+                                                   const DevTransport *__restrict__ tr, const double *__restrict__ Ug,
+                                                   Fields f, int heat, int diffu)
+{
+    constexpr int E = NS + 4;
+    constexpr int NCOP = NS - 1;
+    const long long N = g.N;
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double ymn[NS], ymx[NS];
+#pragma unroll
+    for (int k = 0; k < NS; k++) ymn[k] = 0.0, ymx[k] = 0.0;
+    if (id < N) {
+        double U[E];
+#pragma unroll
+        for (int n = 0; n < E; n++) U[n] = Ug[n * N + id];
+        const double rho = U[0];
+        const double rho1 = 1.0 / rho;
+        double yi[NS];
+        yi[NCOP] = 1.0;
+#pragma unroll
+        for (int ii = 5; ii < E; ii++) yi[ii - 5] = U[ii] * rho1, yi[NCOP] += -yi[ii - 5];
+        f.u[id] = U[1] * rho1;
+        f.v[id] = U[2] * rho1;
+        f.w[id] = U[3] * rho1;
+        const double T = f.T[id], p = f.p[id];
+        const double lT = dlog(T);
+        // get_xi
+        double X[NS], C_total = 0.0;
+#pragma unroll
+        for (int i = 0; i < NS; i++) {
+            X[i] = yi[i] * th->_Wi[i] * 1e-3 * rho;
+            C_total = C_total + X[i];
+        }
+        const double _C_total = 1.0 / C_total;
+#pragma unroll
+        for (int i = 0; i < NS; i++) X[i] = X[i] * _C_total;
+        // Wilke mixture viscosity and conductivity (the conductivity uses the
+        // viscosity's denominators, as Visc_device.h:130-139 does)
+        double mu[NS];
+#pragma unroll
+        for (int k = 0; k < NS; k++) mu[k] = dexp(fit4(tr->fcv + 4 * k, lT));
+        double va = 0.0, tca = 0.0;
+        const double s2 = sqrt(2.0);
+#pragma unroll
+        for (int k = 0; k < NS; k++) {
+            double den = 0.0;
+#pragma unroll
+            for (int i = 0; i < NS; i++) {
+                double phi = tr->phiA[k * MAXNS + i] * sqrt(mu[k] / mu[i]);
+                phi = (phi + 1.0) * (phi + 1.0) * 0.5 / s2;
+                phi = phi * tr->phiB[k * MAXNS + i];
+                den = den + X[i] * phi;
+            }
+            const double _den = 1.0 / den;
+            va = va + X[k] * mu[k] * _den;
+            if (heat) tca = tca + X[k] * dexp(fit4(tr->fct + 4 * k, lT)) * _den;
+        }
+        f.mu[id] = va;
+        if (heat) f.kap[id] = tca;
+        if (diffu) {
+            // mixture-averaged D_km: temp2[k] = 1e-20 + sum_{i != k, i ascending}
+            // (X_i + 1e-40) / (D_ik + 1e-40), with D_ik = exp(fit(ln T)) / p.
+            // The fits are symmetric (Setup copies the lower triangle from
+            // the upper), so each pair's exponential serves both species.
+            double t2[NS];
+#pragma unroll
+            for (int k = 0; k < NS; k++) t2[k] = 1.0e-20;
+#pragma unroll
+            for (int k = 0; k < NS; k++) {
+#pragma unroll
+                for (int i = 0; i < k; i++) {
+                    const double D = dexp(fit4(tr->Dkj + 4 * (i * MAXNS + k), lT)) / p;
+                    t2[k] += (X[i] + 1.0e-40) / (D + 1.0e-40);
+                    t2[i] += (X[k] + 1.0e-40) / (D + 1.0e-40);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < NS; k++) {
+                double t1 = 0.0;
+#pragma unroll
+                for (int i = 0; i < NS; i++)
+                    if (i != k) t1 += (X[i] + 1.0e-40) * th->Wi[i];
+                double D;
+                if (t1 <= 0.0) // sycl::step(ceil(temp1), 0)
+                    D = dexp(fit4(tr->Dkj + 4 * (k * MAXNS + k), lT)) / p;
+                else
+                    D = t1 / t2[k] / rho * C_total;
+                D *= 1.0e-1;
+                f.Dkm[k * N + id] = smax(D, 1.0e-10);
+            }
+            const int i = (int)(id % g.Xmax);
+            const int j = (int)((id / g.Xmax) % g.Ymax);
+            const int kk = (int)(id / ((long long)g.Xmax * g.Ymax));
+            if (i >= g.Bx && i < g.Xmax - g.Bx && j >= g.By && j < g.Ymax - g.By && kk >= g.Bz && kk < g.Zmax - g.Bz) {
+#pragma unroll
+                for (int k = 0; k < NS; k++) ymn[k] = yi[k], ymx[k] = yi[k];
+            }
+        }
+    }
+    if (diffu) {
+        // wave min / max through shuffles, then the block's four waves in LDS
+        // (cells that are not interior contribute 0.0, the value the
+        // reference's reduction combines with)
+        __shared__ double sm[4][2 * NS];
+        const int wv = threadIdx.x >> 6;
+#pragma unroll
+        for (int k = 0; k < NS; k++) {
+            double a = ymn[k], b = ymx[k];
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) {
+                a = smin(a, __shfl_xor(a, off, 64));
+                b = smax(b, __shfl_xor(b, off, 64));
+            }
+            if ((threadIdx.x & 63) == 0) sm[wv][k] = a, sm[wv][NS + k] = b;
+        }
+        __syncthreads();
+        if (threadIdx.x < 2 * NS) {
+            const int q = threadIdx.x;
+            double v = sm[0][q];
+            for (int w = 1; w < 4; w++) v = q < NS ? smin(v, sm[w][q]) : smax(v, sm[w][q]);
+            f.ypart[(long long)blockIdx.x * 2 * NS + q] = v;
+        }
+    }
+}
+
+// the per-block partials of k_transport into red[RED_YMIN + k] (min) and
+// red[RED_YMAX + k] (max), each combined with 0.0 as the reference's
+// sycl::reduction combines with the stored yi_min = yi_max = 0
+template <int NS>
+__global__ void __launch_bounds__(256) k_yreduce(const double *__restrict__ ypart, int nblocks, double *red)
+{
+    __shared__ double s[256];
+    for (int q = 0; q < 2 * NS; q++) {
+        const bool mx = q >= NS;
+        double v = 0.0;
+        for (int b = threadIdx.x; b < nblocks; b += 256) {
+            const double x = ypart[(long long)b * 2 * NS + q];
+            v = mx ? smax(v, x) : smin(v, x);
+        }
+        s[threadIdx.x] = v;
+        __syncthreads();
+        for (int w = 128; w >= 1; w >>= 1) {
+            if (threadIdx.x < w) s[threadIdx.x] = mx ? smax(s[threadIdx.x], s[threadIdx.x + w]) : smin(s[threadIdx.x], s[threadIdx.x + w]);
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) red[(mx ? RED_YMAX + q - NS : RED_YMIN + q)] = s[0];
+        __syncthreads();
+    }
+}
+
+// GetInnerCellCenterDerivativeKernel (Fourth_Order/Visc_Order_kernels.hpp:16-71)
+// over i in [Bwidth_X - 2, Xmax - Bwidth_X + 1] per active direction (index 0
+// of an inactive one); a derivative along an inactive direction is +0
+static __global__ void __launch_bounds__(256) k_vel_deriv(Geom g, Fields f, int i0, int nx, int j0, int ny, int k0,
+                                                          int nz)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)nx * ny * nz) return;
+    const int i = i0 + (int)(t % nx), j = j0 + (int)((t / nx) % ny), k = k0 + (int)(t / ((long long)nx * ny));
+    const long long N = g.N;
+    const long long sy = g.Xmax, sz = (long long)g.Xmax * g.Ymax;
+    const long long id = sz * k + sy * j + i;
+    const double DX = g.DimX, DY = g.DimY, DZ = g.DimZ;
+    const double *u = f.u, *v = f.v, *w = f.w;
+    const double tw = 1.0 / 12.0;
+    double D[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) D[q] = 0.0;
+    if (g.DimX) {
+        const long long m1 = id - 1, m2 = id - 2, p1 = id + 1, p2 = id + 2;
+        D[DUCX] = (8.0 * (u[p1] - u[m1]) - (u[p2] - u[m2])) * g._dx * tw * DX;
+        D[DVCX] = (8.0 * (v[p1] - v[m1]) - (v[p2] - v[m2])) * g._dx * tw * DX * DY;
+        D[DWCX] = (8.0 * (w[p1] - w[m1]) - (w[p2] - w[m2])) * g._dx * tw * DX * DZ;
+    }
+    if (g.DimY) {
+        const long long m1 = id - sy, m2 = id - 2 * sy, p1 = id + sy, p2 = id + 2 * sy;
+        D[DUCY] = (8.0 * (u[p1] - u[m1]) - (u[p2] - u[m2])) * g._dy * tw * DY * DX;
+        D[DVCY] = (8.0 * (v[p1] - v[m1]) - (v[p2] - v[m2])) * g._dy * tw * DY;
+        D[DWCY] = (8.0 * (w[p1] - w[m1]) - (w[p2] - w[m2])) * g._dy * tw * DY * DZ;
+    }
+    if (g.DimZ) {
+        const long long m1 = id - sz, m2 = id - 2 * sz, p1 = id + sz, p2 = id + 2 * sz;
+        D[DUCZ] = (8.0 * (u[p1] - u[m1]) - (u[p2] - u[m2])) * g._dz * tw * DZ * DX;
+        D[DVCZ] = (8.0 * (v[p1] - v[m1]) - (v[p2] - v[m2])) * g._dz * tw * DZ * DY;
+        D[DWCZ] = (8.0 * (w[p1] - w[m1]) - (w[p2] - w[m2])) * g._dz * tw * DZ;
+    }
+#pragma unroll
+    for (int q = 0; q < 9; q++) f.Vde[q * N + id] = D[q];
+}
+
+// CenterDerivativeBCKernel{X,Y,Z} (Visc_kernels.hpp:7-203) for direction d:
+// the 2 * B ghost layers of every line of the other two axes, four
+// components each.  The z no-slip wall uses the mirror offset - k of the x
+// and y kernels (the reference's k - offset indexes outside the array).
+static __global__ void __launch_bounds__(256) k_vde_bc(Geom g, Fields f, int d, int bc_lo, int bc_hi)
+{
+    const int B = d == 0 ? g.Bx : (d == 1 ? g.By : g.Bz);
+    const int inner = d == 0 ? g.X_inner : (d == 1 ? g.Y_inner : g.Z_inner);
+    const int L = d == 0 ? g.Xmax : (d == 1 ? g.Ymax : g.Zmax);
+    const int a1 = d == 0 ? g.Ymax : g.Xmax;             // first other axis
+    const int a2 = d == 2 ? g.Ymax : g.Zmax;             // second other axis
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2LL * B * a1 * a2) return;
+    const int gq = (int)(t % (2 * B));
+    const long long line = t / (2 * B);
+    const int o1 = (int)(line % a1), o2 = (int)(line / a1);
+    const int lo = gq < B;
+    const int s = lo ? gq : gq - B + L - B;
+    const int bc = lo ? bc_lo : bc_hi;
+    const int mirror_offset = lo ? 0 : inner, index_inner = lo ? B : L - B - 1, sign = lo ? 1 : -1;
+    int tt = -1, neg = 0;
+    switch (bc) {
+    case XF_BC_SYMMETRY: tt = 2 * (B + mirror_offset) - 1 - s; break;
+    case XF_BC_PERIODIC: tt = s + sign * inner; break;
+    case XF_BC_OUTFLOW: tt = index_inner; break;
+    case XF_BC_NSLIPWALL: tt = 2 * (B + mirror_offset) - 1 - s, neg = 1; break;
+    case XF_BC_INFLOW: tt = -2; break;
+    default: return; // viscWall, slipWall, innerBlock: unchanged
+    }
+    auto at = [&](int sv) -> long long {
+        const long long sy = g.Xmax, sz = (long long)g.Xmax * g.Ymax;
+        if (d == 0) return sz * o2 + sy * o1 + sv;
+        if (d == 1) return sz * o2 + sy * sv + o1;
+        return sz * sv + sy * o2 + o1;
+    };
+    const int cx[4] = {DUCY, DUCZ, DVCY, DWCZ}, cy[4] = {DVCX, DVCZ, DUCX, DWCZ}, cz[4] = {DWCX, DWCY, DUCX, DVCY};
+    const int *comp = d == 0 ? cx : (d == 1 ? cy : cz);
+    const long long N = g.N, id = at(s);
+    if (tt == -2) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) f.Vde[comp[q] * N + id] = 0.0;
+        return;
+    }
+    const long long tid = at(tt);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const double v = f.Vde[comp[q] * N + tid];
+        f.Vde[comp[q] * N + id] = neg ? -v : v;
+    }
+}
+
+// GetWallViscousFlux{X,Y,Z} with MARCO_PREVISCFLUX / MARCO_VISCFLUX
+// (Fourth_Order/Visc_Order_kernels.hpp:73-342, Fourth_Order/Flux_discrete.h:4-93)
+// on the faces right of cells i in [B - 1, B + inner - 1] along DIR (interior
+// on the other axes).  Writes the flux the sweep subtracts: Fv[n * N + id].
+template <int NS, int DIR>
+__global__ void __launch_bounds__(256) k_visc_flux(Geom g, const double *__restrict__ Ug, Fields f, int heat,
+                                                   int diffu, double Yil_limiter, double Dim_limiter)
+{
+    constexpr int E = NS + 4;
+    constexpr int NCOP = NS - 1;
+    const int nx = g.X_inner + (DIR == 0), ny = g.Y_inner + (DIR == 1), nz = g.Z_inner + (DIR == 2);
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)nx * ny * nz) return;
+    const int i = g.Bx - (DIR == 0) + (int)(t % nx);
+    const int j = g.By - (DIR == 1) + (int)((t / nx) % ny);
+    const int k = g.Bz - (DIR == 2) + (int)(t / ((long long)nx * ny));
+    const long long N = g.N;
+    const long long sy = g.Xmax, sz = (long long)g.Xmax * g.Ymax;
+    const long long st = DIR == 0 ? 1 : (DIR == 1 ? sy : sz);
+    const long long id = sz * k + sy * j + i, m1 = id - st, p1 = id + st, p2 = id + 2 * st;
+    const double DX = g.DimX, DY = g.DimY, DZ = g.DimZ;
+    const double _dl = DIR == 0 ? g._dx : (DIR == 1 ? g._dy : g._dz);
+    const double OT = 1.0 / 3.0, SX = 1.0 / 16.0, TF = 1.0 / 24.0;
+    const double *u = f.u, *v = f.v, *w = f.w, *va = f.mu, *Vd = f.Vde;
+    auto half = [&](const double *a) { return (9.0 * (a[p1] + a[id]) - (a[p2] + a[m1])) * SX; };
+    auto grad = [&](const double *a) { return (27.0 * (a[p1] - a[id]) - (a[p2] - a[m1])) * _dl * TF; };
+    auto pair2 = [&](const double *a, const double *b) {
+        return 9.0 * (a[p1] + a[id]) - (a[p2] + a[m1]) + 9.0 * (b[p1] + b[id]) - (b[p2] + b[m1]);
+    };
+    const double mue = half(va);
+    const double lamada = -2.0 * OT * mue;
+    double f_x, f_y, f_z, u_hlf, v_hlf, w_hlf;
+    if (DIR == 0) {
+        const double *Dvcy = Vd + DVCY * N, *Dwcz = Vd + DWCZ * N, *Ducy = Vd + DUCY * N, *Ducz = Vd + DUCZ * N;
+        f_x = (2.0 * mue + lamada) * (27.0 * (u[p1] - u[id]) - (u[p2] - u[m1])) * _dl * TF;
+        f_x += lamada * pair2(Dvcy, Dwcz) * SX;
+        f_y = mue * (27.0 * (v[p1] - v[id]) - (v[p2] - v[m1])) * _dl * TF * DY;
+        f_y += mue * (9.0 * (Ducy[p1] + Ducy[id]) - (Ducy[p2] + Ducy[m1])) * SX * DY;
+        f_z = mue * (27.0 * (w[p1] - w[id]) - (w[p2] - w[m1])) * _dl * TF * DZ;
+        f_z += mue * (9.0 * (Ducz[p1] + Ducz[id]) - (Ducz[p2] + Ducz[m1])) * SX * DZ;
+        u_hlf = (9.0 * (u[p1] + u[id]) - (u[p2] + u[m1])) * SX;
+        v_hlf = (9.0 * (v[p1] + v[id]) - (v[p2] + v[m1])) * SX * DY;
+        w_hlf = (9.0 * (w[p1] + w[id]) - (w[p2] + w[m1])) * SX * DZ;
+    } else if (DIR == 1) {
+        const double *Dvcx = Vd + DVCX * N, *Dvcz = Vd + DVCZ * N, *Ducx = Vd + DUCX * N, *Dwcz = Vd + DWCZ * N;
+        f_x = mue * (27.0 * (u[p1] - u[id]) - (u[p2] - u[m1])) * _dl * TF * DX;
+        f_x += mue * (9.0 * (Dvcx[p1] + Dvcx[id]) - (Dvcx[p2] + Dvcx[m1])) * SX * DX;
+        f_y = (2.0 * mue + lamada) * (27.0 * (v[p1] - v[id]) - (v[p2] - v[m1])) * _dl * TF;
+        f_y += lamada * pair2(Ducx, Dwcz) * SX;
+        f_z = mue * (27.0 * (w[p1] - w[id]) - (w[p2] - w[m1])) * _dl * TF * DZ;
+        f_z += mue * (9.0 * (Dvcz[p1] + Dvcz[id]) - (Dvcz[p2] + Dvcz[m1])) * SX * DZ;
+        u_hlf = (9.0 * (u[p1] + u[id]) - (u[p2] + u[m1])) * SX * DX;
+        v_hlf = (9.0 * (v[p1] + v[id]) - (v[p2] + v[m1])) * SX;
+        w_hlf = (9.0 * (w[p1] + w[id]) - (w[p2] + w[m1])) * SX * DZ;
+    } else {
+        const double *Dwcx = Vd + DWCX * N, *Dwcy = Vd + DWCY * N, *Ducx = Vd + DUCX * N, *Dvcy = Vd + DVCY * N;
+        f_x = mue * (27.0 * (u[p1] - u[id]) - (u[p2] - u[m1])) * _dl * TF * DX;
+        f_x += mue * (9.0 * (Dwcx[p1] + Dwcx[id]) - (Dwcx[p2] + Dwcx[m1])) * SX * DX;
+        f_y = mue * (27.0 * (v[p1] - v[id]) - (v[p2] - v[m1])) * _dl * TF * DY;
+        f_y += mue * (9.0 * (Dwcy[p1] + Dwcy[id]) - (Dwcy[p2] + Dwcy[m1])) * SX * DY;
+        f_z = (2.0 * mue + lamada) * (27.0 * (w[p1] - w[id]) - (w[p2] - w[m1])) * _dl * TF;
+        f_z += lamada * pair2(Ducx, Dvcy) * SX;
+        u_hlf = (9.0 * (u[p1] + u[id]) - (u[p2] + u[m1])) * SX * DX;
+        v_hlf = (9.0 * (v[p1] + v[id]) - (v[p2] + v[m1])) * SX * DY;
+        w_hlf = (9.0 * (w[p1] + w[id]) - (w[p2] + w[m1])) * SX;
+    }
+    double F[E];
+    F[0] = 0.0;
+    F[1] = f_x;
+    F[2] = f_y;
+    F[3] = f_z;
+    F[4] = f_x * u_hlf + f_y * v_hlf + f_z * w_hlf;
+    if (heat) {
+        double kk = half(f.kap);
+        kk *= grad(f.T);
+        F[4] += kk;
+    }
+    if (diffu) {
+        // mass fractions of the four cells, as Getrhoyi forms them
+        const long long cid[4] = {m1, id, p1, p2};
+        double Y[4][NS];
+        double rw[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const double r = Ug[cid[c]];
+            rw[c] = r;
+            const double r1 = 1.0 / r;
+            Y[c][NCOP] = 1.0;
+#pragma unroll
+            for (int ii = 0; ii < NCOP; ii++) Y[c][ii] = Ug[(5 + ii) * N + cid[c]] * r1, Y[c][NCOP] += -Y[c][ii];
+        }
+        const double rho_wall = (9.0 * (rw[2] + rw[1]) - (rw[3] + rw[0])) * SX;
+        double CorrectTerm = 0.0, Dim_Yil = 1.0e-20, Yi_wall[NS];
+#pragma unroll
+        for (int l = 0; l < NS; l++) {
+            // limits of ConVenction_block.hpp:469-503 (single-process build)
+            double ymax = f.red[RED_YMAX + l];
+            const double ymin = f.red[RED_YMIN + l];
+            double dmax = 0.0;
+            ymax -= ymin;
+            ymax *= Yil_limiter;
+            dmax *= Dim_limiter * ymax;
+            const double *hl = f.hi + l * N, *Dl = f.Dkm + l * N;
+            const double hi_wall = half(hl);
+            const double Dim_wall = half(Dl);
+            const double Yil_wall =
+                smin(smax((27.0 * (Y[2][l] - Y[1][l]) - (Y[3][l] - Y[0][l])) * _dl * TF, -ymax), ymax);
+            Yi_wall[l] = smin(smax((9.0 * (Y[2][l] + Y[1][l]) - (Y[3][l] + Y[0][l])) * SX, 1.0E-20), 1.0);
+            Dim_Yil = smin(smax(Dim_wall * Yil_wall, -dmax), dmax);
+            CorrectTerm += Dim_Yil;
+            F[4] += rho_wall * hi_wall * Dim_Yil;
+        }
+        CorrectTerm *= rho_wall;
+#pragma unroll
+        for (int q = 5; q < E; q++) F[q] = rho_wall * Dim_Yil - Yi_wall[q - 5] * CorrectTerm;
+    } else {
+#pragma unroll
+        for (int q = 5; q < E; q++) F[q] = 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < E; q++) f.Fv[q * N + id] = F[q];
+}
+
+} // namespace xf

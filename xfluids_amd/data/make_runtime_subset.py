@@ -4,8 +4,9 @@ the benchmark workloads use, in the reference's runtime.dat file formats.
 Reads the reference tree (default /root/reference/runtime.dat); run once in
 the build container.  Copies the numeric blocks of the species it needs from
 thermal_dynamics.dat (NASA-9) and thermal_dynamics_janaf.dat (NASA-7), the
-H2/O2 19-reaction mechanisms, and writes the mixture lists of this project's
-own workloads (air/H2 shock-bubble, H2/N2 vs air mixing layer).
+H2/O2 19-reaction mechanisms and the transport tables (transport_data.dat,
+collision_integral.dat), and writes the mixture lists of this project's own
+workloads (air/H2 shock-bubble, H2/N2 vs air mixing layer).
 
     python xfluids_amd/data/make_runtime_subset.py [reference_runtime_dat]
 """
@@ -58,6 +59,11 @@ def copy_mech(src_root, name):
             fo.write(fi.read())
 
 
+def copy_root_file(src_root, fn):
+    with open(os.path.join(src_root, fn)) as fi, open(os.path.join(OUT, fn), "w") as fo:
+        fo.write(fi.read())
+
+
 def write_mixture(rel, names, out_ratio, in_ratio):
     d = os.path.join(OUT, rel)
     os.makedirs(d, exist_ok=True)
@@ -75,6 +81,9 @@ def main():
     write_mixture("H2-O2-N2", ["H2", "O2", "N2"], [0, 21, 79], [1, 0, 0])
     copy_mech(src, "H2O_19_reaction")
     copy_mech(src, "H2O-N2_19_reaction")
+    # Lennard-Jones parameters and collision integrals for the transport fits
+    copy_root_file(src, "transport_data.dat")
+    copy_root_file(src, "collision_integral.dat")
     # mixing layer: air outside (upper stream), H2:N2 = 1:1 inside (lower stream)
     write_mixture("Reaction/H2O-N2_19_reaction", ["H2", "O2", "O", "H", "OH", "HO2", "H2O2", "H2O", "N2"],
                   [0, 21, 0, 0, 0, 0, 0, 0, 79], [1, 0, 0, 0, 0, 0, 0, 0, 1])

@@ -3,6 +3,8 @@
 // Fluid / XFLUIDS classes (src/Fluids.cpp:897-1045, src/XFLUIDS.cpp:105-642).
 #include "../../include/xf_host.h"
 #include "xf_setup.h"
+#include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 
@@ -98,6 +100,7 @@ int xf_setup_override(xf_setup *h, int nx, int ny, int nz, double cfl, int schem
     }
     d.dx = s.dx, d.dy = s.dy, d.dz = s.dz;
     d._dx = 1.0 / s.dx, d._dy = 1.0 / s.dy, d._dz = 1.0 / s.dz;
+    xfh::set_visc_limiters(s);
     return XF_OK;
 }
 
@@ -106,6 +109,49 @@ int xf_setup_set_positivity(xf_setup *h, int on)
     if (!h || (on != 0 && on != 1)) return XF_EINVAL;
     h->s.desc.positivity = on;
     return XF_OK;
+}
+
+int xf_setup_set_viscous(xf_setup *h, int visc, int heat, int diffu)
+{
+    if (!h || visc < 0 || visc > 1 || heat < 0 || heat > 1 || diffu < 0 || diffu > 1) return XF_EINVAL;
+    xfh::Setup &s = h->s;
+    if (visc && !s.desc.cop) return XF_EINVAL;
+    s.desc.visc = visc;
+    s.desc.visc_heat = visc && heat;
+    s.desc.visc_diffu = visc && diffu;
+    if (visc && s.fcv.empty()) {
+        std::string m;
+        if (!xfh::transport_load(s.runtime_root, s, m)) {
+            fprintf(stderr, "xfluids_amd: %s\n", m.c_str());
+            s.desc.visc = s.desc.visc_heat = s.desc.visc_diffu = 0;
+            return XF_EINVAL;
+        }
+    }
+    return XF_OK;
+}
+
+int xf_setup_transport(const xf_setup *h, double *fcv, double *fct, double *Dkj)
+{
+    if (!h || !fcv || !fct || !Dkj) return XF_EINVAL;
+    const xfh::Setup &s = h->s;
+    const size_t NS = s.species.size();
+    std::fill(fcv, fcv + NS * 4, 0.0);
+    std::fill(fct, fct + NS * 4, 0.0);
+    std::fill(Dkj, Dkj + NS * NS * 4, 0.0);
+    if (!s.fcv.empty()) {
+        std::copy(s.fcv.begin(), s.fcv.end(), fcv);
+        std::copy(s.fct.begin(), s.fct.end(), fct);
+        std::copy(s.Dkj.begin(), s.Dkj.end(), Dkj);
+    }
+    return XF_OK;
+}
+
+double xf_setup_transport_exact(const xf_setup *h, int what, int k, int j, double T, double p)
+{
+    if (!h || h->s.chara.empty()) return NAN;
+    const int NS = (int)h->s.species.size();
+    if (what < 0 || what > 2 || k < 0 || k >= NS || j < 0 || j >= NS) return NAN;
+    return xfh::transport_exact(h->s, h->s.runtime_root, what, k, j, T, p);
 }
 
 int xf_setup_desc(const xf_setup *h, xf_case_desc *out)
@@ -183,6 +229,7 @@ int xf_setup_context(const xf_setup *h, int precision, int chem_driver, xf_ctx *
         e = xf_set_thermo(*out, h->s.Hia.data(), h->s.Hib.data(), h->s.Wi.data());
         if (!e && d.NR > 0)
             e = xf_set_mechanism(*out, h->s.Nu_f.data(), h->s.Nu_b.data(), h->s.third.data(), h->s.Rargus.data());
+        if (!e && d.visc) e = xf_set_transport(*out, h->s.fcv.data(), h->s.fct.data(), h->s.Dkj.data());
     }
     if (e) {
         xf_destroy(*out);

@@ -220,6 +220,17 @@ bool setup_load(const std::string &settings, const std::string &runtime_root, Se
     d.react = (int)eq->value("Sources_React", (d.cop && s.NR > 0) ? 1.0 : 0.0) && s.NR > 0;
     // equations.PositivityPreserving (settings/*.json): the flux limiter
     d.positivity = eq->value("PositivityPreserving", 0.0) != 0.0 ? 1 : 0;
+    // viscous terms: compile-time Visc / Visc_Heat / Visc_Diffu in the
+    // reference (CMakeLists.txt:55-57, init_options.cmake:81-91); fitting
+    // temperatures and limiters as read_json.cpp:70,104-105
+    d.visc = eq->value("Visc", 0.0) != 0.0 ? 1 : 0;
+    d.visc_heat = d.visc && eq->value("Visc_Heat", 0.0) != 0.0 ? 1 : 0;
+    d.visc_diffu = d.visc && eq->value("Visc_Diffu", 0.0) != 0.0 ? 1 : 0;
+    s.Tnode = eq->values("ViscosityFittingTnode", {273.15, 500.0, 750.0, 1000.0, 1250.0, 1500.0, 1750.0, 2000.0,
+                                                   2250.0, 2500.0, 2750.0, 3000.0, 5000.0});
+    s.Yil_limiter_json = mesh->value("Yil_limiter", 1.0E10);
+    s.Dim_limiter_json = mesh->value("Dim_limiter", 2.0E-3);
+    s.runtime_root = runtime_root;
     s.Y_out.assign(NS, 1.0);
     s.Y_in.assign(NS, 1.0);
     if (d.cop) {
@@ -255,6 +266,14 @@ bool setup_load(const std::string &settings, const std::string &runtime_root, Se
     d.dx = s.dx, d.dy = s.dy, d.dz = s.dz;
     d._dx = 1.0 / s.dx, d._dy = 1.0 / s.dy, d._dz = 1.0 / s.dz;
     for (int q = 0; q < 6; q++) d.BC[q] = bc_of(bcs, q);
+    set_visc_limiters(s);
+    if (d.visc) {
+        if (!d.cop) {
+            err = "viscous terms need a multi-component mixture";
+            return false;
+        }
+        if (!transport_load(runtime_root, s, err)) return false;
+    }
 
     // init section
     s.blast_mach = ini->value("blast_mach", 0.0);
@@ -264,6 +283,14 @@ bool setup_load(const std::string &settings, const std::string &runtime_root, Se
     s.blast_down = ini->values("blast_downstream", {0, 0, 298.15, 0, 0, 0});
     s.cop_in = ini->values("cop_inside", {s.blast_down[0], s.blast_down[1], s.blast_down[2], 0, 0, 0});
     return true;
+}
+
+void set_visc_limiters(Setup &s)
+{
+    // Setup::init (src/read_ini/src/iniset.cpp:358-359), std::min / std::max
+    xf_case_desc &d = s.desc;
+    d.Dim_limiter = std::min(std::max(s.Dim_limiter_json, 0.0), 1.0);
+    d.Yil_limiter = std::max(std::max(d._dx, d._dy), d._dz) * std::min(std::max(s.Yil_limiter_json, 0.0), 1.0);
 }
 
 // ---- host thermodynamics (initial conditions only) -------------------------

@@ -40,6 +40,11 @@ struct Setup {
     int NR = 0;
     std::vector<int> Nu_f, Nu_b;
     std::vector<double> third, Rargus;
+    // transport (viscous terms): species characteristics (Specie_chara
+    // layout, 9 per species), fitting temperatures, fits
+    std::vector<double> chara, Tnode, fcv, fct, Dkj;
+    double Yil_limiter_json = 1.0E10, Dim_limiter_json = 2.0E-3;
+    std::string runtime_root;
 };
 
 // settings: path of the case JSON; runtime_root: directory holding
@@ -47,6 +52,14 @@ struct Setup {
 // thermo_override: -1 keeps the settings file's "Thermo", 1 NASA-9, 0 JANAF
 bool setup_load(const std::string &settings, const std::string &runtime_root, Setup &s, std::string &err,
                 int thermo_override = -1);
+
+// Lennard-Jones data and transport fits (Setup::GetFitCoefficient); needs
+// species, Wi, the thermo tables and Tnode
+bool transport_load(const std::string &root, Setup &s, std::string &err);
+// kinetic-theory value behind a fit: what 0 viscosity, 1 conductivity, 2 D_kj
+double transport_exact(const Setup &s, const std::string &root, int what, int k, int j, double T, double PP);
+// BlSz.Dim_limiter and BlSz.Yil_limiter from the settings values and _dx, _dy, _dz
+void set_visc_limiters(Setup &s);
 
 // host thermodynamics for initial conditions (plain libm; not the hot path)
 double host_h_species(const Setup &s, double T, int n);          // J/kg

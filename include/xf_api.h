@@ -192,8 +192,10 @@ int xf_kernel_times(xf_ctx *ctx, const char **names, double *avg_ms, int *counts
    back on the context stream between two HIP events, in the configuration
    of RK stage 1 of the fused step: "states" (conservative->primitive on U)
    or "sweep_x"/"sweep_y"/"sweep_z" (face fluxes; the last active direction
-   carries the fused RK update U1 = U + dt LU).  Overwrites only LU, U1 and
-   the primitives, which every stage recomputes. */
+   carries the fused RK update U1 = U + dt LU); with viscous terms also
+   "visc_prep" (transport coefficients and centre gradients) and
+   "visc_x"/"visc_y"/"visc_z" (wall fluxes).  Overwrites only LU, U1, the
+   primitives and the viscous work arrays, which every stage recomputes. */
 int xf_time_kernel(xf_ctx *ctx, const char *name, int reps, double *avg_ms);
 /* Sweep tile shape per direction (0 = x, 1 = y, 2 = z): x 0 = 64 faces x 4
    pencils, 1 = 32 x 8; y, z 0 = 16 x 16, 1 = 32 x 8, 2 = 8 x 32.  Every shape

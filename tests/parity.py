@@ -20,7 +20,7 @@ CASES = {
     "reactive_channel": ("reactive-channel", "reactive-channel", dict(nx=24, ny=24), 3),
     "mixing_layer_3d": ("mixing-layer", "mixing-layer", dict(nx=16, ny=16, nz=16), 2),
     "shock_bubble_visc": ("shock-bubble", "shock-bubble", dict(nx=96, ny=48, viscous=(1, 1, 1)), 6),
-    "mixing_layer_3d_visc": ("mixing-layer", "mixing-layer", dict(nx=16, ny=16, nz=16, viscous=(1, 1, 1)), 2),
+    "mixing_layer_3d_inviscid": ("mixing-layer", "mixing-layer", dict(nx=16, ny=16, nz=16, viscous=(0, 0, 0)), 2),
 }
 
 

@@ -694,10 +694,13 @@ int time_launches(xf_ctx *c, int dir, int reps, double *avg_ms)
     if (c->g.DimX) dirs[nd++] = 0;
     if (c->g.DimY) dirs[nd++] = 1;
     if (c->g.DimZ) dirs[nd++] = 2;
+    // dir: -1 UpdateStates, 0..2 the sweep of that direction, 10 the viscous
+    // preparation, 20..22 the viscous wall flux of direction dir - 20
     int q = -1;
     for (int i = 0; i < nd; i++)
-        if (dirs[i] == dir) q = i;
-    if (dir >= 0 && q < 0) return XF_EINVAL;
+        if (dirs[i] == dir || dirs[i] == dir - 20) q = i;
+    if (((dir >= 0 && dir <= 2) || dir >= 20) && q < 0) return XF_EINVAL;
+    if (dir >= 10 && (!c->d.visc || !c->transport_set)) return XF_EINVAL;
     hipEvent_t a, b;
     XF_HIP(hipEventCreate(&a));
     XF_HIP(hipEventCreate(&b));
@@ -705,6 +708,10 @@ int time_launches(xf_ctx *c, int dir, int reps, double *avg_ms)
     for (int r = 0; r < reps; r++) {
         if (dir < 0) {
             c->ops.states(c, c->f.U, 0);
+        } else if (dir == 10) {
+            c->ops.visc_prep(c, c->f.U);
+        } else if (dir >= 20) {
+            c->ops.visc_flux(c, dir - 20, c->f.U);
         } else {
             int mode = (q > 0) ? 1 : 0;
             if (q == nd - 1) mode |= 2;
@@ -790,6 +797,10 @@ int xf_time_kernel(xf_ctx *c, const char *name, int reps, double *avg_ms)
     if (n == "sweep_x") dir = 0;
     else if (n == "sweep_y") dir = 1;
     else if (n == "sweep_z") dir = 2;
+    else if (n == "visc_prep") dir = 10;
+    else if (n == "visc_x") dir = 20;
+    else if (n == "visc_y") dir = 21;
+    else if (n == "visc_z") dir = 22;
     else if (n != "states") return XF_EINVAL;
     const bool prof = c->prof;
     c->prof = false;

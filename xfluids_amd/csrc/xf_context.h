@@ -225,22 +225,14 @@ template <int NS, bool COP> struct Launch {
             hipLaunchKernelGGL((k_transport<NS>), dim3(nblk(c->N)), dim3(256), 0, c->stream, g, c->d_th, c->d_tr, Ug,
                                c->f, c->d.visc_heat, c->d.visc_diffu);
             if (c->d.visc_diffu)
-                hipLaunchKernelGGL((k_yreduce<NS>), dim3(1), dim3(256), 0, c->stream, c->f.ypart, (int)nblk(c->N),
-                                   c->f.red);
+                hipLaunchKernelGGL((k_yreduce<NS>), dim3(2 * NS), dim3(1024), 0, c->stream, c->f.ypart,
+                                   (int)nblk(c->N), c->f.red);
             const int i0 = g.DimX ? g.Bx - 2 : 0, nx = g.DimX ? g.X_inner + 4 : 1;
             const int j0 = g.DimY ? g.By - 2 : 0, ny = g.DimY ? g.Y_inner + 4 : 1;
             const int k0 = g.DimZ ? g.Bz - 2 : 0, nz = g.DimZ ? g.Z_inner + 4 : 1;
+            const int *bc = c->d.BC;
             hipLaunchKernelGGL(k_vel_deriv, dim3(nblk((long long)nx * ny * nz)), dim3(256), 0, c->stream, g, c->f, i0,
-                               nx, j0, ny, k0, nz);
-            const int dims[3] = {g.DimX, g.DimY, g.DimZ};
-            for (int d = 0; d < 3; d++) {
-                if (!dims[d]) continue;
-                const long long B = d == 0 ? g.Bx : (d == 1 ? g.By : g.Bz);
-                const long long lines = d == 0 ? (long long)g.Ymax * g.Zmax
-                                               : (d == 1 ? (long long)g.Xmax * g.Zmax : (long long)g.Xmax * g.Ymax);
-                hipLaunchKernelGGL(k_vde_bc, dim3(nblk(2 * B * lines)), dim3(256), 0, c->stream, g, c->f, d,
-                                   c->d.BC[2 * d], c->d.BC[2 * d + 1]);
-            }
+                               nx, j0, ny, k0, nz, bc[0], bc[1], bc[2], bc[3], bc[4], bc[5]);
         }
     }
     template <int DIR> static void visc_flux_d(xf_ctx *c, const double *Ug)

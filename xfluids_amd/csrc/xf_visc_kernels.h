@@ -7,8 +7,8 @@
 //   k_transport<NS>   every padded cell: u, v, w, Wilke mu and lambda, D_km;
 //                     per-block min/max of each interior mass fraction
 //   k_yreduce<NS>     one block: those partials into the reduction slots
-//   k_vel_deriv       fourth-order centre derivatives of u, v, w
-//   k_vde_bc          the ghost-cell derivatives each wall flux reads
+//   k_vel_deriv       fourth-order centre derivatives of u, v, w, with the
+//                     ghost-cell rules for the values each wall flux reads
 //   k_visc_flux<NS,DIR> per face of one direction; the sweep of that
 //                     direction subtracts the result from its face flux
 // All arithmetic is FP64 in the reference's operation order, with the same
@@ -32,6 +32,19 @@ enum { DUCX = 0, DVCX = 1, DWCX = 2, DUCY = 3, DVCY = 4, DWCY = 5, DUCZ = 6, DVC
 // sycl::min / sycl::max on doubles
 __device__ __forceinline__ double smin(double x, double y) { return y < x ? y : x; }
 __device__ __forceinline__ double smax(double x, double y) { return x < y ? y : x; }
+
+// RN(a / b) from y = RN(1 / b) (Markstein's correction, exact for a and b
+// in [2^-500, 2^500]); other operands take the IEEE division.  Lets quotients
+// that share a divisor share one reciprocal.
+__device__ __noinline__ double ieee_div(double a, double b) { return a / b; }
+__device__ __forceinline__ double qdiv(double a, double b, double y)
+{
+    double q = fma(fma(-b, a * y, a), y, a * y);
+    const double aa = fabs(a), ab = fabs(b);
+    if (__builtin_expect(!(aa >= 0x1p-500 && aa <= 0x1p500 && ab >= 0x1p-500 && ab <= 0x1p500), 0))
+        q = ieee_div(a, b);
+    return q;
+}
 
 // cubic in ln T (Viscosity / Thermal_conductivity / GetDkj, Visc_device.h:9-90)
 __device__ __forceinline__ double fit4(const double *a, double lT)
@@ -86,9 +99,9 @@ __global__ void __launch_bounds__(256) k_transport(Geom g, const DevThermo *__re
         for (int i = 0; i < NS; i++) X[i] = X[i] * _C_total;
         // Wilke mixture viscosity and conductivity (the conductivity uses the
         // viscosity's denominators, as Visc_device.h:130-139 does)
-        double mu[NS];
+        double mu[NS], rmu[NS];
 #pragma unroll
-        for (int k = 0; k < NS; k++) mu[k] = dexp(fit4(tr->fcv + 4 * k, lT));
+        for (int k = 0; k < NS; k++) mu[k] = dexp(fit4(tr->fcv + 4 * k, lT)), rmu[k] = 1.0 / mu[k];
         double va = 0.0, tca = 0.0;
         const double s2 = sqrt(2.0);
 #pragma unroll
@@ -96,7 +109,7 @@ __global__ void __launch_bounds__(256) k_transport(Geom g, const DevThermo *__re
             double den = 0.0;
 #pragma unroll
             for (int i = 0; i < NS; i++) {
-                double phi = tr->phiA[k * MAXNS + i] * sqrt(mu[k] / mu[i]);
+                double phi = tr->phiA[k * MAXNS + i] * sqrt(qdiv(mu[k], mu[i], rmu[i]));
                 phi = (phi + 1.0) * (phi + 1.0) * 0.5 / s2;
                 phi = phi * tr->phiB[k * MAXNS + i];
                 den = den + X[i] * phi;
@@ -113,15 +126,17 @@ __global__ void __launch_bounds__(256) k_transport(Geom g, const DevThermo *__re
             // The fits are symmetric (Setup copies the lower triangle from
             // the upper), so each pair's exponential serves both species.
             double t2[NS];
+            const double rp = 1.0 / p;
 #pragma unroll
             for (int k = 0; k < NS; k++) t2[k] = 1.0e-20;
 #pragma unroll
             for (int k = 0; k < NS; k++) {
 #pragma unroll
                 for (int i = 0; i < k; i++) {
-                    const double D = dexp(fit4(tr->Dkj + 4 * (i * MAXNS + k), lT)) / p;
-                    t2[k] += (X[i] + 1.0e-40) / (D + 1.0e-40);
-                    t2[i] += (X[k] + 1.0e-40) / (D + 1.0e-40);
+                    const double D = qdiv(dexp(fit4(tr->Dkj + 4 * (i * MAXNS + k), lT)), p, rp);
+                    const double De = D + 1.0e-40, rDe = 1.0 / De;
+                    t2[k] += qdiv(X[i] + 1.0e-40, De, rDe);
+                    t2[i] += qdiv(X[k] + 1.0e-40, De, rDe);
                 }
             }
 #pragma unroll
@@ -134,7 +149,7 @@ __global__ void __launch_bounds__(256) k_transport(Geom g, const DevThermo *__re
                 if (t1 <= 0.0) // sycl::step(ceil(temp1), 0)
                     D = dexp(fit4(tr->Dkj + 4 * (k * MAXNS + k), lT)) / p;
                 else
-                    D = t1 / t2[k] / rho * C_total;
+                    D = qdiv(t1 / t2[k], rho, rho1) * C_total;
                 D *= 1.0e-1;
                 f.Dkm[k * N + id] = smax(D, 1.0e-10);
             }
@@ -175,118 +190,117 @@ __global__ void __launch_bounds__(256) k_transport(Geom g, const DevThermo *__re
 
 // the per-block partials of k_transport into red[RED_YMIN + k] (min) and
 // red[RED_YMAX + k] (max), each combined with 0.0 as the reference's
-// sycl::reduction combines with the stored yi_min = yi_max = 0
+// sycl::reduction combines with the stored yi_min = yi_max = 0; one block per
+// quantity (2 NS blocks)
 template <int NS>
-__global__ void __launch_bounds__(256) k_yreduce(const double *__restrict__ ypart, int nblocks, double *red)
+__global__ void __launch_bounds__(1024) k_yreduce(const double *__restrict__ ypart, int nblocks, double *red)
 {
-    __shared__ double s[256];
-    for (int q = 0; q < 2 * NS; q++) {
-        const bool mx = q >= NS;
-        double v = 0.0;
-        for (int b = threadIdx.x; b < nblocks; b += 256) {
-            const double x = ypart[(long long)b * 2 * NS + q];
-            v = mx ? smax(v, x) : smin(v, x);
-        }
-        s[threadIdx.x] = v;
-        __syncthreads();
-        for (int w = 128; w >= 1; w >>= 1) {
-            if (threadIdx.x < w) s[threadIdx.x] = mx ? smax(s[threadIdx.x], s[threadIdx.x + w]) : smin(s[threadIdx.x], s[threadIdx.x + w]);
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) red[(mx ? RED_YMAX + q - NS : RED_YMIN + q)] = s[0];
+    __shared__ double s[1024];
+    const int q = blockIdx.x;
+    const bool mx = q >= NS;
+    double v = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += 1024) {
+        const double x = ypart[(long long)b * 2 * NS + q];
+        v = mx ? smax(v, x) : smin(v, x);
+    }
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = 512; w >= 1; w >>= 1) {
+        if (threadIdx.x < w) s[threadIdx.x] = mx ? smax(s[threadIdx.x], s[threadIdx.x + w]) : smin(s[threadIdx.x], s[threadIdx.x + w]);
         __syncthreads();
     }
+    if (threadIdx.x == 0) red[mx ? RED_YMAX + q - NS : RED_YMIN + q] = s[0];
 }
 
-// GetInnerCellCenterDerivativeKernel (Fourth_Order/Visc_Order_kernels.hpp:16-71)
-// over i in [Bwidth_X - 2, Xmax - Bwidth_X + 1] per active direction (index 0
-// of an inactive one); a derivative along an inactive direction is +0
+// fourth-order centre derivatives of u, v, w along direction a at cell id
+// (GetInnerCellCenterDerivativeKernel, Fourth_Order/Visc_Order_kernels.hpp:16-71):
+// D[0..2] = du/da, dv/da, dw/da with the reference's DimX_t / DimY_t / DimZ_t
+// factors in its order
+__device__ __forceinline__ void centre_deriv(const Geom &g, const Fields &f, long long id, int a, double *D)
+{
+    const long long st = a == 0 ? 1 : (a == 1 ? (long long)g.Xmax : (long long)g.Xmax * g.Ymax);
+    const long long m1 = id - st, m2 = id - 2 * st, p1 = id + st, p2 = id + 2 * st;
+    const double DX = g.DimX, DY = g.DimY, DZ = g.DimZ, tw = 1.0 / 12.0;
+    const double _d = a == 0 ? g._dx : (a == 1 ? g._dy : g._dz);
+    const double Da = a == 0 ? DX : (a == 1 ? DY : DZ);
+    const double *u = f.u, *v = f.v, *w = f.w;
+    const double du = (8.0 * (u[p1] - u[m1]) - (u[p2] - u[m2])) * _d * tw;
+    const double dv = (8.0 * (v[p1] - v[m1]) - (v[p2] - v[m2])) * _d * tw;
+    const double dw = (8.0 * (w[p1] - w[m1]) - (w[p2] - w[m2])) * _d * tw;
+    if (a == 0) D[0] = du * DX, D[1] = dv * DX * DY, D[2] = dw * DX * DZ;
+    else if (a == 1) D[0] = du * DY * DX, D[1] = dv * DY, D[2] = dw * DY * DZ;
+    else D[0] = du * DZ * DX, D[1] = dv * DZ * DY, D[2] = dw * DZ;
+    (void)Da;
+}
+
+// GetCellCenterDerivative (Visc_block.hpp:8-84) in one pass: the centre
+// derivatives on i in [Bwidth_X - 2, Xmax - Bwidth_X + 1] per active
+// direction (index 0 of an inactive one; a derivative along an inactive
+// direction is +0), and on the ghost cells of that range the
+// CenterDerivativeBCKernel{X,Y,Z} rules (Visc_kernels.hpp:7-203) for the four
+// components each direction's wall flux reads across the boundary: the value
+// at the mirror / periodic / outflow cell (negated for a no-slip wall, 0 for
+// inflow, unchanged for the other codes) is recomputed there with the same
+// operations.  Ghost cells in two directions at once (edges and corners) are
+// read by no wall flux.  The z no-slip wall uses the mirror offset - k of the
+// x and y kernels (the reference's k - offset indexes outside the array).
 static __global__ void __launch_bounds__(256) k_vel_deriv(Geom g, Fields f, int i0, int nx, int j0, int ny, int k0,
-                                                          int nz)
+                                                          int nz, int bc0, int bc1, int bc2, int bc3, int bc4, int bc5)
 {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long long)nx * ny * nz) return;
-    const int i = i0 + (int)(t % nx), j = j0 + (int)((t / nx) % ny), k = k0 + (int)(t / ((long long)nx * ny));
+    const int c[3] = {i0 + (int)(t % nx), j0 + (int)((t / nx) % ny), k0 + (int)(t / ((long long)nx * ny))};
     const long long N = g.N;
     const long long sy = g.Xmax, sz = (long long)g.Xmax * g.Ymax;
-    const long long id = sz * k + sy * j + i;
-    const double DX = g.DimX, DY = g.DimY, DZ = g.DimZ;
-    const double *u = f.u, *v = f.v, *w = f.w;
-    const double tw = 1.0 / 12.0;
+    const long long id = sz * c[2] + sy * c[1] + c[0];
+    const int dims[3] = {g.DimX, g.DimY, g.DimZ};
+    const int Bw[3] = {g.Bx, g.By, g.Bz}, inner[3] = {g.X_inner, g.Y_inner, g.Z_inner};
+    const int Lm[3] = {g.Xmax, g.Ymax, g.Zmax};
+    const int bcs[6] = {bc0, bc1, bc2, bc3, bc4, bc5};
     double D[9];
 #pragma unroll
     for (int q = 0; q < 9; q++) D[q] = 0.0;
-    if (g.DimX) {
-        const long long m1 = id - 1, m2 = id - 2, p1 = id + 1, p2 = id + 2;
-        D[DUCX] = (8.0 * (u[p1] - u[m1]) - (u[p2] - u[m2])) * g._dx * tw * DX;
-        D[DVCX] = (8.0 * (v[p1] - v[m1]) - (v[p2] - v[m2])) * g._dx * tw * DX * DY;
-        D[DWCX] = (8.0 * (w[p1] - w[m1]) - (w[p2] - w[m2])) * g._dx * tw * DX * DZ;
-    }
-    if (g.DimY) {
-        const long long m1 = id - sy, m2 = id - 2 * sy, p1 = id + sy, p2 = id + 2 * sy;
-        D[DUCY] = (8.0 * (u[p1] - u[m1]) - (u[p2] - u[m2])) * g._dy * tw * DY * DX;
-        D[DVCY] = (8.0 * (v[p1] - v[m1]) - (v[p2] - v[m2])) * g._dy * tw * DY;
-        D[DWCY] = (8.0 * (w[p1] - w[m1]) - (w[p2] - w[m2])) * g._dy * tw * DY * DZ;
-    }
-    if (g.DimZ) {
-        const long long m1 = id - sz, m2 = id - 2 * sz, p1 = id + sz, p2 = id + 2 * sz;
-        D[DUCZ] = (8.0 * (u[p1] - u[m1]) - (u[p2] - u[m2])) * g._dz * tw * DZ * DX;
-        D[DVCZ] = (8.0 * (v[p1] - v[m1]) - (v[p2] - v[m2])) * g._dz * tw * DZ * DY;
-        D[DWCZ] = (8.0 * (w[p1] - w[m1]) - (w[p2] - w[m2])) * g._dz * tw * DZ;
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+        if (dims[a]) centre_deriv(g, f, id, a, D + 3 * a);
+    // components read across a boundary of direction d (Vde order
+    // ducx dvcx dwcx ducy dvcy dwcy ducz dvcz dwcz)
+    const int comp[3][4] = {{DUCY, DUCZ, DVCY, DWCZ}, {DVCX, DVCZ, DUCX, DWCZ}, {DWCX, DWCY, DUCX, DVCY}};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        if (!dims[d]) continue;
+        const int s = c[d], B = Bw[d];
+        const bool lo = s < B, hi = s >= B + inner[d];
+        if (!lo && !hi) continue;
+        const int bc = bcs[2 * d + (hi ? 1 : 0)];
+        const int mirror_offset = lo ? 0 : inner[d], index_inner = lo ? B : Lm[d] - B - 1, sign = lo ? 1 : -1;
+        int tt;
+        double fac = 1.0;
+        if (bc == XF_BC_SYMMETRY) tt = 2 * (B + mirror_offset) - 1 - s;
+        else if (bc == XF_BC_PERIODIC) tt = s + sign * inner[d];
+        else if (bc == XF_BC_OUTFLOW) tt = index_inner;
+        else if (bc == XF_BC_NSLIPWALL) tt = 2 * (B + mirror_offset) - 1 - s, fac = -1.0;
+        else if (bc == XF_BC_INFLOW) tt = -1;
+        else continue; // viscWall, slipWall, innerBlock: the centre value stays
+        if (tt < 0) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) D[comp[d][q]] = 0.0;
+            continue;
+        }
+        int ct[3] = {c[0], c[1], c[2]};
+        ct[d] = tt;
+        const long long tid = sz * ct[2] + sy * ct[1] + ct[0];
+        double T9[9];
+#pragma unroll
+        for (int q = 0; q < 9; q++) T9[q] = 0.0;
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+            if (dims[a] && a != d) centre_deriv(g, f, tid, a, T9 + 3 * a);
+#pragma unroll
+        for (int q = 0; q < 4; q++) D[comp[d][q]] = fac < 0.0 ? -T9[comp[d][q]] : T9[comp[d][q]];
     }
 #pragma unroll
     for (int q = 0; q < 9; q++) f.Vde[q * N + id] = D[q];
-}
-
-// CenterDerivativeBCKernel{X,Y,Z} (Visc_kernels.hpp:7-203) for direction d:
-// the 2 * B ghost layers of every line of the other two axes, four
-// components each.  The z no-slip wall uses the mirror offset - k of the x
-// and y kernels (the reference's k - offset indexes outside the array).
-static __global__ void __launch_bounds__(256) k_vde_bc(Geom g, Fields f, int d, int bc_lo, int bc_hi)
-{
-    const int B = d == 0 ? g.Bx : (d == 1 ? g.By : g.Bz);
-    const int inner = d == 0 ? g.X_inner : (d == 1 ? g.Y_inner : g.Z_inner);
-    const int L = d == 0 ? g.Xmax : (d == 1 ? g.Ymax : g.Zmax);
-    const int a1 = d == 0 ? g.Ymax : g.Xmax;             // first other axis
-    const int a2 = d == 2 ? g.Ymax : g.Zmax;             // second other axis
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 2LL * B * a1 * a2) return;
-    const int gq = (int)(t % (2 * B));
-    const long long line = t / (2 * B);
-    const int o1 = (int)(line % a1), o2 = (int)(line / a1);
-    const int lo = gq < B;
-    const int s = lo ? gq : gq - B + L - B;
-    const int bc = lo ? bc_lo : bc_hi;
-    const int mirror_offset = lo ? 0 : inner, index_inner = lo ? B : L - B - 1, sign = lo ? 1 : -1;
-    int tt = -1, neg = 0;
-    switch (bc) {
-    case XF_BC_SYMMETRY: tt = 2 * (B + mirror_offset) - 1 - s; break;
-    case XF_BC_PERIODIC: tt = s + sign * inner; break;
-    case XF_BC_OUTFLOW: tt = index_inner; break;
-    case XF_BC_NSLIPWALL: tt = 2 * (B + mirror_offset) - 1 - s, neg = 1; break;
-    case XF_BC_INFLOW: tt = -2; break;
-    default: return; // viscWall, slipWall, innerBlock: unchanged
-    }
-    auto at = [&](int sv) -> long long {
-        const long long sy = g.Xmax, sz = (long long)g.Xmax * g.Ymax;
-        if (d == 0) return sz * o2 + sy * o1 + sv;
-        if (d == 1) return sz * o2 + sy * sv + o1;
-        return sz * sv + sy * o2 + o1;
-    };
-    const int cx[4] = {DUCY, DUCZ, DVCY, DWCZ}, cy[4] = {DVCX, DVCZ, DUCX, DWCZ}, cz[4] = {DWCX, DWCY, DUCX, DVCY};
-    const int *comp = d == 0 ? cx : (d == 1 ? cy : cz);
-    const long long N = g.N, id = at(s);
-    if (tt == -2) {
-#pragma unroll
-        for (int q = 0; q < 4; q++) f.Vde[comp[q] * N + id] = 0.0;
-        return;
-    }
-    const long long tid = at(tt);
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-        const double v = f.Vde[comp[q] * N + tid];
-        f.Vde[comp[q] * N + id] = neg ? -v : v;
-    }
 }
 
 // GetWallViscousFlux{X,Y,Z} with MARCO_PREVISCFLUX / MARCO_VISCFLUX

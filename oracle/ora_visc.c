@@ -341,7 +341,7 @@ static void wall_flux(ora_fluid *f, int d, size_t id, const double *yil_lim, con
 #undef GRAD
 }
 
-/* the viscous part of GetLU (ConVenction_block.hpp:424-575) for the state
+/* the viscous part of GetLU (ConVenction_block.hpp:424-575; limits :462-503) for the state
    whose primitives the last UpdateStates left in f */
 void ora_viscous_fluxes(ora_fluid *f)
 {

@@ -59,7 +59,7 @@ __device__ __forceinline__ double fit4(const double *a, double lT)
 // Visc_device.h:99-177) with get_xi (Mixing_device.h:31-44) on every padded
 // cell; also GetStates' u, v, w (Update_device.hpp:36) and, with diffusion,
 // each block's min / max of the interior mass fractions
-// (ConVenction_block.hpp:469-490) into ypart[block][2 NS]
+// (ConVenction_block.hpp:462-490) into ypart[block][2 NS]
 template <int NS>
 __global__ void __launch_bounds__(256) k_transport(Geom g, const DevThermo *__restrict__ th,
                                                    const DevTransport *__restrict__ tr, const double *__restrict__ Ug,
@@ -98,7 +98,7 @@ __global__ void __launch_bounds__(256) k_transport(Geom g, const DevThermo *__re
 #pragma unroll
         for (int i = 0; i < NS; i++) X[i] = X[i] * _C_total;
         // Wilke mixture viscosity and conductivity (the conductivity uses the
-        // viscosity's denominators, as Visc_device.h:130-139 does)
+        // viscosity's denominators, as Visc_device.h:121-134 does)
         double mu[NS], rmu[NS];
 #pragma unroll
         for (int k = 0; k < NS; k++) mu[k] = dexp(fit4(tr->fcv + 4 * k, lT)), rmu[k] = 1.0 / mu[k];
@@ -398,7 +398,7 @@ __global__ void __launch_bounds__(256) k_visc_flux(Geom g, const double *__restr
         double CorrectTerm = 0.0, Dim_Yil = 1.0e-20, Yi_wall[NS];
 #pragma unroll
         for (int l = 0; l < NS; l++) {
-            // limits of ConVenction_block.hpp:469-503 (single-process build)
+            // limits of ConVenction_block.hpp:462-503 (single-process build)
             double ymax = f.red[RED_YMAX + l];
             const double ymin = f.red[RED_YMIN + l];
             double dmax = 0.0;

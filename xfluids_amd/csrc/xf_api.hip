@@ -244,7 +244,8 @@ int xf_create(const xf_case_desc *desc, xf_ctx **out)
     if (hipMalloc(&f.hi, (size_t)d.NS * n1) || hipMalloc(&f.red, RED_SIZE * sizeof(double)) ||
         hipMalloc(&f.dt, 4 * sizeof(double)) || hipMalloc(&f.err, 2 * sizeof(int)) ||
         hipMalloc(&f.substeps, (size_t)c->Ninner * sizeof(int)) || hipMalloc(&f.queue, 4 * sizeof(unsigned)) ||
-        hipMalloc(&c->d_th, sizeof(DevThermo)) || hipMalloc(&c->d_mc, sizeof(DevMech)))
+        hipMalloc(&c->d_th, sizeof(DevThermo)) || hipMalloc(&c->d_mc, sizeof(DevMech)) ||
+        hipMalloc(&f.spart, (size_t)nblk(c->N) * 12 * sizeof(double)))
         return fail(XF_EHIP);
     // zero everything once (eigen_block starts at 0, src/Fluids.cpp:427);
     // plain memsets at creation, never inside a captured step
@@ -290,7 +291,7 @@ void xf_destroy(xf_ctx *c)
     void *ptrs[] = {f.U,   f.U1,    f.LU,       c->U2,   f.rho,   f.u,     f.v,   f.w,   f.p,   f.H,
                     f.T,   f.c,     f.gamma,    f.e,     f.Cp,    f.gcv,   f.R,   f.hi,  f.red, f.dt,
                     f.err, f.substeps, f.queue, c->d_th, c->d_mc, c->d_tr, f.mu,  f.kap, f.Dkm, f.Vde,
-                    f.Fv,  f.ypart};
+                    f.Fv,  f.ypart, f.spart};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);

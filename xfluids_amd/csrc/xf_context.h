@@ -114,6 +114,7 @@ template <int NS, bool COP> struct Launch {
         ProfScope ps(c, "states");
         hipLaunchKernelGGL((k_states<NS, COP>), dim3(nblk(c->N)), dim3(256), 0, c->stream, c->g, c->d_th, U, c->f,
                            c->d.ncop_gamma, full);
+        hipLaunchKernelGGL(k_states_fold, dim3(12), dim3(1024), 0, c->stream, c->f.spart, (int)nblk(c->N), c->f.red);
     }
     template <int DIR, int SCH, int PREC, int SHAPE>
     static void sweep_t(xf_ctx *c, const double *Ug, int mode, int stage, const double *Ua, const double *Ub, double *Uo)

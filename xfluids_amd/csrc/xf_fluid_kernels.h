@@ -28,13 +28,6 @@ __device__ __forceinline__ double wave_max(double v)
     return v;
 }
 
-// atomic max of a non-negative double through its bit pattern (ordered as
-// unsigned 64-bit for +0 .. +inf)
-__device__ __forceinline__ void atomic_max_pos(double *addr, double v)
-{
-    atomicMax(reinterpret_cast<unsigned long long *>(addr), (unsigned long long)__double_as_longlong(v));
-}
-
 // ---------------------------------------------------------------------------
 // ghost fill along one axis: one thread per (ghost layer g, transverse a, b)
 // ---------------------------------------------------------------------------
@@ -287,13 +280,41 @@ __global__ void __launch_bounds__(256) k_states(Geom g, const DevThermo *__restr
         atomicMin(&f.err[1], (int)id);
     }
     __syncthreads();
+    // the block's maxima go to spart; k_states_fold combines the blocks (one
+    // pass over the partials instead of same-address atomics from every block)
     if (threadIdx.x < 12) {
         double m = sred[0][threadIdx.x];
         const int nw = (blockDim.x + 63) >> 6;
         for (int q = 1; q < nw; q++) m = sred[q][threadIdx.x] > m ? sred[q][threadIdx.x] : m;
-        if (m > 0.0) atomic_max_pos(&f.red[threadIdx.x < 9 ? RED_GLFP + threadIdx.x : RED_DT + threadIdx.x - 9], m);
+        f.spart[(long long)blockIdx.x * 12 + threadIdx.x] = m;
     }
     if (threadIdx.x == 0 && serr) atomicOr(&f.err[0], serr);
+}
+
+// the per-block GLF and CFL maxima of k_states into the reduction slots: one
+// block per slot (12), each the maximum of the stored slot value and every
+// positive block value, as the per-block atomic maxima would leave it
+static __global__ void __launch_bounds__(1024) k_states_fold(const double *__restrict__ spart, int nblocks, double *red)
+{
+    __shared__ unsigned long long s[1024];
+    const int q = blockIdx.x;
+    unsigned long long v = 0ULL; // +0.0
+    for (int b = threadIdx.x; b < nblocks; b += 1024) {
+        const double x = spart[(long long)b * 12 + q];
+        const unsigned long long bx = (unsigned long long)__double_as_longlong(x);
+        if (x > 0.0 && bx > v) v = bx;
+    }
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = 512; w >= 1; w >>= 1) {
+        if (threadIdx.x < w && s[threadIdx.x + w] > s[threadIdx.x]) s[threadIdx.x] = s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double *slot = &red[q < 9 ? RED_GLFP + q : RED_DT + q - 9];
+        const unsigned long long cur = (unsigned long long)__double_as_longlong(*slot);
+        if (s[0] > cur) *slot = __longlong_as_double((long long)s[0]);
+    }
 }
 
 // ---------------------------------------------------------------------------

@@ -73,6 +73,7 @@ struct Fields {
     // viscous face flux of the direction being swept (E planes)
     double *mu, *kap, *Dkm, *Vde, *Fv;
     double *ypart; // per-block interior min / max of each mass fraction (2 NS per block)
+    double *spart; // per-block GLF and CFL maxima of k_states (12 per block)
 };
 
 // reduction slot layout (doubles, compared as uint64 bit patterns)

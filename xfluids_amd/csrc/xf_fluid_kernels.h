@@ -183,12 +183,24 @@ __global__ void __launch_bounds__(256) k_states(Geom g, const DevThermo *__restr
             T = get_T<NS>(th, yi, tme, T);
             p = rho * R_ * T;
             const TPre t = th->thermo ? tpre<true>(T, 1) : tpre<false>(T, 0);
-            double Cp = 0.0, cpk[NS];
+            // species Cp and h: with the fit range as a wave-uniform value when
+            // every lane shares it (scalar coefficient loads)
+            double cpk[NS], hk[NS];
+            auto species = [&](const TPre &tt) {
 #pragma unroll
-            for (int n = 0; n < NS; n++) {
-                cpk[n] = cp_k(th, t, th->Ri[n], n);
-                Cp += yi[n] * cpk[n];
+                for (int n = 0; n < NS; n++) cpk[n] = cp_k(th, tt, th->Ri[n], n), hk[n] = h_k(th, tt, th->Ri[n], n);
+            };
+            const int r0 = __builtin_amdgcn_readfirstlane(t.r);
+            if (__all(t.r == r0)) {
+                TPre tu = t;
+                tu.r = r0;
+                species(tu);
+            } else {
+                species(t);
             }
+            double Cp = 0.0;
+#pragma unroll
+            for (int n = 0; n < NS; n++) Cp += yi[n] * cpk[n];
             gamma = mix_gamma_cp<NS>(th, yi, Cp);
             // ingredients of the face sound speed (ReconstructSoundSpeed,
             // positive-definite_eigen/Utils_device.hpp:102-140): h_k(T), Cp,
@@ -201,7 +213,7 @@ __global__ void __launch_bounds__(256) k_states(Geom g, const DevThermo *__restr
             for (int n = 0; n < NS; n++) _W += yi[n] * th->_Wi[n];
             cv -= Ru * _W;
 #pragma unroll
-            for (int n = 0; n < NS; n++) f.hi[n * N + id] = h_k(th, t, th->Ri[n], n);
+            for (int n = 0; n < NS; n++) f.hi[n * N + id] = hk[n];
             f.Cp[id] = Cp;
             f.gcv[id] = Cp / cv;
             f.R[id] = R_;

@@ -166,13 +166,25 @@ __device__ __forceinline__ double get_T(const DevThermo *__restrict__ th, const 
         const TPre t = thermo ? tpre<true>(T, 1) : tpre<false>(T, 0);
         double h = 0.0;
         double Cp = 0.0;
+        // the same sums with the fit range as a wave-uniform value when every
+        // active lane uses one range (scalar coefficient loads)
+        auto sums = [&](const TPre &tt) {
 #pragma unroll
-        for (int n = 0; n < NS; n++) {
-            const double Ri = th->Ri[n];
-            h += h_k(th, t, Ri, n) * yi[n];
+            for (int n = 0; n < NS; n++) {
+                const double Ri = th->Ri[n];
+                h += h_k(th, tt, Ri, n) * yi[n];
+            }
+#pragma unroll
+            for (int n = 0; n < NS; n++) Cp += yi[n] * cp_k(th, tt, th->Ri[n], n);
+        };
+        const int r0 = __builtin_amdgcn_readfirstlane(t.r);
+        if (__all(t.r == r0)) {
+            TPre tu = t;
+            tu.r = r0;
+            sums(tu);
+        } else {
+            sums(t);
         }
-#pragma unroll
-        for (int n = 0; n < NS; n++) Cp += yi[n] * cp_k(th, t, th->Ri[n], n);
         const double func_T = h - R * T - e;
         const double dfunc_T = Cp - R;
         double df = dmin(func_T / (dfunc_T + 1.0e-30), 1e-3 * T);

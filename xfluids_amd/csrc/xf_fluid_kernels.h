@@ -15,6 +15,19 @@
 
 namespace xf {
 
+// (i, j, k) of padded cell id: 32-bit divisions when the grid has fewer than
+// 2^31 cells (a uniform branch), 64-bit ones otherwise
+__device__ __forceinline__ void cell_ijk(const Geom &g, long long id, int &i, int &j, int &k)
+{
+    if (g.N < 2147483648LL) {
+        const unsigned u = (unsigned)id, X = (unsigned)g.Xmax, Y = (unsigned)g.Ymax;
+        const unsigned r = u / X;
+        i = (int)(u - r * X), j = (int)(r % Y), k = (int)(r / Y);
+    } else {
+        i = (int)(id % g.Xmax), j = (int)((id / g.Xmax) % g.Ymax), k = (int)(id / ((long long)g.Xmax * g.Ymax));
+    }
+}
+
 // ---------------------------------------------------------------------------
 // wave64 max reduction (butterfly over DPP/permute) of non-negative doubles
 // ---------------------------------------------------------------------------
@@ -109,24 +122,34 @@ static __global__ void __launch_bounds__(256) k_boundary_all(Geom g, double *__r
     const long long ny = g.DimY ? 2LL * g.By * g.Xmax * g.Z_inner : 0;
     const long long nx = g.DimX ? 2LL * g.Bx * g.Y_inner * g.Z_inner : 0;
     int i, j, k;
+    // quotient and remainder in 32 bits when the shell is below 2^31 cells
+    auto dm = [&](long long a, long long b, long long &q, long long &r) {
+        if (nz + ny + nx < 2147483648LL) {
+            const unsigned qa = (unsigned)a / (unsigned)b;
+            q = qa, r = (long long)((unsigned)a - qa * (unsigned)b);
+        } else {
+            q = a / b, r = a % b;
+        }
+    };
+    long long q, r, q2, r2;
     if (t < nz) {
-        const long long per = (long long)g.Ymax * g.Xmax;
-        const int gz = (int)(t / per);
-        const long long r = t % per;
+        dm(t, (long long)g.Ymax * g.Xmax, q, r);
+        const int gz = (int)q;
         k = gz < g.Bz ? gz : gz - g.Bz + g.Zmax - g.Bz;
-        j = (int)(r / g.Xmax), i = (int)(r % g.Xmax);
+        dm(r, g.Xmax, q2, r2);
+        j = (int)q2, i = (int)r2;
     } else if ((t -= nz) < ny) {
-        const long long per = (long long)g.Xmax * g.Z_inner;
-        const int gy = (int)(t / per);
-        const long long r = t % per;
+        dm(t, (long long)g.Xmax * g.Z_inner, q, r);
+        const int gy = (int)q;
         j = gy < g.By ? gy : gy - g.By + g.Ymax - g.By;
-        k = g.Bz + (int)(r / g.Xmax), i = (int)(r % g.Xmax);
+        dm(r, g.Xmax, q2, r2);
+        k = g.Bz + (int)q2, i = (int)r2;
     } else if ((t -= ny) < nx) {
-        const long long per = (long long)g.Y_inner * g.Z_inner;
-        const int gx = (int)(t / per);
-        const long long r = t % per;
+        dm(t, (long long)g.Y_inner * g.Z_inner, q, r);
+        const int gx = (int)q;
         i = gx < g.Bx ? gx : gx - g.Bx + g.Xmax - g.Bx;
-        k = g.Bz + (int)(r / g.Y_inner), j = g.By + (int)(r % g.Y_inner);
+        dm(r, g.Y_inner, q2, r2);
+        k = g.Bz + (int)q2, j = g.By + (int)r2;
     } else
         return;
     int si = i, sj = j, sk = k, flip = 0;
@@ -239,9 +262,8 @@ __global__ void __launch_bounds__(256) k_states(Geom g, const DevThermo *__restr
             f.e[id] = tme;
         }
         // validation on interior cells
-        const int i = (int)(id % g.Xmax);
-        const int j = (int)((id / g.Xmax) % g.Ymax);
-        const int k = (int)(id / ((long long)g.Xmax * g.Ymax));
+        int i, j, k;
+        cell_ijk(g, id, i, j, k);
         const bool interior = i >= g.Bx && i < g.Xmax - g.Bx && j >= g.By && j < g.Ymax - g.By && k >= g.Bz &&
                               k < g.Zmax - g.Bz;
         if (interior) {

@@ -153,9 +153,8 @@ __global__ void __launch_bounds__(256) k_transport(Geom g, const DevThermo *__re
                 D *= 1.0e-1;
                 f.Dkm[k * N + id] = smax(D, 1.0e-10);
             }
-            const int i = (int)(id % g.Xmax);
-            const int j = (int)((id / g.Xmax) % g.Ymax);
-            const int kk = (int)(id / ((long long)g.Xmax * g.Ymax));
+            int i, j, kk;
+            cell_ijk(g, id, i, j, kk);
             if (i >= g.Bx && i < g.Xmax - g.Bx && j >= g.By && j < g.Ymax - g.By && kk >= g.Bz && kk < g.Zmax - g.Bz) {
 #pragma unroll
                 for (int k = 0; k < NS; k++) ymn[k] = yi[k], ymx[k] = yi[k];

@@ -456,12 +456,13 @@ __device__ __forceinline__ void positivity_face(double *Fw, const double *UL, co
 
 // Waves per SIMD the sweep is compiled for (the register budget per lane is
 // 512 / waves): the species count sets how much state a face lane carries
-template <int NS> struct SweepOccupancy {
-    static constexpr int waves = NS == 1 ? 4 : (NS <= 3 ? 3 : 2);
+template <int NS, int PREC> struct SweepOccupancy {
+    // the mixed form's FP32 projections leave room for a fourth wave at NS <= 3
+    static constexpr int waves = NS == 1 ? 4 : (NS <= 3 ? (PREC == XF_MIXED ? 4 : 3) : 2);
 };
 
 template <int NS, bool COP, int DIR, int SCHEME, int PREC, int SHAPE>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SweepOccupancy<NS>::waves)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SweepOccupancy<NS, PREC>::waves)))
 k_sweep(Geom g, const DevThermo *__restrict__ th, const double *__restrict__ Ug,
                                                Fields f, double ncop_gamma, int mode, int stage, const double *Ua,
                                                const double *Ub, double *Uo)

@@ -69,120 +69,122 @@ __global__ void __launch_bounds__(256) k_transport(Geom g, const DevThermo *__re
     constexpr int NCOP = NS - 1;
     const long long N = g.N;
     const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    double ymn[NS], ymx[NS];
-#pragma unroll
-    for (int k = 0; k < NS; k++) ymn[k] = 0.0, ymx[k] = 0.0;
-    if (id < N) {
-        double U[E];
-#pragma unroll
-        for (int n = 0; n < E; n++) U[n] = Ug[n * N + id];
-        const double rho = U[0];
-        const double rho1 = 1.0 / rho;
+    const bool live = id < N;
+    // mass fractions (Getrhoyi), velocities (GetStates) and mole fractions
+    // (get_xi); the mass fractions feed the min / max and are then dropped
+    double X[NS], C_total = 0.0, rho = 1.0, rho1 = 1.0;
+    {
         double yi[NS];
-        yi[NCOP] = 1.0;
 #pragma unroll
-        for (int ii = 5; ii < E; ii++) yi[ii - 5] = U[ii] * rho1, yi[NCOP] += -yi[ii - 5];
-        f.u[id] = U[1] * rho1;
-        f.v[id] = U[2] * rho1;
-        f.w[id] = U[3] * rho1;
-        const double T = f.T[id], p = f.p[id];
-        const double lT = dlog(T);
-        // get_xi
-        double X[NS], C_total = 0.0;
+        for (int k = 0; k < NS; k++) yi[k] = 0.0;
+        if (live) {
+            double U[E];
 #pragma unroll
-        for (int i = 0; i < NS; i++) {
-            X[i] = yi[i] * th->_Wi[i] * 1e-3 * rho;
-            C_total = C_total + X[i];
-        }
-        const double _C_total = 1.0 / C_total;
+            for (int n = 0; n < E; n++) U[n] = Ug[n * N + id];
+            rho = U[0];
+            rho1 = 1.0 / rho;
+            yi[NCOP] = 1.0;
 #pragma unroll
-        for (int i = 0; i < NS; i++) X[i] = X[i] * _C_total;
-        // Wilke mixture viscosity and conductivity (the conductivity uses the
-        // viscosity's denominators, as Visc_device.h:121-134 does)
-        double mu[NS], rmu[NS];
-#pragma unroll
-        for (int k = 0; k < NS; k++) mu[k] = dexp(fit4(tr->fcv + 4 * k, lT)), rmu[k] = 1.0 / mu[k];
-        double va = 0.0, tca = 0.0;
-        const double s2 = sqrt(2.0);
-#pragma unroll
-        for (int k = 0; k < NS; k++) {
-            double den = 0.0;
+            for (int ii = 5; ii < E; ii++) yi[ii - 5] = U[ii] * rho1, yi[NCOP] += -yi[ii - 5];
+            f.u[id] = U[1] * rho1;
+            f.v[id] = U[2] * rho1;
+            f.w[id] = U[3] * rho1;
 #pragma unroll
             for (int i = 0; i < NS; i++) {
-                double phi = tr->phiA[k * MAXNS + i] * sqrt(qdiv(mu[k], mu[i], rmu[i]));
-                phi = (phi + 1.0) * (phi + 1.0) * 0.5 / s2;
-                phi = phi * tr->phiB[k * MAXNS + i];
-                den = den + X[i] * phi;
+                X[i] = yi[i] * th->_Wi[i] * 1e-3 * rho;
+                C_total = C_total + X[i];
             }
-            const double _den = 1.0 / den;
-            va = va + X[k] * mu[k] * _den;
-            if (heat) tca = tca + X[k] * dexp(fit4(tr->fct + 4 * k, lT)) * _den;
+            const double _C_total = 1.0 / C_total;
+#pragma unroll
+            for (int i = 0; i < NS; i++) X[i] = X[i] * _C_total;
         }
-        f.mu[id] = va;
-        if (heat) f.kap[id] = tca;
         if (diffu) {
-            // mixture-averaged D_km: temp2[k] = 1e-20 + sum_{i != k, i ascending}
-            // (X_i + 1e-40) / (D_ik + 1e-40), with D_ik = exp(fit(ln T)) / p.
-            // The fits are symmetric (Setup copies the lower triangle from
-            // the upper), so each pair's exponential serves both species.
-            double t2[NS];
-            const double rp = 1.0 / p;
-#pragma unroll
-            for (int k = 0; k < NS; k++) t2[k] = 1.0e-20;
+            // interior cells' min / max of each mass fraction: wave shuffles,
+            // then the block's four waves in LDS (other cells contribute 0.0,
+            // the value the reference's reduction combines with)
+            int i = 0, j = 0, kk = 0;
+            if (live) cell_ijk(g, id, i, j, kk);
+            const bool in = live && i >= g.Bx && i < g.Xmax - g.Bx && j >= g.By && j < g.Ymax - g.By &&
+                            kk >= g.Bz && kk < g.Zmax - g.Bz;
+            __shared__ double sm[4][2 * NS];
+            const int wv = threadIdx.x >> 6;
 #pragma unroll
             for (int k = 0; k < NS; k++) {
+                double a = in ? yi[k] : 0.0, b = a;
 #pragma unroll
-                for (int i = 0; i < k; i++) {
-                    const double D = qdiv(dexp(fit4(tr->Dkj + 4 * (i * MAXNS + k), lT)), p, rp);
-                    const double De = D + 1.0e-40, rDe = 1.0 / De;
-                    t2[k] += qdiv(X[i] + 1.0e-40, De, rDe);
-                    t2[i] += qdiv(X[k] + 1.0e-40, De, rDe);
+                for (int off = 32; off >= 1; off >>= 1) {
+                    a = smin(a, __shfl_xor(a, off, 64));
+                    b = smax(b, __shfl_xor(b, off, 64));
                 }
+                if ((threadIdx.x & 63) == 0) sm[wv][k] = a, sm[wv][NS + k] = b;
             }
-#pragma unroll
-            for (int k = 0; k < NS; k++) {
-                double t1 = 0.0;
-#pragma unroll
-                for (int i = 0; i < NS; i++)
-                    if (i != k) t1 += (X[i] + 1.0e-40) * th->Wi[i];
-                double D;
-                if (t1 <= 0.0) // sycl::step(ceil(temp1), 0)
-                    D = dexp(fit4(tr->Dkj + 4 * (k * MAXNS + k), lT)) / p;
-                else
-                    D = qdiv(t1 / t2[k], rho, rho1) * C_total;
-                D *= 1.0e-1;
-                f.Dkm[k * N + id] = smax(D, 1.0e-10);
-            }
-            int i, j, kk;
-            cell_ijk(g, id, i, j, kk);
-            if (i >= g.Bx && i < g.Xmax - g.Bx && j >= g.By && j < g.Ymax - g.By && kk >= g.Bz && kk < g.Zmax - g.Bz) {
-#pragma unroll
-                for (int k = 0; k < NS; k++) ymn[k] = yi[k], ymx[k] = yi[k];
+            __syncthreads();
+            if (threadIdx.x < 2 * NS) {
+                const int q = threadIdx.x;
+                double v = sm[0][q];
+                for (int w = 1; w < 4; w++) v = q < NS ? smin(v, sm[w][q]) : smax(v, sm[w][q]);
+                f.ypart[(long long)blockIdx.x * 2 * NS + q] = v;
             }
         }
     }
+    if (!live) return;
+    const double T = f.T[id], p = f.p[id];
+    const double lT = dlog(T);
+    // Wilke mixture viscosity and conductivity (the conductivity uses the
+    // viscosity's denominators, as Visc_device.h:121-134 does)
+    double mu[NS], rmu[NS];
+#pragma unroll
+    for (int k = 0; k < NS; k++) mu[k] = dexp(fit4(tr->fcv + 4 * k, lT)), rmu[k] = 1.0 / mu[k];
+    double va = 0.0, tca = 0.0;
+    const double s2 = sqrt(2.0);
+#pragma unroll
+    for (int k = 0; k < NS; k++) {
+        double den = 0.0;
+#pragma unroll
+        for (int i = 0; i < NS; i++) {
+            double phi = tr->phiA[k * MAXNS + i] * sqrt(qdiv(mu[k], mu[i], rmu[i]));
+            phi = (phi + 1.0) * (phi + 1.0) * 0.5 / s2;
+            phi = phi * tr->phiB[k * MAXNS + i];
+            den = den + X[i] * phi;
+        }
+        const double _den = 1.0 / den;
+        va = va + X[k] * mu[k] * _den;
+        if (heat) tca = tca + X[k] * dexp(fit4(tr->fct + 4 * k, lT)) * _den;
+    }
+    f.mu[id] = va;
+    if (heat) f.kap[id] = tca;
     if (diffu) {
-        // wave min / max through shuffles, then the block's four waves in LDS
-        // (cells that are not interior contribute 0.0, the value the
-        // reference's reduction combines with)
-        __shared__ double sm[4][2 * NS];
-        const int wv = threadIdx.x >> 6;
+        // mixture-averaged D_km: temp2[k] = 1e-20 + sum_{i != k, i ascending}
+        // (X_i + 1e-40) / (D_ik + 1e-40), with D_ik = exp(fit(ln T)) / p.
+        // The fits are symmetric (Setup copies the lower triangle from the
+        // upper), so each pair's exponential serves both species.
+        double t2[NS];
+        const double rp = 1.0 / p;
+#pragma unroll
+        for (int k = 0; k < NS; k++) t2[k] = 1.0e-20;
 #pragma unroll
         for (int k = 0; k < NS; k++) {
-            double a = ymn[k], b = ymx[k];
 #pragma unroll
-            for (int off = 32; off >= 1; off >>= 1) {
-                a = smin(a, __shfl_xor(a, off, 64));
-                b = smax(b, __shfl_xor(b, off, 64));
+            for (int i = 0; i < k; i++) {
+                const double D = qdiv(dexp(fit4(tr->Dkj + 4 * (i * MAXNS + k), lT)), p, rp);
+                const double De = D + 1.0e-40, rDe = 1.0 / De;
+                t2[k] += qdiv(X[i] + 1.0e-40, De, rDe);
+                t2[i] += qdiv(X[k] + 1.0e-40, De, rDe);
             }
-            if ((threadIdx.x & 63) == 0) sm[wv][k] = a, sm[wv][NS + k] = b;
         }
-        __syncthreads();
-        if (threadIdx.x < 2 * NS) {
-            const int q = threadIdx.x;
-            double v = sm[0][q];
-            for (int w = 1; w < 4; w++) v = q < NS ? smin(v, sm[w][q]) : smax(v, sm[w][q]);
-            f.ypart[(long long)blockIdx.x * 2 * NS + q] = v;
+#pragma unroll
+        for (int k = 0; k < NS; k++) {
+            double t1 = 0.0;
+#pragma unroll
+            for (int i = 0; i < NS; i++)
+                if (i != k) t1 += (X[i] + 1.0e-40) * th->Wi[i];
+            double D;
+            if (t1 <= 0.0) // sycl::step(ceil(temp1), 0)
+                D = dexp(fit4(tr->Dkj + 4 * (k * MAXNS + k), lT)) / p;
+            else
+                D = qdiv(t1 / t2[k], rho, rho1) * C_total;
+            D *= 1.0e-1;
+            f.Dkm[k * N + id] = smax(D, 1.0e-10);
         }
     }
 }

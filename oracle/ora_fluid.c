@@ -24,9 +24,9 @@
  *   - dt: brute-force max over the padded grid (numpy);
  *   - ghost fill: numpy restatement of each BC type.
  *   ReconstructSoundSpeed / SoundSpeedMultiSpecies (multi-species Roe c^2)
- *   is pinned only indirectly (it reduces to gamma p/rho for identical
- *   left/right states and to the single-species c^2 for one species):
- *   "parity unpinned" for the general correction term.
+ *   is pinned in part (tests/test_oracle_recon.py): between equal states it
+ *   is (gamma - 1)(q^2/2 + p/rho + c_p T - h), and it is symmetric in the two
+ *   cells; the Roe weighting of the jump terms is "parity unpinned".
  */
 #include "ora_internal.h"
 #include <stdlib.h>
@@ -656,6 +656,21 @@ static double reconstruct_c2(const ora_fluid *f, size_t id_l, size_t id_r, doubl
     double c2 = csqr;
     double c2w = (c2 <= 0.0) ? 1.0 : 0.0;
     c2 = *Gamma * _P * _rho * c2w + (1.0 - c2w) * c2;
+    return c2;
+}
+
+/* The multi-species Roe sound speed of face (id_l, id_r) for the primitives
+   of the last UpdateStates (MARCO_ROE + reconstruct_c2), with b1 and b3 */
+double ora_face_c2(ora_fluid *f, size_t id_l, size_t id_r, double *b1_out, double *b3_out)
+{
+    double D = sqrt(f->rho[id_r] / f->rho[id_l]);
+    double D1 = 1.0 / (D + 1.0);
+    double _P = (f->p[id_l] + D * f->p[id_r]) * D1;
+    double _rho = sqrt(f->rho[id_r] * f->rho[id_l]);
+    double _yi[ORA_MAXNS] = {0}, z[ORA_MAXNS] = {0}, b1 = 0.0, b3 = 0.0, Gamma0;
+    double c2 = reconstruct_c2(f, id_l, id_r, D, D1, _rho, _P, _yi, z, &b1, &b3, &Gamma0);
+    if (b1_out) *b1_out = b1;
+    if (b3_out) *b3_out = b3;
     return c2;
 }
 

@@ -98,6 +98,8 @@ double *ora_field(ora_fluid *f, const char *name);
 void ora_boundary(ora_fluid *f, int which);                 /* FluidBoundaryCondition */
 int ora_update_states(ora_fluid *f, int which);             /* UpdateFluidStateFlux   */
 void ora_compute_lu(ora_fluid *f, int which);               /* GetLU                  */
+/* Roe-averaged multi-species c^2 of face (id_l, id_r) (ReconstructSoundSpeed) */
+double ora_face_c2(ora_fluid *f, size_t id_l, size_t id_r, double *b1, double *b3);
 void ora_rk3(ora_fluid *f, int stage, double dt);           /* UpdateURK3rd           */
 double ora_get_dt(ora_fluid *f);                            /* GetDt                  */
 void ora_chemistry(ora_fluid *f, double dt);                /* ChemeODEQ2Solver       */

@@ -69,6 +69,7 @@ def load(path):
         "ora_eigen_lr": (None, [P, C.c_int, C.c_int, _DP, _DP, _DP, _DP, _DP] + [C.c_double] * 8),
         "ora_evalcv": (None, [P, _DP, _DP, _DP, C.c_double, C.c_double]),
         "ora_chemeq2": (C.c_int, [P, _DP, C.c_double, C.c_double, C.c_double]),
+        "ora_face_c2": (C.c_double, [v, C.c_size_t, C.c_size_t, _DP, _DP]),
     }
     for n, (r, a) in sig.items():
         f = getattr(L, n)

@@ -157,3 +157,41 @@ def test_wenocu6_smooth_data_converges_at_high_order():
         errs.append(abs(cu6(avg, np.zeros(8), delta=h) - face))
     orders = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
     assert np.all(orders > 5.0), (errs, orders)
+
+
+@pytest.mark.parametrize("case,over", [("shock-bubble", dict(nx=16, ny=8)), ("mixing-layer", dict(nx=6, ny=6, nz=6))])
+def test_multispecies_roe_sound_speed(case, over):
+    """ReconstructSoundSpeed (Utils_device.hpp:42-140) on a face whose two
+    cells hold the same state: every jump vanishes, the correction term is
+    zero and the species sums collapse (sum_{k<N} Y_k = 1 - Y_N), leaving
+    c^2 = (gamma - 1) (q^2/2 + p/rho + c_p T - h) with gamma = c_p / c_v and
+    h = sum_k Y_k h_k; on faces between different cells the value is
+    symmetric in the two cells (the Roe weights and the jumps both swap)."""
+    import xfluids_amd as xf
+    from oracle_lib import Oracle, config_from_setup
+    s = xf.Setup(xf.case_path(case), **over)
+    d = s.desc
+    U, T = s.initial(case, seed=1)
+    o = Oracle(config_from_setup(s))
+    o.set_state(U, T)
+    assert o.initialize() == 0
+    L = lib()
+    cfg = o.cfg
+    rho, p, Tt = o.field("rho"), o.field("p"), o.field("T")
+    y = o.field("y").reshape(-1, d.NS)
+    q2 = o.field("u") ** 2 + o.field("v") ** 2 + o.field("w") ** 2
+    b1, b3 = C.c_double(), C.c_double()
+    for cid in range(0, d.ncell, 5):
+        yy = np.ascontiguousarray(y[cid])
+        cp = L.ora_mix_cp(C.byref(cfg), yy.ctypes.data_as(DP), Tt[cid])
+        cv = L.ora_mix_cv(C.byref(cfg), yy.ctypes.data_as(DP), Tt[cid])
+        h = L.ora_mix_h(C.byref(cfg), yy.ctypes.data_as(DP), Tt[cid])
+        want = (cp / cv - 1.0) * (0.5 * q2[cid] + p[cid] / rho[cid] + cp * Tt[cid] - h)
+        got = L.ora_face_c2(o._h, cid, cid, C.byref(b1), C.byref(b3))
+        assert abs(got / want - 1.0) < 1e-13, (cid, got, want)
+    worst = 0.0
+    for cid in range(0, d.ncell - 1, 3):
+        a = L.ora_face_c2(o._h, cid, cid + 1, C.byref(b1), C.byref(b3))
+        b = L.ora_face_c2(o._h, cid + 1, cid, C.byref(b1), C.byref(b3))
+        worst = max(worst, abs(a - b) / abs(a))
+    assert worst < 1e-12, worst

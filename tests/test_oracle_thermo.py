@@ -124,3 +124,24 @@ def test_newton_temperature_roundtrip(cfg):
         cv = L.ora_mix_cv(C.byref(c), yp, T)
         R = L.ora_mix_R(C.byref(c), yp)
         assert abs(cp - cv - R) <= 1e-12 * cp and cp / cv > 1.0
+
+
+def test_newton_step_limits_and_iteration_cap(cfg):
+    """get_T's damped Newton (Mixing_device.h:171-193): each step is clamped to
+    T - 1e-3 T .. T + 1e-2 T and the loop stops after 100 iterations.  From a
+    guess far below (above) the root every step hits the upper (lower) clamp,
+    so the result is the guess grown by 1 % (shrunk by 0.1 %) 100 times,
+    through the same floating-point recurrence T <- T - df."""
+    c, _ = cfg
+    L = lib()
+    DP = C.POINTER(C.c_double)
+    y = np.ascontiguousarray(np.full(len(SPECIES), 1.0 / len(SPECIES)))
+    yp = y.ctypes.data_as(DP)
+    for T_true, T0, frac in ((3500.0, 300.0, -1e-2), (300.0, 3000.0, 1e-3)):
+        e = L.ora_mix_e(C.byref(c), yp, T_true)
+        want = T0
+        for _ in range(100):
+            want = want - frac * want
+        assert L.ora_get_T(C.byref(c), yp, e, T0) == want
+        # one more iteration's worth of room is not taken: the cap is 100
+        assert abs(want - T_true) > 50.0

@@ -95,12 +95,16 @@ void boundary_async(xf_ctx *c, double *U, int reset_red)
                        c->d.BC[1], c->d.BC[2], c->d.BC[3], c->d.BC[4], c->d.BC[5], c->f.red, reset_red);
 }
 
-// UpdateStates; reset = 0 when the preceding boundary launch already zeroed
-// the per-pass maxima
-void states_async(xf_ctx *c, const double *U, int full, int reset = 1)
+// UpdateStates (the pass's GLF and CFL maxima replace the stored ones);
+// fill = 1 also runs FluidBoundaryCondition first, fused into the same pass
+void states_async(xf_ctx *c, double *U, int full, int fill = 0)
 {
-    if (reset) hipLaunchKernelGGL(k_reset_pass_red, dim3(1), dim3(64), 0, c->stream, c->f.red);
-    c->ops.states(c, U, full);
+    int bcs = 0;
+    if (fill) {
+        for (int q = 0; q < 6; q++) bcs |= (c->d.BC[q] & 15) << (4 * q);
+        bcs |= 1 << 28;
+    }
+    c->ops.states(c, U, full, bcs);
 }
 
 // GetLU: sweeps in x, y, z order (each folds its direction's GLF maxima).
@@ -156,8 +160,7 @@ void step_mid_async(xf_ctx *c)
     double *in[3] = {f.U, f.U1, c->U2};
     for (int s = 1; s <= 3; s++) {
         double *Ug = in[s - 1];
-        boundary_async(c, Ug, 1);
-        states_async(c, Ug, 0, 0);
+        states_async(c, Ug, 0, 1); // ghost fill + UpdateStates
         double *Uo = (s == 1) ? f.U1 : (s == 2 ? c->U2 : f.U);
         lu_async(c, Ug, s, f.U, (s == 1) ? f.U1 : Ug, Uo);
     }
@@ -708,7 +711,7 @@ int time_launches(xf_ctx *c, int dir, int reps, double *avg_ms)
     XF_HIP(hipEventRecord(a, c->stream));
     for (int r = 0; r < reps; r++) {
         if (dir < 0) {
-            c->ops.states(c, c->f.U, 0);
+            states_async(c, c->f.U, 0, 1); // as in a step: ghost fill + UpdateStates
         } else if (dir == 10) {
             c->ops.visc_prep(c, c->f.U);
         } else if (dir >= 20) {

@@ -22,7 +22,8 @@ struct EvPair {
 };
 
 struct Ops {
-    void (*states)(xf_ctx *, const double *U, int full);
+    // UpdateStates on U; bcs != 0 (packed BC codes) also fills the ghost cells
+    void (*states)(xf_ctx *, double *U, int full, int bcs);
     void (*sweep)(xf_ctx *, int dir, const double *Ug, int mode, int stage, const double *Ua, const double *Ub,
                   double *Uo);
     void (*chem)(xf_ctx *, int record);
@@ -109,11 +110,11 @@ struct ProfScope {
 };
 
 template <int NS, bool COP> struct Launch {
-    static void states(xf_ctx *c, const double *U, int full)
+    static void states(xf_ctx *c, double *U, int full, int bcs)
     {
         ProfScope ps(c, "states");
         hipLaunchKernelGGL((k_states<NS, COP>), dim3(nblk(c->N)), dim3(256), 0, c->stream, c->g, c->d_th, U, c->f,
-                           c->d.ncop_gamma, full);
+                           c->d.ncop_gamma, full, bcs);
         hipLaunchKernelGGL(k_states_fold, dim3(12), dim3(1024), 0, c->stream, c->f.spart, (int)nblk(c->N), c->f.red);
     }
     template <int DIR, int SCH, int PREC, int SHAPE>

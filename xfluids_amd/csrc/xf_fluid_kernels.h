@@ -169,9 +169,14 @@ static __global__ void __launch_bounds__(256) k_boundary_all(Geom g, double *__r
 // ---------------------------------------------------------------------------
 // conservative -> primitive over all padded cells
 // ---------------------------------------------------------------------------
+// bcs: 0 = the ghost cells already hold their values; otherwise the six BC
+// codes packed 4 bits each (XMIN in bits 0-3 ... ZMAX in bits 20-23) plus
+// bit 28, and every ghost cell first takes its value from its source cell as
+// k_boundary_all forms it (FluidBoundaryCondition, fused into UpdateStates:
+// the sources are interior cells or Inflow ghosts, which no thread writes)
 template <int NS, bool COP>
-__global__ void __launch_bounds__(256) k_states(Geom g, const DevThermo *__restrict__ th, const double *__restrict__ Ug,
-                                                Fields f, double ncop_gamma, int full)
+__global__ void __launch_bounds__(256) k_states(Geom g, const DevThermo *__restrict__ th, double *Ug, Fields f,
+                                                double ncop_gamma, int full, int bcs)
 {
     constexpr int E = COP ? NS + 4 : 5;
     constexpr int NCOP = COP ? NS - 1 : 0;
@@ -185,9 +190,32 @@ __global__ void __launch_bounds__(256) k_states(Geom g, const DevThermo *__restr
     for (int q = 0; q < 3; q++) dtm[q] = 0.0;
     int ebits = 0;
     if (live) {
+        int i, j, k;
+        cell_ijk(g, id, i, j, k);
+        long long src = id;
+        int flip = 0;
+        bool wb = false;
+        if (bcs) {
+            int si = i, sj = j, sk = k;
+            auto bc = [&](int q) { return (bcs >> (4 * q)) & 15; };
+            if (g.DimZ) bc_axis(sk < g.Bz ? bc(4) : bc(5), sk, g.Bz, g.Z_inner, g.Zmax, 3, flip);
+            if (g.DimY) bc_axis(sj < g.By ? bc(2) : bc(3), sj, g.By, g.Y_inner, g.Ymax, 2, flip);
+            if (g.DimX) bc_axis(si < g.Bx ? bc(0) : bc(1), si, g.Bx, g.X_inner, g.Xmax, 1, flip);
+            if (si != i || sj != j || sk != k) {
+                src = (long long)g.Xmax * g.Ymax * sk + (long long)g.Xmax * sj + si;
+                wb = true;
+            }
+        }
         double U[E];
 #pragma unroll
-        for (int n = 0; n < E; n++) U[n] = Ug[n * N + id];
+        for (int n = 0; n < E; n++) {
+            const double v = Ug[n * N + src];
+            U[n] = ((flip >> n) & 1) ? -v : v;
+        }
+        if (wb) {
+#pragma unroll
+            for (int n = 0; n < E; n++) Ug[n * N + id] = U[n];
+        }
         // Getrhoyi
         const double rho = U[0];
         const double rho1 = 1.0 / rho;
@@ -262,8 +290,6 @@ __global__ void __launch_bounds__(256) k_states(Geom g, const DevThermo *__restr
             f.e[id] = tme;
         }
         // validation on interior cells
-        int i, j, k;
-        cell_ijk(g, id, i, j, k);
         const bool interior = i >= g.Bx && i < g.Xmax - g.Bx && j >= g.By && j < g.Ymax - g.By && k >= g.Bz &&
                               k < g.Zmax - g.Bz;
         if (interior) {
@@ -326,8 +352,8 @@ __global__ void __launch_bounds__(256) k_states(Geom g, const DevThermo *__restr
 }
 
 // the per-block GLF and CFL maxima of k_states into the reduction slots: one
-// block per slot (12), each the maximum of the stored slot value and every
-// positive block value, as the per-block atomic maxima would leave it
+// block per slot (12), each the maximum of +0 and every positive block value
+// (the pass's maxima, combined with the 0 the pass starts from)
 static __global__ void __launch_bounds__(1024) k_states_fold(const double *__restrict__ spart, int nblocks, double *red)
 {
     __shared__ unsigned long long s[1024];
@@ -344,23 +370,13 @@ static __global__ void __launch_bounds__(1024) k_states_fold(const double *__res
         if (threadIdx.x < w && s[threadIdx.x + w] > s[threadIdx.x]) s[threadIdx.x] = s[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        double *slot = &red[q < 9 ? RED_GLFP + q : RED_DT + q - 9];
-        const unsigned long long cur = (unsigned long long)__double_as_longlong(*slot);
-        if (s[0] > cur) *slot = __longlong_as_double((long long)s[0]);
-    }
+    if (threadIdx.x == 0) red[q < 9 ? RED_GLFP + q : RED_DT + q - 9] = __longlong_as_double((long long)s[0]);
 }
 
 // ---------------------------------------------------------------------------
 // small control kernels
 // ---------------------------------------------------------------------------
 // zero the per-pass maxima (pending GLF, CFL) before an UpdateStates pass
-static __global__ void k_reset_pass_red(double *red)
-{
-    if (threadIdx.x < 9) red[RED_GLFP + threadIdx.x] = 0.0;
-    if (threadIdx.x < 3) red[RED_DT + threadIdx.x] = 0.0;
-}
-
 // GetDt: dt = CFL / sum_d max_d / Delta_d, written to device memory
 static __global__ void k_dt(Geom g, double *red, double *dt)
 {

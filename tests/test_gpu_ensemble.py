@@ -11,12 +11,14 @@ pytestmark = pytest.mark.gpu
 
 REACTIVE = [dict(nx=24, ny=24, seed=1), dict(nx=32, ny=16, seed=2), dict(nx=16, ny=40, seed=3), dict(nx=8, ny=8, seed=4)]
 INERT = [dict(nx=64, ny=32, seed=1), dict(nx=48, ny=24, seed=2)]
+VISCOUS = [dict(nx=64, ny=32, seed=1, viscous=(1, 1, 1)), dict(nx=48, ny=24, seed=2, viscous=(1, 1, 1)),
+           dict(nx=40, ny=40, seed=3)]
 
 
 def _cases(case, specs):
     out = []
     for sp in specs:
-        s = xf.Setup(xf.case_path(case), nx=sp["nx"], ny=sp["ny"])
+        s = xf.Setup(xf.case_path(case), nx=sp["nx"], ny=sp["ny"], viscous=sp.get("viscous"))
         U, T = s.initial(case, seed=sp["seed"])
         out.append((s, U, T))
     return out
@@ -32,7 +34,8 @@ def _alone(s, U, T, n):
     return Ug, Tg
 
 
-@pytest.mark.parametrize("case,specs,nsteps", [("reactive-channel", REACTIVE, 3), ("shock-bubble", INERT, 4)])
+@pytest.mark.parametrize("case,specs,nsteps", [("reactive-channel", REACTIVE, 3), ("shock-bubble", INERT, 4),
+                                                ("shock-bubble", VISCOUS, 3)])
 @pytest.mark.parametrize("graph", [True, False])
 def test_ensemble_matches_single_cases(case, specs, nsteps, graph):
     announce("ensemble %s graph=%s" % (case, graph))

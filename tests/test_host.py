@@ -158,3 +158,31 @@ def test_checkpoint_format_round_trip(tmp_path):
         p.write_bytes(raw[:-8])
         with pytest.raises(xf.XfError):
             xf.checkpoint_read(p, d)
+
+
+def test_viscous_switches_need_a_mixture_and_wide_ghosts():
+    """The viscous terms read species transport data and reach 4 cells into
+    the ghost layers: a single-species case, or a ghost width below 4, is
+    refused before any device allocation."""
+    import ctypes as C
+    with pytest.raises(xf.XfError):
+        xf.Setup(xf.case_path("isentropic-vortex"), nx=16, ny=16, viscous=(1, 0, 0))
+    s = xf.Setup(xf.case_path("shock-bubble"), nx=16, ny=8, viscous=(1, 1, 1))
+    d = s.desc
+    assert d.visc == 1 and d.visc_heat == 1 and d.visc_diffu == 1
+    # BlSz.Yil_limiter = max(_dx, _dy, _dz) * clamp(1e10, 0, 1); Dim_limiter = clamp(2e-3, 0, 1)
+    assert d.Yil_limiter == max(d._dx, d._dy, d._dz) and d.Dim_limiter == 2e-3
+    bad = CaseDescCopy(d)
+    bad.Bwidth_X, bad.Xmax = 3, bad.X_inner + 6
+    h = C.c_void_p()
+    assert xf.lib().xf_create(C.byref(bad), C.byref(h)) == -1
+    nocop = CaseDescCopy(d)
+    nocop.cop, nocop.NS = 0, 1
+    assert xf.lib().xf_create(C.byref(nocop), C.byref(h)) == -1
+
+
+def CaseDescCopy(d):
+    import ctypes as C
+    out = xf.CaseDesc()
+    C.memmove(C.byref(out), C.byref(d), C.sizeof(d))
+    return out

@@ -46,9 +46,10 @@ TRAFFIC_FILE = os.path.join(PROFILES, "pmc-traffic-shock-bubble-1024x512.md")
 
 
 def measured_traffic():
-    """HBM bytes per launch per kernel class from the committed rocprofv3
-    counter summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
-    factor), or {} when none is committed."""
+    """HBM bytes per launch per kernel class (and per tile shape, key
+    "sweep_y/1") from the committed rocprofv3 counter summary (FETCH_SIZE +
+    WRITE_SIZE, scaled by the copy-kernel factor), or {} when none is
+    committed."""
     out = {}
     if os.path.exists(TRAFFIC_FILE):
         for ln in open(TRAFFIC_FILE):
@@ -90,12 +91,15 @@ def run(args, form):
     return d, value, ms_per_step, kt, finite, tuned
 
 
-def roofline(d, kt):
+def roofline(d, kt, shapes=None):
     # dominant kernel: every class runs three times per step, so the largest
     # average launch duration is also the largest share of the step
     name = max(kt, key=lambda k: kt[k][0])
     avg_ms, cnt = kt[name]
-    traffic = measured_traffic().get(name)
+    tr = measured_traffic()
+    traffic = tr.get(name)
+    if name.startswith("sweep") and shapes is not None:
+        traffic = tr.get("%s/%d" % (name, int(shapes["xyz".index(name[-1])])), traffic)
     if name.startswith("sweep"):
         faces = {"sweep_x": (d.X_inner + 1) * d.Y_inner * d.Z_inner,
                  "sweep_y": d.X_inner * (d.Y_inner + 1) * d.Z_inner,
@@ -418,7 +422,7 @@ def main():
         res[f] = run(args, f)
         progress("%s form timed" % f)
     head = forms[0]
-    d, value, ms, kt, finite, _ = res[head]
+    d, value, ms, kt, finite, head_tuned = res[head]
     line = {
         "metric": "Mcell-updates/s per RK3 step, plain FP64 vs mixed/batched, at stated L1 err",
         "value": value,
@@ -434,7 +438,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "2D inert shock-bubble, Ns=3 (H2,O2,N2), JANAF, WENO5-Z, SSP-RK3, CFL 0.4",
                    "grid": "%dx%d" % (NX, NY), "form": head, "finite": finite},
-        "roofline": roofline(d, kt),
+        "roofline": roofline(d, kt, head_tuned[0]),
         "kernels_ms": {k: v[0] for k, v in kt.items()},
     }
     line["forms"] = {}

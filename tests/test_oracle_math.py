@@ -1,8 +1,8 @@
 """Pins of the oracle's deterministic exp/log against libm (math module).
 
 The plain FP64 GPU form and the oracle share this exp/log method; here it is
-checked against correctly-rounded-ish libm results: within 2 ulp over wide
-ranges, special values exact.
+checked against correctly-rounded-ish libm results: exp within 1 ulp and log
+within 2 ulp over wide ranges, special values exact.
 """
 import math
 
@@ -17,14 +17,16 @@ def ulp_diff(a, b):
     return np.abs(ia - ib)
 
 
-def test_exp_within_2ulp():
+def test_exp_within_1ulp():
     L = lib()
     rng = np.random.default_rng(0)
     xs = np.concatenate([rng.uniform(-700, 700, 20000), rng.uniform(-1, 1, 20000),
-                         rng.uniform(-50, 50, 20000), [0.0, 1.0, -1.0, 0.5, 1e-300, -1e-300]])
+                         rng.uniform(-50, 50, 20000), rng.uniform(-745.13, -708.4, 5000),
+                         rng.uniform(700.0, 709.78, 5000), [0.0, 1.0, -1.0, 0.5, 1e-300, -1e-300]])
     got = np.array([L.ora_exp(float(x)) for x in xs])
     ref = np.array([math.exp(float(x)) for x in xs])
-    assert ulp_diff(got, ref).max() <= 2
+    # the upper and lower bands reach the two-factor 2^k scaling and subnormal results
+    assert ulp_diff(got, ref).max() <= 1
 
 
 def test_log_within_2ulp():

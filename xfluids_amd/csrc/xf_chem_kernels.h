@@ -26,10 +26,23 @@ constexpr int CHEM_MAXSUB = 200000;
 
 __device__ __forceinline__ double pown_i(double x, int n)
 {
+    // the products of the loop below, written out for the usual orders
+    if (n == 1) return x;
+    if (n == 2) return x * x;
+    if (n == 3) return (x * x) * x;
     if (n == 0) return 1.0;
     double r = x;
     for (int k = 1; k < n; k++) r *= x;
     return r;
+}
+
+// Holds a wave-uniform value in scalar registers: the empty asm makes it
+// opaque, so the compiler keeps it instead of loading it again from memory
+// at every use (each reload costs a wait on the scalar cache)
+template <class T> __device__ __forceinline__ T keep_s(T v)
+{
+    asm volatile("" : "+s"(v));
+    return v;
 }
 
 // evalcv (y[0] = T): q, d of length NS + 1
@@ -71,43 +84,66 @@ __device__ __forceinline__ void evalcv_dev(const DevThermo *__restrict__ th, con
 #pragma unroll
     for (int n = 0; n < NS; n++) qi[n] = 0.0, pi[n] = 0.0;
     const int NR = mc->NR;
+    // EaR / T for every reaction from one reciprocal: Markstein's corrected
+    // quotient fma(fma(-T, q0, a), y, q0), q0 = a y, y = RN(1/T), is the
+    // correctly rounded a / T while a, T lie in [2^-500, 2^500] (the remainder
+    // is exact and nothing underflows); a = 0 keeps q0, the signed zero of a/T
+    const double rT = 1.0 / m_tmp;
+    const bool t_ok = __all(m_tmp >= 0x1p-500 && m_tmp <= 0x1p500);
     for (int r = 0; r < NR; r++) {
-        // uniform per-reaction record (scalar loads); species loops run over
-        // compile-time n and test the record's masks, which skips exactly the
-        // zero-coefficient terms (each adds +0 or multiplies by 1 in the
-        // reference, so skipping changes no bit)
+        // uniform per-reaction record, read in one burst of scalar loads and
+        // held in scalar registers; species loops run over compile-time n and
+        // test the record's masks, which skips exactly the zero-coefficient
+        // terms (each adds +0 or multiplies by 1 in the reference, so skipping
+        // changes no bit)
         const DevRxn &R = mc->rx[r];
-        double Kf = R.A * dexp(R.B * logT - R.EaR / m_tmp);
+        const double A = R.A, B = R.B, EaR = R.EaR;
+        const unsigned md = R.md, mt = R.mt, mf = R.mf, mb = R.mb;
+        const unsigned long long pd = R.pd, pf = R.pf, pb = R.pb;
+        const int tind = R.third_ind;
+        double ET;
+        const double aE = fabs(EaR);
+        if (t_ok && (EaR == 0.0 || (aE >= 0x1p-500 && aE <= 0x1p500))) {
+            const double q0 = EaR * rT;
+            ET = EaR == 0.0 ? q0 : fma(fma(-m_tmp, q0, EaR), rT, q0);
+        } else {
+            ET = EaR / m_tmp;
+        }
+        double Kf = A * dexp(B * logT - ET);
         double DeltaGibbs = 0.0;
         int m_dn = 0;
 #pragma unroll
         for (int n = 0; n < NS; n++) {
-            if ((R.md >> n) & 1u) {
-                const int nu = (int)((R.pd >> (4 * n)) & 15ull) - 8;
+            if ((md >> n) & 1u) {
+                const int nu = (int)((pd >> (4 * n)) & 15ull) - 8;
                 DeltaGibbs += nu * (logP - gk[n]);
                 m_dn += nu;
             }
         }
         double Kck = dmin(dexp(DeltaGibbs - m_dn * logStandConc), 1.0E+40);
         double tb = 1.0;
-        if (1 == R.third_ind) {
+        if (1 == tind) {
             tb = 0.0;
+            // every efficiency of the row in one burst of scalar loads
+            double eff[NS];
+#pragma unroll
+            for (int it = 0; it < NS; it++) eff[it] = mc->third[r * NS + it];
 #pragma unroll
             for (int it = 0; it < NS; it++)
-                if ((R.mt >> it) & 1u) tb += mc->third[r * NS + it] * m_cm[it];
+                if ((mt >> it) & 1u) tb += eff[it] * m_cm[it];
         }
         Kf *= tb;
         Kck *= Kf;
 #pragma unroll
         for (int it = 0; it < NS; it++)
-            if ((R.mf >> it) & 1u) Kf *= pown_i(m_cm[it], (int)((R.pf >> (4 * it)) & 15ull));
+            if ((mf >> it) & 1u) Kf *= pown_i(m_cm[it], (int)((pf >> (4 * it)) & 15ull));
 #pragma unroll
         for (int it = 0; it < NS; it++)
-            if ((R.mb >> it) & 1u) Kck *= pown_i(m_cm[it], (int)((R.pb >> (4 * it)) & 15ull));
+            if ((mb >> it) & 1u) Kck *= pown_i(m_cm[it], (int)((pb >> (4 * it)) & 15ull));
 #pragma unroll
         for (int n = 0; n < NS; n++) {
-            if (((R.mf | R.mb) >> n) & 1u) {
-                const int nf = (int)((R.pf >> (4 * n)) & 15ull), nb = (int)((R.pb >> (4 * n)) & 15ull);
+            if (((mf | mb) >> n) & 1u) {
+                const int nf = (int)((pf >> (4 * n)) & 15ull), nb = (int)((pb >> (4 * n)) & 15ull);
                 qi[n] += nb * Kf + nf * Kck;
                 pi[n] += nf * Kf + nb * Kck;
             }

@@ -4,48 +4,66 @@
 // bit, including the chemistry step-size decisions that amplify one-ulp
 // differences.  Vendor math libraries differ between host and device, so both
 // sides use the same published methods:
-//   exp: Cody & Waite reduction x = k ln2 + r (ln2 split into a 32-bit head
-//        and a tail), Taylor series of exp(r) to degree 13 (Horner), scaling
-//        by 2^k through the exponent field.
+//   exp: Cody & Waite reduction x = k ln2 + r with fused multiply-adds (ln2
+//        split into the nearest double and a tail), Taylor series of exp(r) to
+//        degree 13 by fused Horner steps, scaling by 2^k through the exponent
+//        field in two factors.
 //   log: x = m 2^e, m in [sqrt(1/2), sqrt(2)), f = m - 1 (exact),
 //        s = f/(2+f), log(1+f) = f - s (f - R), R = sum_{k=1..11} 2 s^{2k}/(2k+1).
 // Accuracy: within 2 ulp of a correctly rounded result over the ranges used.
-// All files including this header are compiled with -ffp-contract=off.
+// All files including this header are compiled with -ffp-contract=off; the
+// fused multiply-adds of exp are explicit fma calls on both sides.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace xf {
 
+// a double constant materialised in a scalar register pair where it is used
+__device__ __forceinline__ double sconst(double c)
+{
+    asm volatile("" : "+s"(c));
+    return c;
+}
+
 __device__ __forceinline__ double dexp(double x)
 {
-    if (x != x) return x;
-    if (x > 709.782712893384) return __longlong_as_double(0x7ff0000000000000LL);
-    if (x < -745.1332191019412) return 0.0;
-    const double t = x * 0x1.71547652b82fep+0;
-    const double kf = (t + 0x1.8p52) - 0x1.8p52;
+    // the reduction runs on x clamped to [-746, 710] (NaN goes to -746), so k
+    // stays in [-1076, 1024]; inputs outside the finite-result range are
+    // replaced at the end
+    const double xc = fmin(fmax(x, -746.0), 710.0);
+    const double kf = (xc * 0x1.71547652b82fep+0 + 0x1.8p52) - 0x1.8p52;
     const int k = (int)kf;
-    const double r = (x - kf * 0x1.62e42fee00000p-1) - kf * 0x1.a39ef35793c76p-33;
+    // ln2 = LN2_HI + LN2_LO with LN2_HI the double nearest ln2: the first fma
+    // is exact (|result| < 1/2 and a multiple of 2^-54)
+    double r = fma(-kf, 0x1.62e42fefa39efp-1, xc);
+    r = fma(-kf, 0x1.abc9e3b39803fp-56, r);
+    // the coefficients are forced into scalar registers right at their use:
+    // the fma then reads them from there, where a literal in a vector
+    // register per coefficient (hoisted out of loops) would cost two moves
+    // and two vector registers each
     double p = 0x1.6124613a86d09p-33;
-    p = 0x1.1eed8eff8d898p-29 + r * p;
-    p = 0x1.ae64567f544e4p-26 + r * p;
-    p = 0x1.27e4fb7789f5cp-22 + r * p;
-    p = 0x1.71de3a556c734p-19 + r * p;
-    p = 0x1.a01a01a01a01ap-16 + r * p;
-    p = 0x1.a01a01a01a01ap-13 + r * p;
-    p = 0x1.6c16c16c16c17p-10 + r * p;
-    p = 0x1.1111111111111p-7 + r * p;
-    p = 0x1.5555555555555p-5 + r * p;
-    p = 0x1.5555555555555p-3 + r * p;
-    p = 0x1.0p-1 + r * p;
-    p = 1.0 + r * p;
-    p = 1.0 + r * p;
-    if (k > 1023)
-        return (p * __longlong_as_double((long long)(k - 1 + 1023) << 52)) * 2.0;
-    if (k < -1022)
-        return (p * __longlong_as_double((long long)(k + 1000 + 1023) << 52)) *
-               __longlong_as_double((long long)(-1000 + 1023) << 52);
-    return p * __longlong_as_double((long long)(k + 1023) << 52);
+    p = fma(p, r, sconst(0x1.1eed8eff8d898p-29));
+    p = fma(p, r, sconst(0x1.ae64567f544e4p-26));
+    p = fma(p, r, sconst(0x1.27e4fb7789f5cp-22));
+    p = fma(p, r, sconst(0x1.71de3a556c734p-19));
+    p = fma(p, r, sconst(0x1.a01a01a01a01ap-16));
+    p = fma(p, r, sconst(0x1.a01a01a01a01ap-13));
+    p = fma(p, r, sconst(0x1.6c16c16c16c17p-10));
+    p = fma(p, r, sconst(0x1.1111111111111p-7));
+    p = fma(p, r, sconst(0x1.5555555555555p-5));
+    p = fma(p, r, sconst(0x1.5555555555555p-3));
+    p = fma(p, r, 0x1.0p-1);
+    p = fma(p, r, 1.0);
+    p = fma(p, r, 1.0);
+    // 2^k as 2^k1 * 2^k2, both factors normal: the first product is exact and
+    // the second rounds once, also for subnormal results
+    const int k1 = k >> 1, k2 = k - k1;
+    double e = (p * __longlong_as_double((long long)(k1 + 1023) << 52)) *
+               __longlong_as_double((long long)(k2 + 1023) << 52);
+    if (__builtin_expect(!(x >= -745.1332191019412 && x <= 709.782712893384), 0))
+        e = x != x ? x : (x > 0.0 ? __longlong_as_double(0x7ff0000000000000LL) : 0.0);
+    return e;
 }
 
 __device__ __forceinline__ double dlog(double x)

@@ -317,6 +317,12 @@ int xf_set_thermo(xf_ctx *c, const double *Hia, const double *Hib, const double 
     const int NS = c->d.NS;
     std::memcpy(c->h_th.Hia, Hia, sizeof(double) * NS * 21);
     std::memcpy(c->h_th.Hib, Hib, sizeof(double) * NS * 6);
+    for (int r = 0; r < 3; r++)
+        for (int n = 0; n < NS; n++) {
+            double *h = c->h_th.Hc + (r * MAXNS + n) * 9;
+            for (int m = 0; m < 7; m++) h[m] = Hia[n * 21 + m * 3 + r];
+            for (int m = 0; m < 2; m++) h[7 + m] = Hib[n * 6 + m * 3 + r];
+        }
     for (int n = 0; n < NS; n++) {
         // src/read_ini/src/thermal.cpp:158-161
         c->h_th.Wi[n] = Wi[n];

@@ -36,15 +36,6 @@ __device__ __forceinline__ double pown_i(double x, int n)
     return r;
 }
 
-// Holds a wave-uniform value in scalar registers: the empty asm makes it
-// opaque, so the compiler keeps it instead of loading it again from memory
-// at every use (each reload costs a wait on the scalar cache)
-template <class T> __device__ __forceinline__ T keep_s(T v)
-{
-    asm volatile("" : "+s"(v));
-    return v;
-}
-
 // evalcv (y[0] = T): q, d of length NS + 1
 template <int NS>
 __device__ __forceinline__ void evalcv_dev(const DevThermo *__restrict__ th, const DevMech *__restrict__ mc,

@@ -16,6 +16,11 @@ constexpr int MAXE = XF_MAXNS + 4;
 struct DevThermo {
     double Hia[MAXNS * 21];
     double Hib[MAXNS * 6];
+    // the same coefficients regrouped per (fit range r, species n): Hia's
+    // seven then Hib's two, Hc[(r * MAXNS + n) * 9 + m], so that one range of
+    // one species is 72 contiguous bytes (two scalar loads when r is
+    // wave-uniform, instead of nine strided ones)
+    double Hc[3 * MAXNS * 9];
     double Wi[MAXNS], _Wi[MAXNS], Ri[MAXNS];
     int thermo; // 1 NASA-9, 0 JANAF
 };

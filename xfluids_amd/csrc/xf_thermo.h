@@ -22,7 +22,7 @@ struct TPre {
     double T;    // max(T, 200)
     double iT;   // 1/T
     double lT;   // log(T) (only when requested)
-    int r;       // fit range index into Hia[.. * 3 + r]
+    int r;       // fit range index (Hia[.. * 3 + r], Hc[(r * MAXNS + n) * 9 + ..])
 };
 
 template <bool NEED_LOG>
@@ -43,8 +43,9 @@ __device__ __forceinline__ TPre tpre(double T0, int thermo)
     return t;
 }
 
-#define XF_HA(m) th->Hia[n * 21 + (m) * 3 + r]
-#define XF_HB(m) th->Hib[n * 6 + (m) * 3 + r]
+// Hia[n * 21 + m * 3 + r] and Hib[n * 6 + m * 3 + r], read from the regrouped copy
+#define XF_HA(m) th->Hc[(r * MAXNS + n) * 9 + (m)]
+#define XF_HB(m) th->Hc[(r * MAXNS + n) * 9 + 7 + (m)]
 
 // HeatCapacity_NASA / HeatCapacity_JANAF (Thermo_device.h:10-40)
 __device__ __forceinline__ double cp_k(const DevThermo *__restrict__ th, const TPre &t, double Ri, int n)

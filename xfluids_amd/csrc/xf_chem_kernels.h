@@ -167,12 +167,33 @@ __device__ __forceinline__ double alpha_of(double r)
 // ProductionRates.h:488-530); each later one is a predictor/corrector pass
 // (ProductionRates.h:531-638).  The sequence of operations per cell is the
 // reference's; only the place of the rate evaluation in the code is shared.
-template <int NS> struct ChemLane {
+//
+// Where the saved state (ys, qs, ds) and rtau live is a policy: registers for
+// the one-lane-per-cell kernel, LDS for the queue kernel, whose persistent
+// one-wave blocks would otherwise spill them to scratch memory at two waves
+// per SIMD.  Only the storage differs; the arithmetic is the same.
+template <int NL> struct ChemRegs {
+    double a[4][NL];
+    __device__ __forceinline__ double &at(int k, int i) { return a[k][i]; }
+};
+// arrays k = 0..3 of lane `lane`: p[(k * NL + i) * 64], lane-contiguous rows
+// (consecutive lanes hit consecutive banks)
+template <int NL> struct ChemLds {
+    double *p;
+    __device__ __forceinline__ double &at(int k, int i) { return p[(k * NL + i) * 64]; }
+};
+
+template <int NS, class St = ChemRegs<NS + 1>> struct ChemLane {
     static constexpr int NL = NS + 1;
-    double y[NL], ys[NL], qs[NL], ds[NL], q[NL], d[NL], rtau[NL];
+    double y[NL], q[NL], d[NL];
+    St st;
     double dt, dtg, dtmin, epsmax, ts, tn, rho, m_p;
     int passes;
     bool started;
+    __device__ __forceinline__ double &ys(int i) { return st.at(0, i); }
+    __device__ __forceinline__ double &qs(int i) { return st.at(1, i); }
+    __device__ __forceinline__ double &ds(int i) { return st.at(2, i); }
+    __device__ __forceinline__ double &rtau(int i) { return st.at(3, i); }
 
     __device__ __forceinline__ void begin(double dtg_, double rho_, double p_)
     {
@@ -200,10 +221,11 @@ template <int NS> struct ChemLane {
         // predictor
 #pragma unroll
         for (int i = 0; i < NL; i++) {
-            rtau[i] = dt * ds[i] / ys[i];
-            const double alpha = alpha_of(rtau[i]);
-            const double scr = dt * (qs[i] - ds[i]) / (1.0 + alpha * rtau[i]);
-            y[i] = dmax(ys[i] + scr, ymin);
+            const double rt = dt * ds(i) / ys(i);
+            rtau(i) = rt;
+            const double alpha = alpha_of(rt);
+            const double scr = dt * (qs(i) - ds(i)) / (1.0 + alpha * rt);
+            y[i] = dmax(ys(i) + scr, ymin);
         }
         tn = ts + dt;
         return 0;
@@ -228,7 +250,7 @@ template <int NS> struct ChemLane {
             dtmin *= dt;
             ts = 0.0, tn = 0.0;
 #pragma unroll
-            for (int i = 0; i < NL; i++) ys[i] = y[i], qs[i] = q[i], ds[i] = d[i];
+            for (int i = 0; i < NL; i++) ys(i) = y[i], qs(i) = q[i], ds(i) = d[i];
             started = true;
             return 0;
         }
@@ -237,14 +259,15 @@ template <int NS> struct ChemLane {
 #pragma unroll
         for (int i = 0; i < NL; i++) {
             const double yp = y[i];
-            const double rtaub = 0.5 * (rtau[i] + dt * d[i] / yp);
+            const double rtaub = 0.5 * (rtau(i) + dt * d[i] / yp);
             const double alpha = alpha_of(rtaub);
-            const double qt = (1.0 - alpha) * qs[i] + alpha * q[i];
-            const double scr = (qt * dt - rtaub * ys[i]) / (1.0 + alpha * rtaub);
-            const double scr2 = dmax(ys[i] + scr, 0.0);
+            const double ysi = ys(i);
+            const double qt = (1.0 - alpha) * qs(i) + alpha * q[i];
+            const double scr = (qt * dt - rtaub * ysi) / (1.0 + alpha * rtaub);
+            const double scr2 = dmax(ysi + scr, 0.0);
             double scr1 = fabs(scr2 - yp);
             y[i] = dmax(scr2, ymin);
-            if ((0.25 * (ys[i] + y[i])) > ymin) {
+            if ((0.25 * (ysi + y[i])) > ymin) {
                 scr1 = scr1 / y[i];
                 eps = dmax(0.5 * (scr1 + dmin(fabs(q[i] - d[i]) / (q[i] + d[i] + 1.0e-30), scr1)), eps);
             }
@@ -268,7 +291,7 @@ template <int NS> struct ChemLane {
         epsmax = 1.0;
         ts = tn;
 #pragma unroll
-        for (int i = 0; i < NL; i++) ys[i] = y[i], qs[i] = q[i], ds[i] = d[i];
+        for (int i = 0; i < NL; i++) ys(i) = y[i], qs(i) = q[i], ds(i) = d[i];
         return 0;
     }
 
@@ -391,10 +414,12 @@ struct ChemCase {
 // bounded by CHEM_MAXSUB passes.  The counter must be zero at launch.
 template <int NS, class Cases>
 __device__ __forceinline__ void chem_queue_run(const Cases &cases, long long total, const DevThermo *__restrict__ th,
-                                               const DevMech *__restrict__ mc, unsigned long long *queue, int record)
+                                               const DevMech *__restrict__ mc, unsigned long long *queue, int record,
+                                               double *lds)
 {
     const int lane = threadIdx.x & 63;
-    ChemLane<NS> L;
+    ChemLane<NS, ChemLds<NS + 1>> L;
+    L.st.p = lds + lane;
     ChemCase cc{};
     long long tl = -1, id = 0;
     double e0 = 0.0;
@@ -482,6 +507,12 @@ template <int NS> struct ChemOccupancy {
     static constexpr int waves = NS <= 3 ? 3 : 2;
 };
 
+// LDS of one queue block (one wave): ys, qs, ds and rtau of its 64 lanes,
+// 20 KiB at NS = 9, so the 8 blocks of two waves per SIMD fill the CU's 160 KiB
+template <int NS> struct ChemQueueLds {
+    static constexpr int doubles = 4 * (NS + 1) * 64;
+};
+
 template <int NS>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ChemOccupancy<NS>::waves)))
 k_chem_queue(Geom g, const DevThermo *__restrict__ th, const DevMech *__restrict__ mc, Fields f, int record)
@@ -491,8 +522,9 @@ k_chem_queue(Geom g, const DevThermo *__restrict__ th, const DevMech *__restrict
     one.c.g = g;
     one.c.U = f.U, one.c.T = f.T, one.c.p = f.p, one.c.dt = f.dt;
     one.c.substeps = f.substeps, one.c.err = f.err;
+    __shared__ double lds[ChemQueueLds<NS>::doubles];
     chem_queue_run<NS>(one, (long long)g.X_inner * g.Y_inner * g.Z_inner, th, mc, (unsigned long long *)f.queue,
-                       record);
+                       record, lds);
 }
 
 // one queue over the cells of every case of an ensemble (shared mechanism)
@@ -502,7 +534,8 @@ k_chem_queue_multi(const ChemCase *__restrict__ cases, int ncases, long long tot
                    const DevMech *__restrict__ mc, unsigned long long *queue, int record)
 {
     ManyCases many{cases, ncases};
-    chem_queue_run<NS>(many, total, th, mc, queue, record);
+    __shared__ double lds[ChemQueueLds<NS>::doubles];
+    chem_queue_run<NS>(many, total, th, mc, queue, record, lds);
 }
 
 // zero the work-queue counter (a kernel, so the step graph holds no memset)

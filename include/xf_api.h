@@ -116,6 +116,22 @@ int xf_set_thermo(xf_ctx *ctx, const double *Hia, const double *Hib, const doubl
 int xf_set_mechanism(xf_ctx *ctx, const int *Nu_f, const int *Nu_b, const double *React_ThirdCoef,
                      const double *Rargus);
 
+/* Compiled mechanisms.  The library is built with the records of the
+   mechanisms shipped under xfluids_amd/data as compile-time constants; when
+   the tables given to xf_set_mechanism equal one of them field by field, the
+   chemistry kernels run a rate evaluation specialised to it (same arithmetic,
+   bit-identical results, no per-reaction table reads or mask tests).  No
+   counterpart in the reference, whose kernels read the tables.
+   xf_mech_specialization_index: the compiled mechanism matching these tables
+   (same arguments as xf_set_mechanism plus NS, NR), or -1; host only.
+   xf_set_chem_specialization: on = 0 keeps the generic table loop for this
+   context (the default is on); drops a captured step graph.
+   xf_chem_specialization: the specialisation in use, or -1. */
+int xf_mech_specialization_index(int NS, int NR, const int *Nu_f, const int *Nu_b, const double *React_ThirdCoef,
+                                 const double *Rargus);
+int xf_set_chem_specialization(xf_ctx *ctx, int on);
+int xf_chem_specialization(const xf_ctx *ctx);
+
 /* Transport fits in the reference's layout, each a cubic in ln T:
    fitted_coefficients_visc[NS][4] (ln mu_k, Pa s), fitted_coefficients_therm[NS][4]
    (ln lambda_k, W/(m K)) and Dkj_matrix[NS*NS][4] (ln(p D_kj), D in cm^2/s,

@@ -52,3 +52,30 @@ def test_queue_matches_per_cell_steps(case):
     assert np.array_equal(a[0].view(np.uint64), b[0].view(np.uint64))
     assert np.array_equal(a[1].view(np.uint64), b[1].view(np.uint64))
     assert np.array_equal(a[2], b[2])
+
+
+@pytest.mark.parametrize("case", ["reactive_shock_tube", "reactive_channel", "mixing_layer_3d"])
+@pytest.mark.parametrize("driver", [xf.XF_CHEM_PER_CELL, xf.XF_CHEM_QUEUE])
+def test_compiled_mechanism_matches_table_loop(case, driver):
+    """The rate evaluation specialised to a shipped mechanism (records as
+    compile-time constants) against the generic loop over the device tables:
+    full steps, bit for bit, and the shipped mechanisms do select it."""
+    announce("compiled mechanism vs table loop %s driver %d" % (case, driver))
+    s, U, T, nsteps = make(case)
+    res = {}
+    for on in (True, False):
+        ctx = _ctx(s, U, T, driver)
+        ctx.set_chem_specialization(on)
+        spec = ctx.chem_specialization()
+        if on:
+            assert spec >= 0 and spec == s.mech_specialization_index()
+        else:
+            assert spec == -1
+        dts = ctx.steps(nsteps, graph=True, record_dt=True)
+        Ug, Tg = ctx.download(0)
+        ctx.close()
+        res[on] = (Ug, Tg, dts)
+    a, b = res[True], res[False]
+    assert np.array_equal(a[0].view(np.uint64), b[0].view(np.uint64))
+    assert np.array_equal(a[1].view(np.uint64), b[1].view(np.uint64))
+    assert np.array_equal(a[2], b[2])

@@ -186,3 +186,35 @@ def CaseDescCopy(d):
     out = xf.CaseDesc()
     C.memmove(C.byref(out), C.byref(d), C.sizeof(d))
     return out
+
+
+def test_shipped_mechanisms_have_compiled_rate_evaluations():
+    """Every reactive case's tables (parsed by the host reader) equal the
+    compile-time records of a shipped mechanism, and only exactly those: a
+    changed Arrhenius factor, third-body efficiency or stoichiometric
+    coefficient finds none (host-side matching, no GPU)."""
+    found = set()
+    for case in ("reactive-channel", "reactive-shock-tube", "mixing-layer"):
+        s = xf.Setup(xf.case_path(case))
+        d = s.desc
+        k = s.mech_specialization_index()
+        assert k >= 0, case
+        found.add(k)
+        Nu_f, Nu_b, third, Rargus = s.mechanism()
+        L = xf.lib()
+        idx = lambda *a: L.xf_mech_specialization_index(d.NS, d.NR, *a)
+        i32 = lambda a: a.ctypes.data_as(xf.C.POINTER(xf.C.c_int))
+        f64 = lambda a: a.ctypes.data_as(xf.C.POINTER(xf.C.c_double))
+        assert idx(i32(Nu_f), i32(Nu_b), f64(third), f64(Rargus)) == k
+        R2 = Rargus.copy()
+        R2[6 * 3] = np.nextafter(R2[6 * 3], np.inf)        # A of reaction 3, one ulp
+        assert idx(i32(Nu_f), i32(Nu_b), f64(third), f64(R2)) == -1
+        t2 = third.copy()
+        t2[np.flatnonzero(t2)[0]] *= 1.5
+        assert idx(i32(Nu_f), i32(Nu_b), f64(t2), f64(Rargus)) == -1
+        f2 = Nu_f.copy()
+        f2[0] += 1
+        assert idx(i32(f2), i32(Nu_b), f64(third), f64(Rargus)) == -1
+        assert L.xf_mech_specialization_index(d.NS - 1, d.NR, i32(Nu_f), i32(Nu_b), f64(third), f64(Rargus)) == -1
+    # the two shipped mechanisms (with argon and with nitrogen) are distinct
+    assert len(found) == 2

@@ -90,6 +90,9 @@ _SIGS = {
     "xf_set_thermo": (C.c_int, [_V, _DP, _DP, _DP]),
     "xf_set_mechanism": (C.c_int, [_V, _IP, _IP, _DP, _DP]),
     "xf_set_transport": (C.c_int, [_V, _DP, _DP, _DP]),
+    "xf_mech_specialization_index": (C.c_int, [C.c_int, C.c_int, _IP, _IP, _DP, _DP]),
+    "xf_set_chem_specialization": (C.c_int, [_V, C.c_int]),
+    "xf_chem_specialization": (C.c_int, [_V]),
     "xf_upload_state": (C.c_int, [_V, C.c_int, _DP, _DP]),
     "xf_download_state": (C.c_int, [_V, C.c_int, _DP, _DP]),
     "xf_download_prims": (C.c_int, [_V] + [_DP] * 10),
@@ -242,6 +245,15 @@ class Setup:
     def transport_exact(self, what, k, j, T, p=101325.0):
         """Kinetic-theory value behind a fit: what 0 mu_k [Pa s], 1 lambda_k [W/(m K)], 2 D_kj [cm^2/s]."""
         return lib().xf_setup_transport_exact(self._h, int(what), int(k), int(j), float(T), float(p))
+
+    def mech_specialization_index(self):
+        """Compiled mechanism whose records equal this case's tables, or -1
+        (xf_mech_specialization_index; host only)."""
+        d = self.desc
+        if d.NR <= 0:
+            return -1
+        Nu_f, Nu_b, third, Rargus = self.mechanism()
+        return int(lib().xf_mech_specialization_index(d.NS, d.NR, _i(Nu_f), _i(Nu_b), _d(third), _d(Rargus)))
 
     def mechanism(self):
         d = self.desc
@@ -418,6 +430,16 @@ class Context:
         out = np.zeros(3, np.int32)
         self._chk(lib().xf_get_tile_shape(self._h, _i(out)), "xf_get_tile_shape")
         return out
+
+    def set_chem_specialization(self, on=True):
+        """Use the compiled rate evaluation of a shipped mechanism when the
+        tables match one (default), or the generic table loop
+        (xf_set_chem_specialization)."""
+        self._chk(lib().xf_set_chem_specialization(self._h, 1 if on else 0), "xf_set_chem_specialization")
+
+    def chem_specialization(self):
+        """Index of the compiled mechanism in use, or -1 (xf_chem_specialization)."""
+        return int(lib().xf_chem_specialization(self._h))
 
     def autotune(self, reps=20):
         """Pick the fastest sweep tile shape per direction (xf_autotune);

@@ -41,6 +41,9 @@ def _run(cmd):
 
 
 def build_library(verbose=False):
+    # compile-time records of the shipped mechanisms (csrc/gen/xf_mech_gen.h)
+    from xfluids_amd import gen_mech
+    gen_mech.generate()
     os.makedirs(OBJ, exist_ok=True)
     jobs = []
     for s in HIP_SOURCES:

@@ -363,12 +363,12 @@ int xf_set_transport(xf_ctx *c, const double *fcv, const double *fct, const doub
     return XF_OK;
 }
 
-int xf_set_mechanism(xf_ctx *c, const int *Nu_f, const int *Nu_b, const double *third, const double *Rargus)
+// The reference's reaction tables plus the per-reaction records of the rate
+// kernel, from the host arrays (reaction.cpp:6-58, 170-172)
+static int build_mech(DevMech &m, int NS, int NR, const int *Nu_f, const int *Nu_b, const double *third,
+                      const double *Rargus)
 {
-    if (!c) return XF_EINVAL;
-    const int NS = c->d.NS, NR = c->d.NR;
-    if (NR > 0 && (!Nu_f || !Nu_b || !third || !Rargus)) return XF_EINVAL;
-    DevMech &m = c->h_mc;
+    m = DevMech{};
     m.NR = NR;
     for (int r = 0; r < NR; r++) {
         double sum = 0.0;
@@ -396,10 +396,49 @@ int xf_set_mechanism(xf_ctx *c, const int *Nu_f, const int *Nu_b, const double *
             if (nb != 0) x.mb |= 1u << j, x.pb |= (unsigned long long)nb << (4 * j);
         }
     }
-    XF_HIP(hipMemcpy(c->d_mc, &c->h_mc, sizeof(DevMech), hipMemcpyHostToDevice));
-    c->mech_set = true;
     return XF_OK;
 }
+
+int xf_set_mechanism(xf_ctx *c, const int *Nu_f, const int *Nu_b, const double *third, const double *Rargus)
+{
+    if (!c) return XF_EINVAL;
+    const int NS = c->d.NS, NR = c->d.NR;
+    if (NR > 0 && (!Nu_f || !Nu_b || !third || !Rargus)) return XF_EINVAL;
+    const int rc = build_mech(c->h_mc, NS, NR, Nu_f, Nu_b, third, Rargus);
+    if (rc != XF_OK) return rc;
+    XF_HIP(hipMemcpy(c->d_mc, &c->h_mc, sizeof(DevMech), hipMemcpyHostToDevice));
+    c->mech_set = true;
+    // a compiled specialisation runs only on exactly its own records
+    c->mech_spec = c->spec_allowed ? mech_spec_find(NS, NR, c->h_mc.rx, c->h_mc.third) : -1;
+    c->ops.prepare(c);
+    // a captured step graph may hold the previous chemistry kernel
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec), c->gexec = nullptr;
+    if (c->graph) (void)hipGraphDestroy(c->graph), c->graph = nullptr;
+    return XF_OK;
+}
+
+int xf_mech_specialization_index(int NS, int NR, const int *Nu_f, const int *Nu_b, const double *third,
+                                 const double *Rargus)
+{
+    if (NS < 1 || NS > MAXNS || NR < 1 || NR > MAXNR || !Nu_f || !Nu_b || !third || !Rargus) return -1;
+    static DevMech m; // host-only scratch (the record build does not touch the GPU)
+    if (build_mech(m, NS, NR, Nu_f, Nu_b, third, Rargus) != XF_OK) return -1;
+    return mech_spec_find(NS, NR, m.rx, m.third);
+}
+
+int xf_set_chem_specialization(xf_ctx *c, int on)
+{
+    if (!c) return XF_EINVAL;
+    c->spec_allowed = on != 0;
+    c->mech_spec = (c->spec_allowed && c->mech_set) ? mech_spec_find(c->d.NS, c->d.NR, c->h_mc.rx, c->h_mc.third) : -1;
+    c->ops.prepare(c);
+    // a captured step graph holds the previous chemistry kernel
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec), c->gexec = nullptr;
+    if (c->graph) (void)hipGraphDestroy(c->graph), c->graph = nullptr;
+    return XF_OK;
+}
+
+int xf_chem_specialization(const xf_ctx *c) { return c ? c->mech_spec : -1; }
 
 static double *which_ptr(xf_ctx *c, int which)
 {
@@ -912,7 +951,7 @@ void ens_chem(xf_ensemble *e)
 {
     xf_ctx *c0 = e->cases[0];
     c0->ops.chem_multi(e->stream, e->d_cases, (int)e->cases.size(), e->total, c0->d_th, c0->d_mc, e->d_queue,
-                       e->blocks, 0);
+                       e->blocks, 0, c0->mech_spec);
 }
 
 // one step of every case: each case's stages on its own stream, the
@@ -986,7 +1025,7 @@ int xf_ensemble_create(xf_ctx *const *cases, int ncases, xf_ensemble **out)
         return fail(XF_EHIP);
     unsigned long long z[2] = {0, 0};
     if (hipMemcpy(e->d_queue, z, sizeof(z), hipMemcpyHostToDevice) != hipSuccess) return fail(XF_EHIP);
-    if (e->react) e->blocks = c0->ops.multi_blocks(e->total);
+    if (e->react) e->blocks = c0->ops.multi_blocks(e->total, c0->mech_spec);
     *out = e;
     return XF_OK;
 }

@@ -19,6 +19,8 @@
 // reaches it raises XF_ERR_CHEM.  The reference loop is unbounded.
 #pragma once
 #include "xf_thermo.h"
+#include "gen/xf_mech_gen.h"
+#include <utility>
 
 namespace xf {
 
@@ -36,8 +38,103 @@ __device__ __forceinline__ double pown_i(double x, int n)
     return r;
 }
 
-// evalcv (y[0] = T): q, d of length NS + 1
-template <int NS>
+// What one reaction reads besides its record: T, the shared reciprocal of T
+// and whether it qualifies for the corrected quotient, the logarithms
+struct RxnCtx {
+    double m_tmp, rT, logT, logP, logStandConc;
+    bool t_ok;
+};
+
+// One reaction's contribution to the production and loss rates (evalcore,
+// ProductionRates.h:117-176).  The record R and the third-body efficiencies
+// eff(s) come either from the device tables (generic loop) or from
+// compile-time constants (a specialisation of a shipped mechanism): the code
+// is this one function in both cases, so the results are the same bits.
+// Species loops run over compile-time n and test the record's masks, which
+// skips exactly the zero-coefficient terms (each adds +0 or multiplies by 1 in
+// the reference, so skipping changes no bit).
+template <int NS, class Eff>
+__device__ __forceinline__ void rxn_step(const DevRxn R, Eff eff, const RxnCtx &c, const double *gk,
+                                         const double *m_cm, double *qi, double *pi)
+{
+    const double A = R.A, B = R.B, EaR = R.EaR;
+    const unsigned md = R.md, mt = R.mt, mf = R.mf, mb = R.mb;
+    const unsigned long long pd = R.pd, pf = R.pf, pb = R.pb;
+    const int tind = R.third_ind;
+    // EaR / T from the shared reciprocal: Markstein's corrected quotient
+    // fma(fma(-T, q0, a), y, q0), q0 = a y, y = RN(1/T), is the correctly
+    // rounded a / T while a, T lie in [2^-500, 2^500] (the remainder is exact
+    // and nothing underflows); a = 0 keeps q0, the signed zero of a / T
+    double ET;
+    const double aE = fabs(EaR);
+    if (c.t_ok && (EaR == 0.0 || (aE >= 0x1p-500 && aE <= 0x1p500))) {
+        const double q0 = EaR * c.rT;
+        ET = EaR == 0.0 ? q0 : fma(fma(-c.m_tmp, q0, EaR), c.rT, q0);
+    } else {
+        ET = EaR / c.m_tmp;
+    }
+    double Kf = A * dexp(B * c.logT - ET);
+    double DeltaGibbs = 0.0;
+    int m_dn = 0;
+#pragma unroll
+    for (int n = 0; n < NS; n++) {
+        if ((md >> n) & 1u) {
+            const int nu = (int)((pd >> (4 * n)) & 15ull) - 8;
+            DeltaGibbs += nu * (c.logP - gk[n]);
+            m_dn += nu;
+        }
+    }
+    double Kck = dmin(dexp(DeltaGibbs - m_dn * c.logStandConc), 1.0E+40);
+    double tb = 1.0;
+    if (1 == tind) {
+        tb = 0.0;
+        // every efficiency of the row first (one burst of scalar loads when
+        // they come from the tables)
+        double e[NS];
+#pragma unroll
+        for (int it = 0; it < NS; it++) e[it] = eff(it);
+#pragma unroll
+        for (int it = 0; it < NS; it++)
+            if ((mt >> it) & 1u) tb += e[it] * m_cm[it];
+    }
+    Kf *= tb;
+    Kck *= Kf;
+#pragma unroll
+    for (int it = 0; it < NS; it++)
+        if ((mf >> it) & 1u) Kf *= pown_i(m_cm[it], (int)((pf >> (4 * it)) & 15ull));
+#pragma unroll
+    for (int it = 0; it < NS; it++)
+        if ((mb >> it) & 1u) Kck *= pown_i(m_cm[it], (int)((pb >> (4 * it)) & 15ull));
+#pragma unroll
+    for (int n = 0; n < NS; n++) {
+        if (((mf | mb) >> n) & 1u) {
+            const int nf = (int)((pf >> (4 * n)) & 15ull), nb = (int)((pb >> (4 * n)) & 15ull);
+            qi[n] += nb * Kf + nf * Kck;
+            pi[n] += nf * Kf + nb * Kck;
+        }
+    }
+}
+
+// Reaction r of compiled mechanism K: its record as a compile-time constant
+template <int NS, int K, int r>
+__device__ __forceinline__ void rxn_spec(const RxnCtx &c, const double *gk, const double *m_cm, double *qi,
+                                         double *pi)
+{
+    constexpr DevRxn R = MechSpec<K>::rx(r);
+    rxn_step<NS>(R, [](int s) { return MechSpec<K>::eff(r, s); }, c, gk, m_cm, qi, pi);
+}
+template <int NS, int K, int... r>
+__device__ __forceinline__ void rxns_spec(std::integer_sequence<int, r...>, const RxnCtx &c, const double *gk,
+                                          const double *m_cm, double *qi, double *pi)
+{
+    (rxn_spec<NS, K, r>(c, gk, m_cm, qi, pi), ...);
+}
+
+// evalcv (y[0] = T): q, d of length NS + 1.  SPEC >= 0: the reactions of
+// compiled mechanism SPEC, unrolled with their records as constants (the
+// caller has checked that the device tables hold exactly those records);
+// SPEC < 0: the loop over the records in the device tables.
+template <int NS, int SPEC = -1>
 __device__ __forceinline__ void evalcv_dev(const DevThermo *__restrict__ th, const DevMech *__restrict__ mc,
                                            const double *y, double *q, double *d, double m_dens, double m_p)
 {
@@ -74,70 +171,20 @@ __device__ __forceinline__ void evalcv_dev(const DevThermo *__restrict__ th, con
     double *qi = q + 1, *pi = d + 1;
 #pragma unroll
     for (int n = 0; n < NS; n++) qi[n] = 0.0, pi[n] = 0.0;
-    const int NR = mc->NR;
-    // EaR / T for every reaction from one reciprocal: Markstein's corrected
-    // quotient fma(fma(-T, q0, a), y, q0), q0 = a y, y = RN(1/T), is the
-    // correctly rounded a / T while a, T lie in [2^-500, 2^500] (the remainder
-    // is exact and nothing underflows); a = 0 keeps q0, the signed zero of a/T
-    const double rT = 1.0 / m_tmp;
-    const bool t_ok = __all(m_tmp >= 0x1p-500 && m_tmp <= 0x1p500);
-    for (int r = 0; r < NR; r++) {
-        // uniform per-reaction record, read in one burst of scalar loads and
-        // held in scalar registers; species loops run over compile-time n and
-        // test the record's masks, which skips exactly the zero-coefficient
-        // terms (each adds +0 or multiplies by 1 in the reference, so skipping
-        // changes no bit)
-        const DevRxn &R = mc->rx[r];
-        const double A = R.A, B = R.B, EaR = R.EaR;
-        const unsigned md = R.md, mt = R.mt, mf = R.mf, mb = R.mb;
-        const unsigned long long pd = R.pd, pf = R.pf, pb = R.pb;
-        const int tind = R.third_ind;
-        double ET;
-        const double aE = fabs(EaR);
-        if (t_ok && (EaR == 0.0 || (aE >= 0x1p-500 && aE <= 0x1p500))) {
-            const double q0 = EaR * rT;
-            ET = EaR == 0.0 ? q0 : fma(fma(-m_tmp, q0, EaR), rT, q0);
-        } else {
-            ET = EaR / m_tmp;
-        }
-        double Kf = A * dexp(B * logT - ET);
-        double DeltaGibbs = 0.0;
-        int m_dn = 0;
-#pragma unroll
-        for (int n = 0; n < NS; n++) {
-            if ((md >> n) & 1u) {
-                const int nu = (int)((pd >> (4 * n)) & 15ull) - 8;
-                DeltaGibbs += nu * (logP - gk[n]);
-                m_dn += nu;
-            }
-        }
-        double Kck = dmin(dexp(DeltaGibbs - m_dn * logStandConc), 1.0E+40);
-        double tb = 1.0;
-        if (1 == tind) {
-            tb = 0.0;
-            // every efficiency of the row in one burst of scalar loads
-            double eff[NS];
-#pragma unroll
-            for (int it = 0; it < NS; it++) eff[it] = mc->third[r * NS + it];
-#pragma unroll
-            for (int it = 0; it < NS; it++)
-                if ((mt >> it) & 1u) tb += eff[it] * m_cm[it];
-        }
-        Kf *= tb;
-        Kck *= Kf;
-#pragma unroll
-        for (int it = 0; it < NS; it++)
-            if ((mf >> it) & 1u) Kf *= pown_i(m_cm[it], (int)((pf >> (4 * it)) & 15ull));
-#pragma unroll
-        for (int it = 0; it < NS; it++)
-            if ((mb >> it) & 1u) Kck *= pown_i(m_cm[it], (int)((pb >> (4 * it)) & 15ull));
-#pragma unroll
-        for (int n = 0; n < NS; n++) {
-            if (((mf | mb) >> n) & 1u) {
-                const int nf = (int)((pf >> (4 * n)) & 15ull), nb = (int)((pb >> (4 * n)) & 15ull);
-                qi[n] += nb * Kf + nf * Kck;
-                pi[n] += nf * Kf + nb * Kck;
-            }
+    RxnCtx c;
+    c.m_tmp = m_tmp, c.logT = logT, c.logP = logP, c.logStandConc = logStandConc;
+    c.rT = 1.0 / m_tmp;
+    c.t_ok = __all(m_tmp >= 0x1p-500 && m_tmp <= 0x1p500);
+    if constexpr (SPEC >= 0) {
+        static_assert(MechSpec<SPEC>::NS == NS, "mechanism specialisation for another species count");
+        rxns_spec<NS, SPEC>(std::make_integer_sequence<int, MechSpec<SPEC>::NR>{}, c, gk, m_cm, qi, pi);
+    } else {
+        const int NR = mc->NR;
+        for (int r = 0; r < NR; r++) {
+            // the uniform record, read in one burst of scalar loads
+            const DevRxn R = mc->rx[r];
+            const double *er = mc->third + r * NS;
+            rxn_step<NS>(R, [er](int it) { return er[it]; }, c, gk, m_cm, qi, pi);
         }
     }
     double q0 = 0.0, d0 = 0.0;
@@ -183,7 +230,7 @@ template <int NL> struct ChemLds {
     __device__ __forceinline__ double &at(int k, int i) { return p[(k * NL + i) * 64]; }
 };
 
-template <int NS, class St = ChemRegs<NS + 1>> struct ChemLane {
+template <int NS, class St = ChemRegs<NS + 1>, int SPEC = -1> struct ChemLane {
     static constexpr int NL = NS + 1;
     double y[NL], q[NL], d[NL];
     St st;
@@ -299,17 +346,17 @@ template <int NS, class St = ChemRegs<NS + 1>> struct ChemLane {
     __device__ __forceinline__ int advance(const DevThermo *__restrict__ th, const DevMech *__restrict__ mc)
     {
         if (before() < 0) return -1;
-        evalcv_dev<NS>(th, mc, y, q, d, rho, m_p);
+        evalcv_dev<NS, SPEC>(th, mc, y, q, d, rho, m_p);
         return after();
     }
 };
 
 // Chemeq2 on one cell.  Returns predictor passes, or -1 at the safety bound.
-template <int NS>
+template <int NS, int SPEC>
 __device__ int chemeq2_dev(const DevThermo *__restrict__ th, const DevMech *__restrict__ mc, double *y, double dtg,
                            double rho, double m_p)
 {
-    ChemLane<NS> L;
+    ChemLane<NS, ChemRegs<NS + 1>, SPEC> L;
 #pragma unroll
     for (int i = 0; i < NS + 1; i++) L.y[i] = y[i];
     L.begin(dtg, rho, m_p);
@@ -336,7 +383,7 @@ __device__ __forceinline__ double mix_e(const DevThermo *__restrict__ th, const 
 }
 
 // One interior cell of ChemeODEQ2SolverKernel1.
-template <int NS>
+template <int NS, int SPEC>
 __device__ __forceinline__ int chem_cell(const DevThermo *__restrict__ th, const DevMech *__restrict__ mc, Fields &f,
                                          long long N, long long id, double dtc)
 {
@@ -352,7 +399,7 @@ __device__ __forceinline__ int chem_cell(const DevThermo *__restrict__ th, const
         y[NS] += -y[1 + k];
     }
     const double e0 = mix_e<NS>(th, y + 1, y[0]);
-    const int passes = chemeq2_dev<NS>(th, mc, y, dtc, rho, f.p[id]);
+    const int passes = chemeq2_dev<NS, SPEC>(th, mc, y, dtc, rho, f.p[id]);
     const double e1 = mix_e<NS>(th, y + 1, y[0]);
 #pragma unroll
     for (int n = 0; n < NS - 1; n++) f.U[(5 + n) * N + id] = y[1 + n] * rho;
@@ -363,7 +410,7 @@ __device__ __forceinline__ int chem_cell(const DevThermo *__restrict__ th, const
 }
 
 // one lane per interior cell
-template <int NS>
+template <int NS, int SPEC>
 __global__ void __launch_bounds__(256) k_chem(Geom g, const DevThermo *__restrict__ th,
                                               const DevMech *__restrict__ mc, Fields f, int record)
 {
@@ -374,7 +421,7 @@ __global__ void __launch_bounds__(256) k_chem(Geom g, const DevThermo *__restric
     const int j = g.By + (int)((t / g.X_inner) % g.Y_inner);
     const int k = g.Bz + (int)(t / ((long long)g.X_inner * g.Y_inner));
     const long long id = (long long)g.Xmax * g.Ymax * k + (long long)g.Xmax * j + i;
-    const int passes = chem_cell<NS>(th, mc, f, g.N, id, f.dt[1]);
+    const int passes = chem_cell<NS, SPEC>(th, mc, f, g.N, id, f.dt[1]);
     if (record) f.substeps[t] = passes;
     if (passes < 0) {
         atomicOr(&f.err[0], EBIT_CHEM);
@@ -412,13 +459,13 @@ struct ChemCase {
 // per-cell driver.  Termination: the counter only grows, a wave stops once
 // every lane has drawn an index >= total and holds no cell, and a cell is
 // bounded by CHEM_MAXSUB passes.  The counter must be zero at launch.
-template <int NS, class Cases>
+template <int NS, int SPEC, class Cases>
 __device__ __forceinline__ void chem_queue_run(const Cases &cases, long long total, const DevThermo *__restrict__ th,
                                                const DevMech *__restrict__ mc, unsigned long long *queue, int record,
                                                double *lds)
 {
     const int lane = threadIdx.x & 63;
-    ChemLane<NS, ChemLds<NS + 1>> L;
+    ChemLane<NS, ChemLds<NS + 1>, SPEC> L;
     L.st.p = lds + lane;
     ChemCase cc{};
     long long tl = -1, id = 0;
@@ -513,7 +560,7 @@ template <int NS> struct ChemQueueLds {
     static constexpr int doubles = 4 * (NS + 1) * 64;
 };
 
-template <int NS>
+template <int NS, int SPEC>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ChemOccupancy<NS>::waves)))
 k_chem_queue(Geom g, const DevThermo *__restrict__ th, const DevMech *__restrict__ mc, Fields f, int record)
 {
@@ -523,19 +570,19 @@ k_chem_queue(Geom g, const DevThermo *__restrict__ th, const DevMech *__restrict
     one.c.U = f.U, one.c.T = f.T, one.c.p = f.p, one.c.dt = f.dt;
     one.c.substeps = f.substeps, one.c.err = f.err;
     __shared__ double lds[ChemQueueLds<NS>::doubles];
-    chem_queue_run<NS>(one, (long long)g.X_inner * g.Y_inner * g.Z_inner, th, mc, (unsigned long long *)f.queue,
+    chem_queue_run<NS, SPEC>(one, (long long)g.X_inner * g.Y_inner * g.Z_inner, th, mc, (unsigned long long *)f.queue,
                        record, lds);
 }
 
 // one queue over the cells of every case of an ensemble (shared mechanism)
-template <int NS>
+template <int NS, int SPEC>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ChemOccupancy<NS>::waves)))
 k_chem_queue_multi(const ChemCase *__restrict__ cases, int ncases, long long total, const DevThermo *__restrict__ th,
                    const DevMech *__restrict__ mc, unsigned long long *queue, int record)
 {
     ManyCases many{cases, ncases};
     __shared__ double lds[ChemQueueLds<NS>::doubles];
-    chem_queue_run<NS>(many, total, th, mc, queue, record, lds);
+    chem_queue_run<NS, SPEC>(many, total, th, mc, queue, record, lds);
 }
 
 // zero the work-queue counter (a kernel, so the step graph holds no memset)

@@ -29,9 +29,10 @@ struct Ops {
     void (*chem)(xf_ctx *, int record);
     void (*prepare)(xf_ctx *);
     // shared queue over the cells of several cases (ensembles)
+    // spec: compiled mechanism whose records the tables hold, or -1
     void (*chem_multi)(hipStream_t, const ChemCase *, int ncases, long long total, const DevThermo *,
-                       const DevMech *, unsigned long long *queue, int blocks, int record);
-    int (*multi_blocks)(long long total);
+                       const DevMech *, unsigned long long *queue, int blocks, int record, int spec);
+    int (*multi_blocks)(long long total, int spec);
     // viscous terms of GetLU: transport coefficients and centre gradients of
     // the state Ug, then the wall flux of one direction (before its sweep)
     void (*visc_prep)(xf_ctx *, const double *Ug);
@@ -63,6 +64,10 @@ struct xf_ctx {
     Ops ops{};
     bool has_ops = false;
     int chem_blocks = 0; // resident one-wave blocks of the work-queue chemistry kernel
+    // compiled mechanism the device tables match (MechSpec<mech_spec>), or -1;
+    // spec_allowed = false keeps the generic loop (xf_set_chem_specialization)
+    int mech_spec = -1;
+    bool spec_allowed = true;
     int shape[3] = {0, 0, 0}; // sweep tile shape per direction (SweepShape)
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -169,51 +174,85 @@ template <int NS, bool COP> struct Launch {
         if (dir == 1) sweep_d<1>(c, Ug, mode, stage, Ua, Ub, Uo);
         if (dir == 2) sweep_d<2>(c, Ug, mode, stage, Ua, Ub, Uo);
     }
+    // calls f(std::integral_constant<int, K>) for the compiled mechanism K
+    // == spec if one exists for this species count, else with K = -1 (the
+    // generic loop over the device tables)
+    template <int K, class F> static void spec_switch(int spec, F &f)
+    {
+        if constexpr (K < XF_NMECH_SPEC) {
+            if constexpr (COP && MechSpec<K>::NS == NS) {
+                if (spec == K) {
+                    f(std::integral_constant<int, K>{});
+                    return;
+                }
+            }
+            spec_switch<K + 1>(spec, f);
+        } else {
+            f(std::integral_constant<int, -1>{});
+        }
+    }
     static void chem(xf_ctx *c, int record)
     {
         if constexpr (COP) {
             ProfScope ps(c, "chem");
-            if (c->d.chem_driver == XF_CHEM_QUEUE) {
-                hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, c->stream, c->f.queue);
-                hipLaunchKernelGGL((k_chem_queue<NS>), dim3(c->chem_blocks), dim3(64), 0, c->stream, c->g, c->d_th,
-                                   c->d_mc, c->f, record);
-            } else {
-                hipLaunchKernelGGL((k_chem<NS>), dim3(nblk(c->Ninner)), dim3(256), 0, c->stream, c->g, c->d_th,
-                                   c->d_mc, c->f, record);
-            }
+            auto launch = [&](auto K) {
+                constexpr int S = decltype(K)::value;
+                if (c->d.chem_driver == XF_CHEM_QUEUE) {
+                    hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, c->stream, c->f.queue);
+                    hipLaunchKernelGGL((k_chem_queue<NS, S>), dim3(c->chem_blocks), dim3(64), 0, c->stream, c->g,
+                                       c->d_th, c->d_mc, c->f, record);
+                } else {
+                    hipLaunchKernelGGL((k_chem<NS, S>), dim3(nblk(c->Ninner)), dim3(256), 0, c->stream, c->g,
+                                       c->d_th, c->d_mc, c->f, record);
+                }
+            };
+            spec_switch<0>(c->mech_spec, launch);
         }
     }
     // persistent lanes of the queue driver: as many one-wave blocks as stay
-    // resident, never more than the cells need (queried once, outside any
-    // stream capture)
+    // resident, never more than the cells need (queried outside any stream
+    // capture, again once the mechanism has picked the kernel)
     static void prepare(xf_ctx *c)
     {
         if constexpr (COP) {
             int per_cu = 0, ncu = 0, dev = 0;
             (void)hipGetDevice(&dev);
             (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_chem_queue<NS>, 64, 0);
+            auto query = [&](auto K) {
+                constexpr int S = decltype(K)::value;
+                (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_chem_queue<NS, S>, 64, 0);
+            };
+            spec_switch<0>(c->mech_spec, query);
             const long long want = (c->Ninner + 63) / 64;
             const long long nb = (long long)std::max(per_cu, 1) * std::max(ncu, 1);
             c->chem_blocks = (int)std::max(1LL, std::min(nb, want));
         }
     }
     static void chem_multi(hipStream_t st, const ChemCase *cases, int ncases, long long total, const DevThermo *th,
-                           const DevMech *mc, unsigned long long *queue, int blocks, int record)
+                           const DevMech *mc, unsigned long long *queue, int blocks, int record, int spec)
     {
         if constexpr (COP) {
-            hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, st, (unsigned int *)queue);
-            hipLaunchKernelGGL((k_chem_queue_multi<NS>), dim3(blocks), dim3(64), 0, st, cases, ncases, total, th, mc,
-                               queue, record);
+            auto launch = [&](auto K) {
+                constexpr int S = decltype(K)::value;
+                hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, st, (unsigned int *)queue);
+                hipLaunchKernelGGL((k_chem_queue_multi<NS, S>), dim3(blocks), dim3(64), 0, st, cases, ncases, total,
+                                   th, mc, queue, record);
+            };
+            spec_switch<0>(spec, launch);
         }
     }
-    static int multi_blocks(long long total)
+    static int multi_blocks(long long total, int spec)
     {
         int per_cu = 0, ncu = 0, dev = 0;
         if constexpr (COP) {
             (void)hipGetDevice(&dev);
             (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_chem_queue_multi<NS>, 64, 0);
+            auto query = [&](auto K) {
+                constexpr int S = decltype(K)::value;
+                (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_chem_queue_multi<NS, S>,
+                                                                   64, 0);
+            };
+            spec_switch<0>(spec, query);
         }
         const long long want = (total + 63) / 64;
         const long long nb = (long long)std::max(per_cu, 1) * std::max(ncu, 1);

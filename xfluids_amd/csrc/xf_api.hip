@@ -8,6 +8,7 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <memory>
 #include <vector>
 
 using namespace xf;
@@ -421,9 +422,10 @@ int xf_mech_specialization_index(int NS, int NR, const int *Nu_f, const int *Nu_
                                  const double *Rargus)
 {
     if (NS < 1 || NS > MAXNS || NR < 1 || NR > MAXNR || !Nu_f || !Nu_b || !third || !Rargus) return -1;
-    static DevMech m; // host-only scratch (the record build does not touch the GPU)
-    if (build_mech(m, NS, NR, Nu_f, Nu_b, third, Rargus) != XF_OK) return -1;
-    return mech_spec_find(NS, NR, m.rx, m.third);
+    // host memory only: the record build does not touch the GPU
+    std::unique_ptr<DevMech> m(new DevMech());
+    if (build_mech(*m, NS, NR, Nu_f, Nu_b, third, Rargus) != XF_OK) return -1;
+    return mech_spec_find(NS, NR, m->rx, m->third);
 }
 
 int xf_set_chem_specialization(xf_ctx *c, int on)

@@ -41,6 +41,15 @@ ALG_BYTES_PER_CELL = 8 * (8 * 7 + 6)
 # reference performs whose result is unused (terms multiplied by zero,
 # projections of the stencil points i-3 and i+4); the HIP sweep skips it.
 FACE_FLOPS = 4121
+# The same counting run, per unit of the other stages of one step: UpdateStates
+# per padded cell (data dependent: Newton iterations on the seeded field), the
+# flux divergence per padded and per interior cell, the RK3 update per
+# interior cell, GetDt per padded cell.
+STATES_FLOPS = 246.485
+LU_PADDED_FLOPS = 4.0
+LU_INNER_FLOPS = 42.0
+RK3_INNER_FLOPS = 14.0
+DT_PADDED_FLOPS = 8.0026
 PROFILES = os.path.join(ROOT, "profiles")
 TRAFFIC_FILE = os.path.join(PROFILES, "pmc-traffic-shock-bubble-1024x512.md")
 
@@ -115,6 +124,82 @@ def roofline(d, kt, shapes=None):
     return {"kernel": name, "bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
             "frac": achieved / PEAK_HBM_GBS, "traffic": traffic, "avg_launch_ms": avg_ms,
             "timed_launches": cnt}
+
+
+def step_roofline(d, value, fp64_tflops=None):
+    """Whole-step fractions: counted FP64 operations and compulsory HBM bytes
+    per cell-update of the C2 step (dt + three stages), times the measured
+    cell-updates/s, against the datasheet peaks (and the measured fma rate)."""
+    cells = d.X_inner * d.Y_inner * d.Z_inner
+    padded = d.Xmax * d.Ymax * d.Zmax
+    faces = ((d.X_inner + 1) * d.Y_inner * d.Z_inner if d.DimX else 0) + \
+            (d.X_inner * (d.Y_inner + 1) * d.Z_inner if d.DimY else 0) + \
+            (d.X_inner * d.Y_inner * (d.Z_inner + 1) if d.DimZ else 0)
+    stage = faces * FACE_FLOPS + padded * (STATES_FLOPS + LU_PADDED_FLOPS) + cells * (LU_INNER_FLOPS + RK3_INNER_FLOPS)
+    flops_cu = (3 * stage + padded * DT_PADDED_FLOPS) / cells
+    tflops = flops_cu * value * 1e6 / 1e12
+    gbs = ALG_BYTES_PER_CELL * value * 1e6 / 1e9
+    out = {"flops_per_cell_update": flops_cu, "bytes_per_cell_update": ALG_BYTES_PER_CELL,
+           "achieved_tflops": tflops, "frac_fp64_peak": tflops / PEAK_FP64_TFLOPS,
+           "achieved_gbs": gbs, "frac_hbm_peak": gbs / PEAK_HBM_GBS,
+           "intensity_flop_per_byte": flops_cu / ALG_BYTES_PER_CELL, "bound": "fp64"}
+    if fp64_tflops:
+        out["frac_measured_fma_rate"] = tflops / fp64_tflops
+    return out
+
+
+def vortex_leg(steps=200, n=256):
+    """The isentropic vortex (single species, WENO5-JS, periodic, CFL 0.5):
+    throughput of both forms, and after `steps` steps the L1 error of rho
+    against the exact solution (the initial vortex advected by (t, t) modulo
+    the 10 x 10 domain) and the drift of the interior sums of rho, rho u,
+    rho v and E (conserved to round-off on a periodic grid)."""
+    case = "isentropic-vortex"
+    s = xf.Setup(xf.case_path(case), nx=n, ny=n)
+    d = s.desc
+    U, T = s.initial(case)
+    E = d.E
+    shp = (d.Zmax, d.Ymax, d.Xmax, E)
+    sl = (0, slice(d.Bwidth_Y, d.Bwidth_Y + d.Y_inner), slice(d.Bwidth_X, d.Bwidth_X + d.X_inner))
+    inner = lambda A: np.asarray(A).reshape(shp)[sl]
+    sums0 = inner(U).sum(axis=(0, 1))
+    out = {"workload": "2D isentropic vortex, single species, WENO5-JS, SSP-RK3, CFL 0.5, periodic",
+           "grid": "%dx%d" % (n, n), "steps": steps, "forms": {}}
+    for form in ("plain", "mixed"):
+        prec = xf.XF_MIXED if form == "mixed" else xf.XF_PLAIN_FP64
+        ctx = s.context(precision=prec)
+        ctx.upload(U, T)
+        ctx.initialize()
+        ctx.steps(5, graph=True)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.steps_async(steps, graph=True)
+        ctx.synchronize()
+        t1 = time.perf_counter()
+        ctx.close()
+        # accuracy run from the initial state, recording dt
+        ctx = s.context(precision=prec)
+        ctx.upload(U, T)
+        ctx.initialize()
+        dts = ctx.steps(steps, graph=True, record_dt=True)
+        Ug, _ = ctx.download(0)
+        ctx.close()
+        t = float(np.sum(dts))
+        dx = 10.0 / n
+        x = (np.arange(n) + 0.5) * dx
+        X, Y = np.meshgrid(x, x)
+        xv = np.mod(X - t, 10.0) - 5.0
+        yv = np.mod(Y - t, 10.0) - 5.0
+        rho_ex = (1.0 - 25.0 * np.exp(1.0 - (xv ** 2 + yv ** 2)) / (28.0 * np.pi ** 2)) ** 2.5
+        Ui = inner(Ug)
+        sums1 = Ui.sum(axis=(0, 1))
+        scale = np.abs(sums0).max()
+        out["forms"][form] = {
+            "value": n * n * steps / (t1 - t0) / 1e6, "ms_per_step": (t1 - t0) * 1e3 / steps,
+            "time": t, "l1_abs_rho_vs_exact": float(np.abs(Ui[:, :, 0] - rho_ex).mean()),
+            "conservation_drift": {k: float(abs(sums1[q] - sums0[q]) / scale)
+                                   for k, q in (("rho", 0), ("rho_u", 1), ("rho_v", 2), ("E", 4))}}
+    return out
 
 
 def copy_probe(nbytes=1 << 30, reps=5):
@@ -377,9 +462,23 @@ def cpu_baseline(sample_steps):
     dt = time.perf_counter() - t0
     cells = d.X_inner * d.Y_inner
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": cells * sample_steps / dt / 1e6, "unit": "Mcell-updates/s", "cores": threads,
-            "kind": "port", "sample": "%d steps of the %dx%d workload after 1 warm-up step (%.1f s)" %
-            (sample_steps, NX, NY, dt)}
+    out = {"value": cells * sample_steps / dt / 1e6, "unit": "Mcell-updates/s", "cores": threads,
+           "kind": "port", "sample": "%d steps of the %dx%d workload after 1 warm-up step (%.1f s)" %
+           (sample_steps, NX, NY, dt)}
+    # the same oracle on one thread (its OpenMP runtime is this process's)
+    try:
+        import ctypes
+        gomp = ctypes.CDLL("libgomp.so.1")
+        gomp.omp_set_num_threads(1)
+        t0 = time.perf_counter()
+        o.steps(2)
+        dt1 = time.perf_counter() - t0
+        gomp.omp_set_num_threads(threads)
+        out["single_thread"] = {"value": cells * 2 / dt1 / 1e6, "cores": 1,
+                                "sample": "2 further steps (%.1f s)" % dt1}
+    except OSError:
+        pass
+    return out
 
 
 def progress(msg):
@@ -405,7 +504,8 @@ def main():
     ap.add_argument("--reactive-only", type=int, default=-1, metavar="I",
                     help="profiling aid: run only reactive workload I and print its JSON")
     ap.add_argument("--copy-probe", action="store_true",
-                    help="also run a 1 GiB device copy (counter scaling under rocprofv3 --pmc)")
+                    help="accepted for older command lines: the 1 GiB copy and the fma probe always run")
+    ap.add_argument("--no-vortex", action="store_true", help="skip the isentropic-vortex leg")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
@@ -460,8 +560,16 @@ def main():
     if args.ensemble_cases > 0:
         sides = [int(v) for v in args.ensemble_sides.split(",")]
         line["ensemble"] = ensemble_leg(args.ensemble_cases, sides, args.ensemble_steps, 1, log=progress)
-    if args.copy_probe:
-        line["copy_probe"] = copy_probe()
+    # measured ceilings next to the datasheet peaks: streaming copy (HBM) and
+    # independent FP64 fma chains (VALU)
+    line["copy_probe"] = copy_probe()
+    fp64 = xf.fma_probe(10)
+    line["fp64_probe"] = {"tflops": fp64, "peak_datasheet": PEAK_FP64_TFLOPS}
+    line["roofline"]["frac_measured_fma_rate"] = line["roofline"]["achieved"] / fp64
+    line["step_roofline"] = step_roofline(d, value, fp64)
+    if not args.no_vortex:
+        line["vortex"] = vortex_leg()
+        progress("vortex leg done")
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.cpu_steps)
     print(json.dumps(line))

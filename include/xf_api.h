@@ -251,6 +251,11 @@ int xf_debug_math(const double *x, double *exp_out, double *log_out, int n);
    ms per launch.  A plain copy of known size: the HBM bandwidth reference
    and the scale for the memory-traffic counters.  No reference counterpart. */
 int xf_copy_probe(size_t nbytes, int reps, double *avg_ms);
+/* FP64 fused multiply-add throughput: eight independent fma chains per lane
+   on 2048 blocks of 256 lanes, reps launches after one warm-up on the null
+   stream; TFLOP/s with two flops per fma.  The measured counterpart of the
+   datasheet FP64 peak the roofline quotes.  No reference counterpart. */
+int xf_fma_probe(int reps, double *tflops);
 
 #ifdef __cplusplus
 }

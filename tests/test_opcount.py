@@ -35,3 +35,22 @@ def test_bench_face_flops_is_the_counted_figure():
     r = count_flops.measure("shock-bubble")
     assert r["solve_max_residual"] < 1e-6
     assert r["face_x_flops"] == r["face_y_flops"] == bench.FACE_FLOPS
+
+
+def test_bench_step_constants_are_the_counted_figures():
+    """The per-unit counts bench.py's whole-step roofline uses: exact for
+    the fixed-cost stages, to 1 % for UpdateStates and GetDt (their counts
+    depend on the Newton iterations and maxima of the seeded field)."""
+    sys.path.insert(0, ROOT)
+    import bench
+    r = count_flops.measure("shock-bubble")
+    assert r["lu_per_padded_cell_flops"] == bench.LU_PADDED_FLOPS
+    assert r["lu_per_inner_cell_flops"] == bench.LU_INNER_FLOPS
+    assert r["rk3_per_inner_cell_flops"] == bench.RK3_INNER_FLOPS
+    assert abs(r["states_per_padded_cell_flops"] / bench.STATES_FLOPS - 1) < 0.01
+    assert abs(r["dt_per_padded_cell_flops"] / bench.DT_PADDED_FLOPS - 1) < 0.01
+    s = xf.Setup(xf.case_path(bench.CASE), nx=bench.NX, ny=bench.NY)
+    sr = bench.step_roofline(s.desc, 800.0, 70.0)
+    # about 25.7 k operations per cell-update: 3 stages of ~2 faces per cell
+    assert 25000 < sr["flops_per_cell_update"] < 26500
+    assert abs(sr["achieved_tflops"] - sr["flops_per_cell_update"] * 800e6 / 1e12) < 1e-9

@@ -138,6 +138,7 @@ _SIGS = {
     "xf_load_checkpoint": (C.c_int, [_V, C.c_char_p, _IP, _DP, C.POINTER(C.c_float)]),
     "xf_debug_math": (C.c_int, [_DP, _DP, _DP, C.c_int]),
     "xf_copy_probe": (C.c_int, [C.c_size_t, C.c_int, _DP]),
+    "xf_fma_probe": (C.c_int, [C.c_int, _DP]),
     "xf_set_tile_shape": (C.c_int, [_V, C.c_int, C.c_int]),
     "xf_get_tile_shape": (C.c_int, [_V, _IP]),
     "xf_autotune": (C.c_int, [_V, C.c_int, _IP, _DP]),
@@ -487,6 +488,15 @@ def copy_probe(nbytes=1 << 30, reps=5):
     if st != 0:
         raise XfError(st, "xf_copy_probe")
     return ms.value
+
+
+def fma_probe(reps=10):
+    """Achieved FP64 fused multiply-add throughput in TFLOP/s (xf_fma_probe)."""
+    tf = C.c_double(0.0)
+    st = lib().xf_fma_probe(reps, C.byref(tf))
+    if st != 0:
+        raise XfError(st, "xf_fma_probe")
+    return tf.value
 
 
 class Ensemble:

@@ -52,6 +52,24 @@ __global__ void __launch_bounds__(256) k_copy(const double2 *__restrict__ src, d
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) dst[t] = src[t];
 }
 
+// FP64 issue-rate probe: 8 independent fused multiply-add chains per lane,
+// iters x 8 fmas each, one result per lane written so nothing is dropped
+__global__ void __launch_bounds__(256) k_fma_probe(double *out, int iters, double a, double b)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    double x[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) x[q] = 1e-3 * (double)(t + q);
+    for (int i = 0; i < iters; i++) {
+#pragma unroll
+        for (int q = 0; q < 8; q++) x[q] = fma(x[q], a, b);
+    }
+    double sum = 0.0;
+#pragma unroll
+    for (int q = 0; q < 8; q++) sum += x[q];
+    out[t] = sum;
+}
+
 __global__ void k_set_double(double *p, double v)
 {
     if (threadIdx.x == 0) p[0] = v;
@@ -1078,6 +1096,37 @@ void xf_ensemble_destroy(xf_ensemble *e)
     if (e->d_queue) (void)hipFree(e->d_queue);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
+}
+
+int xf_fma_probe(int reps, double *tflops)
+{
+    if (reps <= 0 || !tflops) return XF_EINVAL;
+    // 8 waves on each SIMD of 256 CUs; every lane writes its own out[t]
+    const unsigned blocks = 256 * 8, threads = 256;
+    const int iters = 4096;
+    double *out = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMalloc(&out, (size_t)blocks * threads * sizeof(double));
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_fma_probe, dim3(blocks), dim3(threads), 0, 0, out, iters, 0.999999, 1e-6); // warm-up
+        e = hipEventRecord(e0, 0);
+    }
+    for (int r = 0; e == hipSuccess && r < reps; r++) {
+        hipLaunchKernelGGL(k_fma_probe, dim3(blocks), dim3(threads), 0, 0, out, iters, 0.999999, 1e-6);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(e1, 0);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    // two flops per fma
+    if (e == hipSuccess) *tflops = 2.0 * 8.0 * iters * (double)blocks * threads * reps / (ms * 1e-3) / 1e12;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (out) (void)hipFree(out);
+    return e == hipSuccess ? XF_OK : XF_EHIP;
 }
 
 int xf_copy_probe(size_t nbytes, int reps, double *avg_ms)

@@ -297,6 +297,23 @@ def reactive_leg(label, case, over, steps, warmup=2, seed=1):
     base = out["forms"]["plain/per-cell"]["value"]
     for k, v in out["forms"].items():
         v["speedup_vs_plain_per_cell"] = v["value"] / base
+    # the fluid part alone: the same case and steps with the Reaction stage
+    # switched off (equations.Sources_React = 0), both precision forms
+    out["fluid_only"] = {}
+    si = xf.Setup(xf.case_path(case), reaction=False, **over)
+    for prec in ("plain", "mixed"):
+        ctx = si.context(precision=xf.XF_MIXED if prec == "mixed" else xf.XF_PLAIN_FP64)
+        ctx.upload(U, T)
+        ctx.initialize()
+        ctx.autotune(reps=5)
+        ctx.steps(warmup, graph=True)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.steps_async(steps, graph=True)
+        ctx.synchronize()
+        t1 = time.perf_counter()
+        ctx.close()
+        out["fluid_only"][prec] = {"value": cells * steps / (t1 - t0) / 1e6, "ms_per_step": (t1 - t0) * 1e3 / steps}
     return out
 
 

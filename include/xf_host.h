@@ -27,6 +27,10 @@ int xf_setup_override(xf_setup *s, int nx, int ny, int nz, double cfl, int schem
 /* Switch the positivity-preserving flux limiter on (1) or off (0), as the
    settings key equations.PositivityPreserving does (src/read_ini/settings/read_json.cpp:68). */
 int xf_setup_set_positivity(xf_setup *s, int on);
+/* Switch the Reaction stage on (1, needs a mechanism) or off (0), as the
+   settings key equations.Sources_React does (Sources_React, src/XFLUIDS.cpp:211-225):
+   off gives the inert (fluid-only) step of a reacting case. */
+int xf_setup_set_reaction(xf_setup *s, int on);
 /* Switch the viscous terms (Visc, Visc_Heat, Visc_Diffu of CMakeLists.txt:55-57,
    settings keys equations.Visc / Visc_Heat / Visc_Diffu here); fits the
    transport properties (Setup::GetFitCoefficient, viscfit.cpp:148-193) when

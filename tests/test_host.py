@@ -218,3 +218,13 @@ def test_shipped_mechanisms_have_compiled_rate_evaluations():
         assert L.xf_mech_specialization_index(d.NS - 1, d.NR, i32(Nu_f), i32(Nu_b), f64(third), f64(Rargus)) == -1
     # the two shipped mechanisms (with argon and with nitrogen) are distinct
     assert len(found) == 2
+
+
+def test_reaction_switch():
+    """equations.Sources_React as a setup switch: off gives the inert step of
+    a reacting case; on needs a mechanism."""
+    s = xf.Setup(xf.case_path("reactive-channel"), reaction=False)
+    assert s.desc.react == 0 and s.desc.NR == 19
+    assert xf.Setup(xf.case_path("reactive-channel"), reaction=True).desc.react == 1
+    with pytest.raises(xf.XfError):
+        xf.Setup(xf.case_path("shock-bubble"), reaction=True)

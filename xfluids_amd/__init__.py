@@ -120,6 +120,7 @@ _SIGS = {
     "xf_setup_free": (None, [_V]),
     "xf_setup_override": (C.c_int, [_V, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int]),
     "xf_setup_set_positivity": (C.c_int, [_V, C.c_int]),
+    "xf_setup_set_reaction": (C.c_int, [_V, C.c_int]),
     "xf_setup_set_viscous": (C.c_int, [_V, C.c_int, C.c_int, C.c_int]),
     "xf_setup_transport": (C.c_int, [_V, _DP, _DP, _DP]),
     "xf_setup_transport_exact": (C.c_double, [_V, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double]),
@@ -189,7 +190,7 @@ class Setup:
     """Case setup from a settings JSON (reference schema) + runtime.dat."""
 
     def __init__(self, settings, runtime_root=RUNTIME_DAT, nx=0, ny=0, nz=0, cfl=-1.0, scheme=-1, thermo=-1,
-                 positivity=-1, viscous=None):
+                 positivity=-1, viscous=None, reaction=None):
         L = lib()
         h = C.c_void_p()
         err = C.create_string_buffer(512)
@@ -205,6 +206,11 @@ class Setup:
             st = L.xf_setup_set_positivity(h, int(positivity))
             if st:
                 raise XfError(st, "xf_setup_set_positivity")
+        if reaction is not None:
+            # equations.Sources_React: False gives the inert step of a reacting case
+            st = L.xf_setup_set_reaction(h, 1 if reaction else 0)
+            if st:
+                raise XfError(st, "xf_setup_set_reaction")
         if viscous is not None:
             # (visc, heat, diffu) switches, as equations.Visc / Visc_Heat / Visc_Diffu
             v, h_, df = (list(viscous) + [0, 0, 0])[:3]

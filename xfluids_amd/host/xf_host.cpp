@@ -111,6 +111,14 @@ int xf_setup_set_positivity(xf_setup *h, int on)
     return XF_OK;
 }
 
+int xf_setup_set_reaction(xf_setup *h, int on)
+{
+    if (!h || (on != 0 && on != 1)) return XF_EINVAL;
+    if (on && (h->s.NR <= 0 || !h->s.desc.cop)) return XF_EINVAL;
+    h->s.desc.react = on;
+    return XF_OK;
+}
+
 int xf_setup_set_viscous(xf_setup *h, int visc, int heat, int diffu)
 {
     if (!h || visc < 0 || visc > 1 || heat < 0 || heat > 1 || diffu < 0 || diffu > 1) return XF_EINVAL;

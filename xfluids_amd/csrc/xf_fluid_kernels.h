@@ -31,13 +31,13 @@ __device__ __forceinline__ void cell_ijk(const Geom &g, long long id, int &i, in
 // ---------------------------------------------------------------------------
 // wave64 max reduction (butterfly over DPP/permute) of non-negative doubles
 // ---------------------------------------------------------------------------
+// maximum over the wave of magnitudes (>= +0, or NaN): fmax is one
+// instruction per step and drops a NaN, as the fold of the block maxima does
+// (it keeps only values > 0)
 __device__ __forceinline__ double wave_max(double v)
 {
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        const double o = __shfl_xor(v, off, 64);
-        v = (o > v) ? o : v;
-    }
+    for (int off = 32; off >= 1; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
     return v;
 }
 
@@ -325,10 +325,15 @@ __global__ void __launch_bounds__(256) k_states(Geom g, const DevThermo *__restr
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     if (threadIdx.x == 0) serr = 0;
     __syncthreads();
+    // the directions the case has (the others stay +0)
+    const int dimv[3] = {g.DimX, g.DimY, g.DimZ};
 #pragma unroll
-    for (int q = 0; q < 9; q++) glf[q] = wave_max(glf[q]);
+    for (int d = 0; d < 3; d++) {
+        if (!dimv[d]) continue;
 #pragma unroll
-    for (int q = 0; q < 3; q++) dtm[q] = wave_max(dtm[q]);
+        for (int q = 0; q < 3; q++) glf[3 * d + q] = wave_max(glf[3 * d + q]);
+        dtm[d] = wave_max(dtm[d]);
+    }
     if (lane == 0) {
 #pragma unroll
         for (int q = 0; q < 9; q++) sred[wv][q] = glf[q];

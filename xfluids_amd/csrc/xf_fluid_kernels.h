@@ -29,11 +29,11 @@ __device__ __forceinline__ void cell_ijk(const Geom &g, long long id, int &i, in
 }
 
 // ---------------------------------------------------------------------------
-// wave64 max reduction (butterfly over DPP/permute) of non-negative doubles
+// wave64 max reduction (butterfly over lane permutes) of non-negative doubles
 // ---------------------------------------------------------------------------
-// maximum over the wave of magnitudes (>= +0, or NaN): fmax is one
-// instruction per step and drops a NaN, as the fold of the block maxima does
-// (it keeps only values > 0)
+// maximum over the wave of magnitudes (>= +0, or NaN): fmax takes two
+// instructions per step where a compare and two selects took three, and it
+// drops a NaN, as the fold of the block maxima does (it keeps only values > 0)
 __device__ __forceinline__ double wave_max(double v)
 {
 #pragma unroll

@@ -94,6 +94,9 @@ def field_errors(s, Uo, To, Ug, Tg):
     if Yo is not None:
         out["l1_abs_Y"] = float(np.abs(Yg - Yo).mean())
         out["linf_abs_Y"] = float(np.abs(Yg - Yo).max())
+        # per species k, in the case's species order
+        out["l1_abs_Yk"] = [float(v) for v in np.abs(Yg - Yo).mean(axis=0)]
+        out["linf_abs_Yk"] = [float(v) for v in np.abs(Yg - Yo).max(axis=0)]
         out["Y_range_oracle"] = [float(Yo.min()), float(Yo.max())]
         out["Y_range_gpu"] = [float(Yg.min()), float(Yg.max())]
     return out

@@ -370,32 +370,13 @@ __device__ __forceinline__ void cop_c2(const DevThermo *__restrict__ th, RoeFace
 
 namespace xf {
 
+
 // GLF dissipation of characteristic field n (Artificial_type 3): the running
 // maximum of |eigen_local[n]|; fields 1..E-2 share the |u_n| maximum.
 // GetLU's GLF reduction happens here: the running maximum is folded with
 // the maximum of the last UpdateStates pass.  Every block computes the same
 // max(running, pass) and block 0 stores it back; readers before or after the
 // store see the same value (max is idempotent).
-// XCD-aware tile order.  Workgroups are dispatched round-robin over the 8
-// XCDs (linear id b runs on XCD b % 8), and each XCD has its own L2.  The
-// tile a workgroup works on is chosen so that XCD x owns one contiguous run
-// of tiles in row-major order: neighbouring tiles, which share halo cells and
-// the cache lines that straddle tile edges, then meet in the same L2.  The
-// map is a bijection of [0, nb) for any nb.
-constexpr int kXcds = 8;
-__device__ __forceinline__ void xcd_tile(int &tx, int &ty, int &tz)
-{
-    const int gx = (int)gridDim.x, gy = (int)gridDim.y;
-    const int nb = gx * gy * (int)gridDim.z;
-    const int b = (int)blockIdx.x + gx * ((int)blockIdx.y + gy * (int)blockIdx.z);
-    const int q = nb / kXcds, r = nb % kXcds;
-    const int x = b % kXcds, slot = b / kXcds;
-    const int l = (x < r) ? x * (q + 1) + slot : r * (q + 1) + (x - r) * q + slot;
-    tx = l % gx;
-    ty = (l / gx) % gy;
-    tz = l / (gx * gy);
-}
-
 template <int DIR, int E>
 __device__ __forceinline__ double glf_alpha(const double *red, int n)
 {

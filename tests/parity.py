@@ -15,6 +15,7 @@ CASES = {
     "shock_bubble_cu6": ("shock-bubble", "shock-bubble", dict(nx=96, ny=48, scheme=2), 6),
     "vortex_cu6": ("isentropic-vortex", "isentropic-vortex", dict(nx=48, ny=48, scheme=2), 8),
     "shock_bubble_pp": ("shock-bubble", "shock-bubble", dict(nx=96, ny=48, positivity=1), 6),
+    "shock_bubble_ns2": ("shock-bubble-h2n2", "shock-bubble", dict(nx=96, ny=48), 6),
     "reactive_shock_tube": ("reactive-shock-tube", "reactive-shock-tube", dict(nx=512), 8),
     "reactive_shock_tube_pp": ("reactive-shock-tube", "reactive-shock-tube", dict(nx=512, positivity=1), 8),
     "reactive_channel": ("reactive-channel", "reactive-channel", dict(nx=24, ny=24), 3),

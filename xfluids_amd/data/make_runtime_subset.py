@@ -79,6 +79,9 @@ def main():
     write_mixture("NO-COP", ["NCOP"], [1], [1])
     # air outside (O2:N2 = 21:79 molar), pure H2 inside the bubble; N2 last
     write_mixture("H2-O2-N2", ["H2", "O2", "N2"], [0, 21, 79], [1, 0, 0])
+    # two-species variant of the same bubble (N2 outside, H2 inside): the
+    # NS = 2 kernel instances
+    write_mixture("H2-N2", ["H2", "N2"], [0, 1], [1, 0])
     copy_mech(src, "H2O_19_reaction")
     copy_mech(src, "H2O-N2_19_reaction")
     # Lennard-Jones parameters and collision integrals for the transport fits

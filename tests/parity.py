@@ -16,6 +16,7 @@ CASES = {
     "vortex_cu6": ("isentropic-vortex", "isentropic-vortex", dict(nx=48, ny=48, scheme=2), 8),
     "shock_bubble_pp": ("shock-bubble", "shock-bubble", dict(nx=96, ny=48, positivity=1), 6),
     "shock_bubble_ns2": ("shock-bubble-h2n2", "shock-bubble", dict(nx=96, ny=48), 6),
+    "shock_bubble_3d": ("shock-bubble-3d", "shock-bubble", dict(nx=32, ny=16, nz=16), 4),
     "reactive_shock_tube": ("reactive-shock-tube", "reactive-shock-tube", dict(nx=512), 8),
     "reactive_shock_tube_pp": ("reactive-shock-tube", "reactive-shock-tube", dict(nx=512, positivity=1), 8),
     "reactive_channel": ("reactive-channel", "reactive-channel", dict(nx=24, ny=24), 3),

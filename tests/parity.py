@@ -14,6 +14,9 @@ CASES = {
     "shock_bubble_js_nasa": ("shock-bubble", "shock-bubble", dict(nx=96, ny=48, scheme=0, thermo=1), 6),
     "shock_bubble_cu6": ("shock-bubble", "shock-bubble", dict(nx=96, ny=48, scheme=2), 6),
     "vortex_cu6": ("isentropic-vortex", "isentropic-vortex", dict(nx=48, ny=48, scheme=2), 8),
+    # the ghost fill of the wall and symmetry codes: x viscous wall / slip wall,
+    # y symmetry / no-slip wall
+    "vortex_walls": ("isentropic-vortex-walls", "isentropic-vortex", dict(nx=48, ny=48), 8),
     "shock_bubble_pp": ("shock-bubble", "shock-bubble", dict(nx=96, ny=48, positivity=1), 6),
     "shock_bubble_ns2": ("shock-bubble-h2n2", "shock-bubble", dict(nx=96, ny=48), 6),
     "shock_bubble_3d": ("shock-bubble-3d", "shock-bubble", dict(nx=32, ny=16, nz=16), 4),

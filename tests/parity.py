@@ -22,6 +22,8 @@ CASES = {
     "shock_bubble_3d": ("shock-bubble-3d", "shock-bubble", dict(nx=32, ny=16, nz=16), 4),
     "reactive_shock_tube": ("reactive-shock-tube", "reactive-shock-tube", dict(nx=512), 8),
     "reactive_shock_tube_pp": ("reactive-shock-tube", "reactive-shock-tube", dict(nx=512, positivity=1), 8),
+    # Lie splitting: one full-dt Reaction after the RK stages (XFLUIDS.cpp:211-225)
+    "reactive_shock_tube_lie": ("reactive-shock-tube-lie", "reactive-shock-tube", dict(nx=512), 8),
     "reactive_channel": ("reactive-channel", "reactive-channel", dict(nx=24, ny=24), 3),
     "mixing_layer_3d": ("mixing-layer", "mixing-layer", dict(nx=16, ny=16, nz=16), 2),
     "shock_bubble_visc": ("shock-bubble", "shock-bubble", dict(nx=96, ny=48, viscous=(1, 1, 1)), 6),

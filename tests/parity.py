@@ -25,6 +25,8 @@ CASES = {
     # Lie splitting: one full-dt Reaction after the RK stages (XFLUIDS.cpp:211-225)
     "reactive_shock_tube_lie": ("reactive-shock-tube-lie", "reactive-shock-tube", dict(nx=512), 8),
     "reactive_channel": ("reactive-channel", "reactive-channel", dict(nx=24, ny=24), 3),
+    # NASA-7 (JANAF) fits in the chemistry's Gibbs energies and the states
+    "reactive_channel_janaf": ("reactive-channel", "reactive-channel", dict(nx=24, ny=24, thermo=0), 3),
     "mixing_layer_3d": ("mixing-layer", "mixing-layer", dict(nx=16, ny=16, nz=16), 2),
     "shock_bubble_visc": ("shock-bubble", "shock-bubble", dict(nx=96, ny=48, viscous=(1, 1, 1)), 6),
     "mixing_layer_3d_inviscid": ("mixing-layer", "mixing-layer", dict(nx=16, ny=16, nz=16, viscous=(0, 0, 0)), 2),

@@ -30,6 +30,12 @@ CASES = {
     "mixing_layer_3d": ("mixing-layer", "mixing-layer", dict(nx=16, ny=16, nz=16), 2),
     "shock_bubble_visc": ("shock-bubble", "shock-bubble", dict(nx=96, ny=48, viscous=(1, 1, 1)), 6),
     "mixing_layer_3d_inviscid": ("mixing-layer", "mixing-layer", dict(nx=16, ny=16, nz=16, viscous=(0, 0, 0)), 2),
+    # combinations: WENO-CU6 in 3-D (three and nine species, the latter viscous
+    # and reacting), and CU6 with the limiter and the viscous terms together
+    "shock_bubble_3d_cu6": ("shock-bubble-3d", "shock-bubble", dict(nx=32, ny=16, nz=16, scheme=2), 4),
+    "mixing_layer_3d_cu6": ("mixing-layer", "mixing-layer", dict(nx=16, ny=16, nz=16, scheme=2), 2),
+    "shock_bubble_cu6_pp_visc": ("shock-bubble", "shock-bubble",
+                                 dict(nx=96, ny=48, scheme=2, positivity=1, viscous=(1, 1, 1)), 6),
 }
 
 

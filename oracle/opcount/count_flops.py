@@ -13,7 +13,8 @@ and reports operations per unit of work:
   * per interior cell-update of one full step.
 
 Classes: add/sub, mul, div, sqrt, exp, log, pow; "flops" = the sum of the
-first four (each counts 1; the oracle has no fused multiply-add).
+first four (each counts 1; a fused multiply-add counts as one mul and one
+add, the two operations of the algorithm it stands for).
 
     python oracle/opcount/count_flops.py [--case shock-bubble] [--json]
 """

@@ -67,6 +67,8 @@ inline xf_cd sqrt(xf_cd a) { xf_cnt[3]++; return xf_cd(std::sqrt(a.v)); }
 inline xf_cd exp(xf_cd a) { xf_cnt[4]++; return xf_cd(std::exp(a.v)); }
 inline xf_cd log(xf_cd a) { xf_cnt[5]++; return xf_cd(std::log(a.v)); }
 inline xf_cd pow(xf_cd a, xf_cd b) { xf_cnt[6]++; return xf_cd(std::pow(a.v, b.v)); }
+// a fused multiply-add is one multiplication and one addition of the algorithm
+inline xf_cd fma(xf_cd a, xf_cd b, xf_cd c) { xf_cnt[0]++; xf_cnt[1]++; return xf_cd(std::fma(a.v, b.v, c.v)); }
 inline bool isnan(xf_cd a) { return std::isnan(a.v); }
 inline bool isinf(xf_cd a) { return std::isinf(a.v); }
 

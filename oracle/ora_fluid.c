@@ -703,8 +703,8 @@ static void face_flux(ora_fluid *f, int dir, size_t id_l, size_t id_r, long stri
         c2 = Gamma0 * _P / _rho;
         b1 = (Gamma0 - 1.0) / c2;
     }
-    double uf[10], ff[10], pp[10], mm[10], f_flux, _p[ORA_MAXE][ORA_MAXE], eig[ORA_MAXE], eigen_value,
-        artificial_viscosity;
+    double uf[10], ff[10], pp[10], mm[10], f_flux[ORA_MAXE], _r[ORA_MAXE][ORA_MAXE], eig[ORA_MAXE],
+        eigen_value, artificial_viscosity;
     const double Roe_type = 0.0, LLF_type = 0.0, GLF_type = 1.0; /* Artificial_type 3 (GLF) */
     for (int n = 0; n < E; n++) {
         double eigen_local_max = 0.0;
@@ -718,25 +718,26 @@ static void face_flux(ora_fluid *f, int dir, size_t id_l, size_t id_r, long stri
             long id_local = (long)id_l + m * stride;
             uf[m + 3] = 0.0;
             ff[m + 3] = 0.0;
+            /* the reference's multiply-add pairs, each fused (one rounding) */
             for (int n1 = 0; n1 < E; n1++) {
-                uf[m + 3] = uf[m + 3] + UI[E * id_local + n1] * eig[n1];
-                ff[m + 3] = ff[m + 3] + Fl[E * id_local + n1] * eig[n1];
+                uf[m + 3] = fma(UI[E * id_local + n1], eig[n1], uf[m + 3]);
+                ff[m + 3] = fma(Fl[E * id_local + n1], eig[n1], ff[m + 3]);
             }
-            pp[m + 3] = 0.5 * (ff[m + 3] + artificial_viscosity * uf[m + 3]);
-            mm[m + 3] = 0.5 * (ff[m + 3] - artificial_viscosity * uf[m + 3]);
+            pp[m + 3] = 0.5 * fma(artificial_viscosity, uf[m + 3], ff[m + 3]);
+            mm[m + 3] = 0.5 * fma(-artificial_viscosity, uf[m + 3], ff[m + 3]);
         }
         if (c->scheme == ORA_WENOCU6)
-            f_flux = ora_wenocu6(pp, mm, dir == 0 ? c->dx : (dir == 1 ? c->dy : c->dz));
+            f_flux[n] = ora_wenocu6(pp, mm, dir == 0 ? c->dx : (dir == 1 ? c->dy : c->dz));
         else
-            f_flux = (c->scheme == ORA_WENO5Z) ? ora_weno5z(pp, mm) : ora_weno5js(pp, mm);
-        ora_eigen_lr(c, dir, n, NULL, eig, &eigen_value, z, _yi, c2, _rho, _u, _v, _w, _H, b1, b3);
-        for (int n1 = 0; n1 < E; n1++)
-            _p[n][n1] = f_flux * eig[n1];
+            f_flux[n] = (c->scheme == ORA_WENO5Z) ? ora_weno5z(pp, mm) : ora_weno5js(pp, mm);
+        ora_eigen_lr(c, dir, n, NULL, _r[n], &eigen_value, z, _yi, c2, _rho, _u, _v, _w, _H, b1, b3);
     }
+    /* _p[n1][n] = f_flux[n1] * R[n1][n] summed over n1 in increasing order,
+       fused as above */
     for (int n = 0; n < E; n++) {
         double fluxl = 0.0;
         for (int n1 = 0; n1 < E; n1++)
-            fluxl += _p[n1][n];
+            fluxl = fma(f_flux[n1], _r[n1][n], fluxl);
         Fwall[E * id_l + n] = fluxl;
     }
 }

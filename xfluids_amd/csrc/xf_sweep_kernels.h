@@ -613,9 +613,9 @@ k_sweep(Geom g, const DevThermo *__restrict__ th, const double *__restrict__ Ug,
         auto split_weno = [&](double av, const double *uf, const double *ff) -> double {
             double pp[7], mm[7];
 #pragma unroll
-            for (int m = 1; m <= PHI; m++) pp[m] = 0.5 * (ff[m] + av * uf[m]);
+            for (int m = 1; m <= PHI; m++) pp[m] = 0.5 * fma(av, uf[m], ff[m]);
 #pragma unroll
-            for (int m = PLO; m <= 6; m++) mm[m] = 0.5 * (ff[m] - av * uf[m]);
+            for (int m = PLO; m <= 6; m++) mm[m] = 0.5 * fma(-av, uf[m], ff[m]);
             return weno_face<SCHEME>(pp, mm, dl);
         };
         auto split_weno_f32 = [&](float av, const float *uf, const float *ff) -> double {
@@ -656,7 +656,8 @@ k_sweep(Geom g, const DevThermo *__restrict__ th, const double *__restrict__ Ug,
                 for (int m = 0; m < NCOP; m++) L[5 + m] = rf.z[m];
             }
             // projections, component-outer: every accumulator still sums
-            // over n1 in increasing order; the empty asm is a compiler-only
+            // over n1 in increasing order, one fused multiply-add per term
+            // (plain form); the empty asm is a compiler-only
             // fence that keeps the LDS loads of one component from being
             // hoisted above the previous one (register pressure)
             double fl;
@@ -687,32 +688,33 @@ k_sweep(Geom g, const DevThermo *__restrict__ th, const double *__restrict__ Ug,
 #pragma unroll
                     for (int m = 1; m <= 6; m++) {
                         const int li = lds_ix<DIR, TS, TW>(fidx + m - 1, tcol);
-                        uf[m] = uf[m] + sU[n1 * NC + li] * Ln;
-                        ff[m] = ff[m] + sF[n1 * NC + li] * Ln;
+                        uf[m] = fma(sU[n1 * NC + li], Ln, uf[m]);
+                        ff[m] = fma(sF[n1 * NC + li], Ln, ff[m]);
                     }
                     asm volatile("" ::: "memory");
                 }
                 fl = split_weno(glf_alpha<DIR, E>(f.red, n), uf, ff);
             }
             // right column n, accumulated in the reference's summation order
+            // with one fused multiply-add per entry
             if (n == 0 || n == E - 1) {
-                flux[0] += fl * 1.0;
+                flux[0] += fl;
 #pragma unroll
                 for (int k = 0; k < 3; k++) {
                     const double Rk = (k == DIR) ? (n == 0 ? vr3[k] - _c : vr3[k] + _c) : vr3[k];
-                    flux[1 + k] += fl * Rk;
+                    flux[1 + k] = fma(fl, Rk, flux[1 + k]);
                 }
-                flux[4] += fl * (n == 0 ? _H - un * _c : _H + un * _c);
+                flux[4] = fma(fl, n == 0 ? _H - un * _c : _H + un * _c, flux[4]);
 #pragma unroll
-                for (int m = 0; m < NCOP; m++) flux[5 + m] += fl * rf._yi[m];
+                for (int m = 0; m < NCOP; m++) flux[5 + m] = fma(fl, rf._yi[m], flux[5 + m]);
             } else {
-                flux[0] += fl * b1;
-                flux[1] += fl * (_u * b1);
-                flux[2] += fl * (_v * b1);
-                flux[3] += fl * (_w * b1);
-                flux[4] += fl * (_H * b1 - 1.0);
+                flux[0] = fma(fl, b1, flux[0]);
+                flux[1] = fma(fl, _u * b1, flux[1]);
+                flux[2] = fma(fl, _v * b1, flux[2]);
+                flux[3] = fma(fl, _w * b1, flux[3]);
+                flux[4] = fma(fl, _H * b1 - 1.0, flux[4]);
 #pragma unroll
-                for (int m = 0; m < NCOP; m++) flux[5 + m] += fl * (b1 * rf._yi[m]);
+                for (int m = 0; m < NCOP; m++) flux[5 + m] = fma(fl, b1 * rf._yi[m], flux[5 + m]);
             }
         };
 
@@ -720,7 +722,9 @@ k_sweep(Geom g, const DevThermo *__restrict__ th, const double *__restrict__ Ug,
         // the E-3 rows with two structural non-zeros (L[0] and +-1 at kk; the
         // right column has +-1 at kk and an energy entry R4), in a runtime
         // loop; contributions go to flux[kk] through a masked add (+0.0 on
-        // the other components is exact: no accumulator is ever -0.0)
+        // the other components is exact: no accumulator is ever -0.0).
+        // A product by an exact 0 or +-1 rounds the same fused or unfused,
+        // so these rows equal the full fused dot products of the oracle.
 #pragma unroll 1
         for (int n = 1; n <= E - 2; n++) {
             if (n == 1 + DIR) {
@@ -777,7 +781,7 @@ k_sweep(Geom g, const DevThermo *__restrict__ th, const double *__restrict__ Ug,
 #pragma unroll
             for (int k = 1; k < E; k++)
                 if (k != 4) flux[k] += (k == kk) ? t : 0.0;
-            flux[4] += fl * R4;
+            flux[4] = fma(fl, R4, flux[4]);
         }
         dense_row(E - 1);
         // positivity-preserving limiter on the faces right of interior cells

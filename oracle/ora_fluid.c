@@ -385,14 +385,19 @@ double ora_weno5js(const double *pp, const double *mm)
     return (temf + temm) * six;
 }
 
-/* weno5Z_P / weno5Z_M body: WENO5s_schemes.hpp:246-316 */
+/* weno5Z_P / weno5Z_M body: WENO5s_schemes.hpp:246-316.  Each multiply-add
+   pair of the indicators and the candidate polynomials is fused, in the
+   reference's association: a*b + c becomes fma(a, b, c). */
 static double weno5z_body(double v1, double v2, double v3, double v4, double v5)
 {
     double s1, s2, s3, a1, a2, a3, w1, w2, w3;
     double epsilon = 1.0e-6;
-    s1 = 13.0 * (v1 - 2.0 * v2 + v3) * (v1 - 2.0 * v2 + v3) + 3.0 * (v1 - 4.0 * v2 + 3.0 * v3) * (v1 - 4.0 * v2 + 3.0 * v3);
-    s2 = 13.0 * (v2 - 2.0 * v3 + v4) * (v2 - 2.0 * v3 + v4) + 3.0 * (v2 - v4) * (v2 - v4);
-    s3 = 13.0 * (v3 - 2.0 * v4 + v5) * (v3 - 2.0 * v4 + v5) + 3.0 * (3.0 * v3 - 4.0 * v4 + v5) * (3.0 * v3 - 4.0 * v4 + v5);
+    double d1 = fma(-2.0, v2, v1) + v3, e1 = fma(3.0, v3, fma(-4.0, v2, v1)); /* v1 - 2v2 + v3, v1 - 4v2 + 3v3 */
+    double d2 = fma(-2.0, v3, v2) + v4, e2 = v2 - v4;
+    double d3 = fma(-2.0, v4, v3) + v5, e3 = fma(-4.0, v4, 3.0 * v3) + v5;  /* 3v3 - 4v4 + v5 */
+    s1 = fma(13.0 * d1, d1, (3.0 * e1) * e1);                               /* 13 d^2 + 3 e^2 */
+    s2 = fma(13.0 * d2, d2, (3.0 * e2) * e2);
+    s3 = fma(13.0 * d3, d3, (3.0 * e3) * e3);
     double s5 = fabs(s1 - s3);
     a1 = 0.1 * (1.0 + s5 / (s1 + epsilon));
     a2 = 0.6 * (1.0 + s5 / (s2 + epsilon));
@@ -400,8 +405,8 @@ static double weno5z_body(double v1, double v2, double v3, double v4, double v5)
     w1 = a1 / (a1 + a2 + a3);
     w2 = a2 / (a1 + a2 + a3);
     w3 = a3 / (a1 + a2 + a3);
-    return w1 * (2.0 * v1 - 7.0 * v2 + 11.0 * v3) / 6.0 + w2 * (-v2 + 5.0 * v3 + 2.0 * v4) / 6.0 +
-           w3 * (2.0 * v3 + 5.0 * v4 - v5) / 6.0;
+    return w1 * fma(11.0, v3, fma(-7.0, v2, 2.0 * v1)) / 6.0 + w2 * fma(2.0, v4, fma(5.0, v3, -v2)) / 6.0 +
+           w3 * (fma(5.0, v4, 2.0 * v3) - v5) / 6.0;
 }
 
 double ora_weno5z(const double *pp, const double *mm)

@@ -104,13 +104,18 @@ __device__ __forceinline__ double div6_fast(double x, bool &slow)
 // WENO5-Z body (weno5Z_P / weno5Z_M).  The three weights share the divisor
 // S = a1 + a2 + a3 >= 1 and have numerators >= 0.1, so they come from one
 // correctly rounded reciprocal RN(1/S); the divisions by 6 use div6_fast.  Every
-// quotient is the IEEE-rounded one, bit for bit.
+// quotient is the IEEE-rounded one, bit for bit.  The indicators and the
+// candidate polynomials fuse each multiply-add pair of the reference's
+// expressions (same association, one rounding per pair).
 __device__ __forceinline__ double weno5z_body(double v1, double v2, double v3, double v4, double v5)
 {
     const double epsilon = 1.0e-6;
-    const double s1 = 13.0 * (v1 - 2.0 * v2 + v3) * (v1 - 2.0 * v2 + v3) + 3.0 * (v1 - 4.0 * v2 + 3.0 * v3) * (v1 - 4.0 * v2 + 3.0 * v3);
-    const double s2 = 13.0 * (v2 - 2.0 * v3 + v4) * (v2 - 2.0 * v3 + v4) + 3.0 * (v2 - v4) * (v2 - v4);
-    const double s3 = 13.0 * (v3 - 2.0 * v4 + v5) * (v3 - 2.0 * v4 + v5) + 3.0 * (3.0 * v3 - 4.0 * v4 + v5) * (3.0 * v3 - 4.0 * v4 + v5);
+    const double d1 = fma(-2.0, v2, v1) + v3, e1 = fma(3.0, v3, fma(-4.0, v2, v1));
+    const double d2 = fma(-2.0, v3, v2) + v4, e2 = v2 - v4;
+    const double d3 = fma(-2.0, v4, v3) + v5, e3 = fma(-4.0, v4, 3.0 * v3) + v5;
+    const double s1 = fma(13.0 * d1, d1, (3.0 * e1) * e1);
+    const double s2 = fma(13.0 * d2, d2, (3.0 * e2) * e2);
+    const double s3 = fma(13.0 * d3, d3, (3.0 * e3) * e3);
     const double s5 = fabs(s1 - s3);
     const double a1 = 0.1 * (1.0 + s5 / (s1 + epsilon));
     const double a2 = 0.6 * (1.0 + s5 / (s2 + epsilon));
@@ -120,9 +125,9 @@ __device__ __forceinline__ double weno5z_body(double v1, double v2, double v3, d
     const double w1 = quot_from_recip(a1, S, rS);
     const double w2 = quot_from_recip(a2, S, rS);
     const double w3 = quot_from_recip(a3, S, rS);
-    const double x1 = w1 * (2.0 * v1 - 7.0 * v2 + 11.0 * v3);
-    const double x2 = w2 * (-v2 + 5.0 * v3 + 2.0 * v4);
-    const double x3 = w3 * (2.0 * v3 + 5.0 * v4 - v5);
+    const double x1 = w1 * fma(11.0, v3, fma(-7.0, v2, 2.0 * v1));
+    const double x2 = w2 * fma(2.0, v4, fma(5.0, v3, -v2));
+    const double x3 = w3 * (fma(5.0, v4, 2.0 * v3) - v5);
     bool slow = false;
     double q1 = div6_fast(x1, slow), q2 = div6_fast(x2, slow), q3 = div6_fast(x3, slow);
     if (slow) q1 = x1 / 6.0, q2 = x2 / 6.0, q3 = x3 / 6.0;

@@ -9,8 +9,9 @@
  * Every function cites the reference file:line it restates (paths relative to
  * the XFluids source tree).  Layout follows the reference: U[Emax*id + n],
  * y[NUM_SPECIES*id + k], id = Xmax*Ymax*k + Xmax*j + i, primitives one array
- * per variable.  Build with -ffp-contract=off (no FMA contraction), matching
- * the HIP side.
+ * per variable.  Build with -ffp-contract=off (no compiler FMA contraction),
+ * matching the HIP side; fused multiply-adds are explicit fma() calls at the
+ * same places on both sides.
  *
  * Parity status per function (what pins it) is listed in the header comment
  * of each source file; functions with no closed-form / invariant / library

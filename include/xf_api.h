@@ -52,8 +52,13 @@ extern "C" {
 #define XF_WENOCU6 2 /* WENOCU6_GPU, schemes/WENO6s_schemes.hpp:5-80 (SCHEME_ORDER 6) */
 
 /* precision forms */
-#define XF_PLAIN_FP64 0 /* everything FP64, bitwise reproducible vs the CPU restatement */
+#define XF_PLAIN_FP64 0 /* everything FP64 in the reference's expression order, each
+                           product and sum rounded as written (bitwise vs the CPU
+                           restatement) */
 #define XF_MIXED 1      /* FP32 smoothness indicators, weights and projections */
+#define XF_FUSED_FP64 2 /* FP64 with the multiply-add pairs of the characteristic
+                           projections, GLF split, back-projection and WENO5-Z
+                           rounded once (fma); not the reference's rounding */
 
 /* chemistry drivers */
 #define XF_CHEM_PER_CELL 0 /* one lane per cell */
@@ -79,7 +84,7 @@ typedef struct xf_case_desc {
     int react;          /* Sources_React */
     int strang;         /* 1 = Strang splitting, 0 = Lie */
     double ncop_gamma;  /* NCOP_Gamma */
-    int precision;      /* XF_PLAIN_FP64 or XF_MIXED */
+    int precision;      /* XF_PLAIN_FP64, XF_MIXED or XF_FUSED_FP64 */
     int chem_driver;    /* XF_CHEM_PER_CELL or XF_CHEM_QUEUE */
     int positivity;     /* PositivityPreserving flux limiter on the face fluxes
                            (PositivityPreserving_kernels.hpp:5-76) */

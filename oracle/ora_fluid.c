@@ -385,10 +385,35 @@ double ora_weno5js(const double *pp, const double *mm)
     return (temf + temm) * six;
 }
 
-/* weno5Z_P / weno5Z_M body: WENO5s_schemes.hpp:246-316.  Each multiply-add
-   pair of the indicators and the candidate polynomials is fused, in the
-   reference's association: a*b + c becomes fma(a, b, c). */
+/* weno5Z_P / weno5Z_M body: WENO5s_schemes.hpp:246-316, in the reference's
+   expression order (each product rounded, then each sum).  The reference
+   writes every indicator difference twice, 13.0 * (d) * (d); both copies
+   round to the same value, so it is formed once. */
 static double weno5z_body(double v1, double v2, double v3, double v4, double v5)
+{
+    double s1, s2, s3, a1, a2, a3, w1, w2, w3;
+    double epsilon = 1.0e-6;
+    double d1 = v1 - 2.0 * v2 + v3, e1 = v1 - 4.0 * v2 + 3.0 * v3;
+    double d2 = v2 - 2.0 * v3 + v4, e2 = v2 - v4;
+    double d3 = v3 - 2.0 * v4 + v5, e3 = 3.0 * v3 - 4.0 * v4 + v5;
+    s1 = 13.0 * d1 * d1 + 3.0 * e1 * e1;
+    s2 = 13.0 * d2 * d2 + 3.0 * e2 * e2;
+    s3 = 13.0 * d3 * d3 + 3.0 * e3 * e3;
+    double s5 = fabs(s1 - s3);
+    a1 = 0.1 * (1.0 + s5 / (s1 + epsilon));
+    a2 = 0.6 * (1.0 + s5 / (s2 + epsilon));
+    a3 = 0.3 * (1.0 + s5 / (s3 + epsilon));
+    w1 = a1 / (a1 + a2 + a3);
+    w2 = a2 / (a1 + a2 + a3);
+    w3 = a3 / (a1 + a2 + a3);
+    return w1 * (2.0 * v1 - 7.0 * v2 + 11.0 * v3) / 6.0 + w2 * (-v2 + 5.0 * v3 + 2.0 * v4) / 6.0 +
+           w3 * (2.0 * v3 + 5.0 * v4 - v5) / 6.0;
+}
+
+/* The same body with each multiply-add pair of the indicators and candidate
+   polynomials fused (a*b + c -> fma(a, b, c), same association): the
+   arithmetic of the fused FP64 device form, not the reference's. */
+static double weno5z_body_fused(double v1, double v2, double v3, double v4, double v5)
 {
     double s1, s2, s3, a1, a2, a3, w1, w2, w3;
     double epsilon = 1.0e-6;
@@ -412,6 +437,12 @@ static double weno5z_body(double v1, double v2, double v3, double v4, double v5)
 double ora_weno5z(const double *pp, const double *mm)
 {
     return weno5z_body(pp[1], pp[2], pp[3], pp[4], pp[5]) + weno5z_body(mm[6], mm[5], mm[4], mm[3], mm[2]);
+}
+
+double ora_weno5z_fused(const double *pp, const double *mm)
+{
+    return weno5z_body_fused(pp[1], pp[2], pp[3], pp[4], pp[5]) +
+           weno5z_body_fused(mm[6], mm[5], mm[4], mm[3], mm[2]);
 }
 
 /* WENOCU6_BODYGPU: src/solver_Reconstruction/schemes/WENO6s_schemes.hpp:5-51,
@@ -723,26 +754,41 @@ static void face_flux(ora_fluid *f, int dir, size_t id_l, size_t id_r, long stri
             long id_local = (long)id_l + m * stride;
             uf[m + 3] = 0.0;
             ff[m + 3] = 0.0;
-            /* the reference's multiply-add pairs, each fused (one rounding) */
-            for (int n1 = 0; n1 < E; n1++) {
-                uf[m + 3] = fma(UI[E * id_local + n1], eig[n1], uf[m + 3]);
-                ff[m + 3] = fma(Fl[E * id_local + n1], eig[n1], ff[m + 3]);
+            if (c->fused) {
+                /* fused FP64 form: each multiply-add pair rounded once */
+                for (int n1 = 0; n1 < E; n1++) {
+                    uf[m + 3] = fma(UI[E * id_local + n1], eig[n1], uf[m + 3]);
+                    ff[m + 3] = fma(Fl[E * id_local + n1], eig[n1], ff[m + 3]);
+                }
+                pp[m + 3] = 0.5 * fma(artificial_viscosity, uf[m + 3], ff[m + 3]);
+                mm[m + 3] = 0.5 * fma(-artificial_viscosity, uf[m + 3], ff[m + 3]);
+            } else {
+                for (int n1 = 0; n1 < E; n1++) {
+                    uf[m + 3] = uf[m + 3] + UI[E * id_local + n1] * eig[n1];
+                    ff[m + 3] = ff[m + 3] + Fl[E * id_local + n1] * eig[n1];
+                }
+                pp[m + 3] = 0.5 * (ff[m + 3] + artificial_viscosity * uf[m + 3]);
+                mm[m + 3] = 0.5 * (ff[m + 3] - artificial_viscosity * uf[m + 3]);
             }
-            pp[m + 3] = 0.5 * fma(artificial_viscosity, uf[m + 3], ff[m + 3]);
-            mm[m + 3] = 0.5 * fma(-artificial_viscosity, uf[m + 3], ff[m + 3]);
         }
         if (c->scheme == ORA_WENOCU6)
             f_flux[n] = ora_wenocu6(pp, mm, dir == 0 ? c->dx : (dir == 1 ? c->dy : c->dz));
+        else if (c->scheme == ORA_WENO5Z)
+            f_flux[n] = c->fused ? ora_weno5z_fused(pp, mm) : ora_weno5z(pp, mm);
         else
-            f_flux[n] = (c->scheme == ORA_WENO5Z) ? ora_weno5z(pp, mm) : ora_weno5js(pp, mm);
+            f_flux[n] = ora_weno5js(pp, mm);
         ora_eigen_lr(c, dir, n, NULL, _r[n], &eigen_value, z, _yi, c2, _rho, _u, _v, _w, _H, b1, b3);
     }
-    /* _p[n1][n] = f_flux[n1] * R[n1][n] summed over n1 in increasing order,
-       fused as above */
+    /* F-flux reconstruction: _p[n1][n] = f_flux[n1] * R[n1][n], summed over
+       n1 in increasing order (fused form: one fma per term) */
     for (int n = 0; n < E; n++) {
         double fluxl = 0.0;
-        for (int n1 = 0; n1 < E; n1++)
-            fluxl = fma(f_flux[n1], _r[n1][n], fluxl);
+        for (int n1 = 0; n1 < E; n1++) {
+            if (c->fused)
+                fluxl = fma(f_flux[n1], _r[n1][n], fluxl);
+            else
+                fluxl += f_flux[n1] * _r[n1][n];
+        }
         Fwall[E * id_l + n] = fluxl;
     }
 }

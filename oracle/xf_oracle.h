@@ -9,9 +9,12 @@
  * Every function cites the reference file:line it restates (paths relative to
  * the XFluids source tree).  Layout follows the reference: U[Emax*id + n],
  * y[NUM_SPECIES*id + k], id = Xmax*Ymax*k + Xmax*j + i, primitives one array
- * per variable.  Build with -ffp-contract=off (no compiler FMA contraction),
- * matching the HIP side; fused multiply-adds are explicit fma() calls at the
- * same places on both sides.
+ * per variable.  Build with -ffp-contract=off (no compiler FMA contraction):
+ * every product and sum rounds as the reference writes it.  The one
+ * exception is the fused FP64 variant of the face flux (cfg.fused = 1), which
+ * rounds each multiply-add pair of the characteristic projections, GLF
+ * split, back-projection and WENO5-Z once (explicit fma() calls) to check
+ * the device form with the same arithmetic.
  *
  * Parity status per function (what pins it) is listed in the header comment
  * of each source file; functions with no closed-form / invariant / library
@@ -81,6 +84,10 @@ typedef struct {
     /* transport fits of Setup::GetFitCoefficient (src/read_ini/src/viscfit.cpp:148-193):
        ln(mu_k), ln(lambda_k) and ln(p D_kj) as cubics in ln T */
     double fcv[ORA_MAXNS * 4], fct[ORA_MAXNS * 4], Dkj[ORA_MAXNS * ORA_MAXNS * 4];
+    /* 0: the face flux in the reference's expression order (default);
+       1: its multiply-add pairs fused, the arithmetic of the fused FP64
+       device form */
+    int fused;
 } ora_config;
 
 typedef struct ora_fluid ora_fluid;
@@ -126,6 +133,7 @@ double ora_mix_R(const ora_config *c, const double *yi);
 double ora_get_T(const ora_config *c, const double *yi, double e, double T0);
 double ora_weno5js(const double *pp, const double *mm);     /* weno5old_GPU(&pp[3],&mm[3]) */
 double ora_weno5z(const double *pp, const double *mm);      /* weno5Z_P + weno5Z_M        */
+double ora_weno5z_fused(const double *pp, const double *mm); /* the same, multiply-adds fused */
 double ora_wenocu6(const double *pp, const double *mm, double delta); /* WENOCU6_GPU(&pp[3],&mm[3],dl) */
 /* left row n / right column n of the Roe eigen-system in direction dir
    (0,1,2); z, yi of length NS-1 */

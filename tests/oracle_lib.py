@@ -35,7 +35,7 @@ class OraConfig(C.Structure):
                 ("visc", C.c_int), ("visc_heat", C.c_int), ("visc_diffu", C.c_int),
                 ("Yil_limiter", C.c_double), ("Dim_limiter", C.c_double),
                 ("fcv", C.c_double * (MAXNS * 4)), ("fct", C.c_double * (MAXNS * 4)),
-                ("Dkj", C.c_double * (MAXNS * MAXNS * 4))]
+                ("Dkj", C.c_double * (MAXNS * MAXNS * 4)), ("fused", C.c_int)]
 
 
 _lib = None
@@ -65,6 +65,7 @@ def load(path):
         "ora_mix_h": (C.c_double, [P, _DP, C.c_double]), "ora_mix_e": (C.c_double, [P, _DP, C.c_double]),
         "ora_mix_R": (C.c_double, [P, _DP]), "ora_get_T": (C.c_double, [P, _DP, C.c_double, C.c_double]),
         "ora_weno5js": (C.c_double, [_DP, _DP]), "ora_weno5z": (C.c_double, [_DP, _DP]),
+        "ora_weno5z_fused": (C.c_double, [_DP, _DP]),
         "ora_wenocu6": (C.c_double, [_DP, _DP, C.c_double]),
         "ora_eigen_lr": (None, [P, C.c_int, C.c_int, _DP, _DP, _DP, _DP, _DP] + [C.c_double] * 8),
         "ora_evalcv": (None, [P, _DP, _DP, _DP, C.c_double, C.c_double]),
@@ -87,8 +88,10 @@ def lib():
 RU = 6.02214076e26 * 1.380649e-23 * 1.0E-3
 
 
-def config_from_setup(setup, use_libm=0, **over):
-    """OraConfig carrying the same geometry and tables as setup.context()."""
+def config_from_setup(setup, use_libm=0, fused=0, **over):
+    """OraConfig carrying the same geometry and tables as setup.context().
+    fused = 1 selects the face-flux arithmetic of the fused FP64 device form
+    (multiply-add pairs rounded once) instead of the reference's order."""
     d = setup.desc
     c = OraConfig()
     for name, _ in OraConfig._fields_:
@@ -100,6 +103,7 @@ def config_from_setup(setup, use_libm=0, **over):
     for q in range(6):
         c.BC[q] = d.BC[q]
     c.use_libm = use_libm
+    c.fused = fused
     if d.cop:
         Hia, Hib, Wi = setup.thermo_tables()
         NS = d.NS

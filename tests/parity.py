@@ -51,8 +51,8 @@ def make(case, seed=7, precision=xf.XF_PLAIN_FP64):
     return s, U, T, nsteps
 
 
-def oracle_for(s, U, T, use_libm=0):
-    o = Oracle(config_from_setup(s, use_libm=use_libm))
+def oracle_for(s, U, T, use_libm=0, fused=0):
+    o = Oracle(config_from_setup(s, use_libm=use_libm, fused=fused))
     o.set_state(U, T)
     return o
 

@@ -1,0 +1,69 @@
+"""Parity at the workloads' own step counts, on reduced grids.
+
+The isentropic vortex runs its 200 steps (64^2 instead of 256^2) and the
+shock-bubble interaction its 500 steps (128 x 64 instead of 1024 x 512).  The
+plain FP64 form must stay bit-identical to the oracle over the whole horizon;
+the mixed form must keep L1 relative errors of rho, p and T within 1e-6 and
+the mass fractions within 1e-6 of the oracle's range.
+"""
+import numpy as np
+import pytest
+
+import xfluids_amd as xf
+from parity import announce, compare, field_errors, make, oracle_for
+
+pytestmark = pytest.mark.gpu
+
+LONG = {"vortex": 200, "shock_bubble_z_janaf": 500, "shock_bubble_js_nasa": 500}
+TOL_L1 = 1e-6
+TOL_Y = 1e-6
+
+_oracle_cache = {}
+
+
+def _oracle(case):
+    if case not in _oracle_cache:
+        s, U, T, _ = make(case)
+        o = oracle_for(s, U, T)
+        assert o.initialize() == 0
+        st, dts = o.steps(LONG[case])
+        assert st == 0
+        _oracle_cache[case] = (s, U, T, o.field("U").copy(), o.field("T").copy(), dts.copy())
+    return _oracle_cache[case]
+
+
+def _gpu(s, U, T, nsteps, precision):
+    ctx = s.context(precision=precision)
+    ctx.upload(U, T)
+    ctx.initialize()
+    dts = ctx.steps(nsteps, graph=True, record_dt=True)
+    Ug, Tg = ctx.download(0)
+    ctx.close()
+    return Ug, Tg, dts
+
+
+@pytest.mark.parametrize("case", list(LONG))
+def test_plain_bitwise_over_full_horizon(case):
+    announce("plain_bitwise_over_full_horizon %s %d steps" % (case, LONG[case]))
+    s, U, T, Uo, To, dts_o = _oracle(case)
+    Ug, Tg, dts_g = _gpu(s, U, T, LONG[case], xf.XF_PLAIN_FP64)
+    eqU, relU = compare(Uo, Ug, s.desc.E)
+    print("bitwise U %.6f max rel %s" % (eqU, np.array2string(relU, precision=2)))
+    assert eqU == 1.0
+    assert np.array_equal(dts_o, dts_g)
+
+
+@pytest.mark.parametrize("case", list(LONG))
+def test_mixed_error_over_full_horizon(case):
+    announce("mixed_error_over_full_horizon %s %d steps" % (case, LONG[case]))
+    s, U, T, Uo, To, _ = _oracle(case)
+    Ug, Tg, _ = _gpu(s, U, T, LONG[case], xf.XF_MIXED)
+    err = field_errors(s, Uo, To, Ug, Tg)
+    print({k: v for k, v in err.items() if not k.endswith("Yk")})
+    for k in ("rho", "p", "T"):
+        if "l1_rel_" + k in err:
+            assert err["l1_rel_" + k] <= TOL_L1, (k, err)
+    if "Y_range_gpu" in err:
+        lo, hi = err["Y_range_oracle"]
+        glo, ghi = err["Y_range_gpu"]
+        assert glo >= lo - TOL_Y and ghi <= hi + TOL_Y, err

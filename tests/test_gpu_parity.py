@@ -2,8 +2,11 @@
 
 Bar: within 1e-10 relative (max |dU| / max |U| per conserved component, and
 T) after several full time steps.  The plain form is built to be bit-identical
-(same operation order, no FMA contraction, the same exp/log); the tests print
-the bit-exact fraction and require it for the fluid-only cases.
+(the reference's operation order and rounding, no FMA contraction, the same
+exp/log); the tests print the bit-exact fraction and require it for the
+fluid-only cases.  The fused FP64 form is held to the same bar against the
+oracle's fused variant (cfg.fused = 1), and its distance from the reference's
+rounding is printed.
 """
 import numpy as np
 import pytest
@@ -16,8 +19,8 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-10
 
 
-def _run_gpu(s, U, T, nsteps, mode):
-    ctx = s.context()
+def _run_gpu(s, U, T, nsteps, mode, precision=xf.XF_PLAIN_FP64):
+    ctx = s.context(precision=precision)
     ctx.upload(U, T)
     ctx.initialize()
     if mode == "staged":
@@ -66,6 +69,32 @@ def test_steps_match_oracle(case):
     assert np.abs(dts_o - dts_g).max() <= TOL * dts_o.max()
     if not d.react:
         assert eqU == 1.0, "fluid-only plain form should be bit-identical"
+
+
+FUSED_CASES = ["vortex", "shock_bubble_z_janaf", "shock_bubble_js_nasa", "shock_bubble_3d", "shock_bubble_pp",
+               "shock_bubble_visc", "mixing_layer_3d", "reactive_shock_tube"]
+
+
+@pytest.mark.parametrize("case", FUSED_CASES)
+def test_fused_form_matches_fused_oracle(case):
+    announce("fused_form_matches_fused_oracle " + case)
+    s, U, T, nsteps = make(case)
+    d = s.desc
+    o = oracle_for(s, U, T, fused=1)
+    assert o.initialize() == 0
+    st, dts_o = o.steps(nsteps)
+    assert st == 0
+    Ug, Tg, dts_g = _run_gpu(s, U, T, nsteps, "graph", precision=xf.XF_FUSED_FP64)
+    eqU, relU = compare(o.field("U"), Ug, d.E)
+    r = oracle_for(s, U, T)
+    r.initialize()
+    r.steps(nsteps)
+    eqR, relR = compare(r.field("U"), Ug, d.E)
+    print("fused form vs fused oracle: bitwise U %.6f max rel %s; vs reference rounding: bitwise %.4f max rel %s" %
+          (eqU, np.array2string(relU, precision=2), eqR, np.array2string(relR, precision=2)))
+    assert np.all(relU <= TOL), relU
+    if not d.react:
+        assert eqU == 1.0
 
 
 @pytest.mark.parametrize("mode", ["graph", "staged"])

@@ -224,7 +224,7 @@ int xf_create(const xf_case_desc *desc, xf_ctx **out)
     if (d.Xmax != d.X_inner + 2 * d.Bwidth_X || d.Ymax != d.Y_inner + 2 * d.Bwidth_Y ||
         d.Zmax != d.Z_inner + 2 * d.Bwidth_Z)
         return XF_EINVAL;
-    if (d.precision != XF_PLAIN_FP64 && d.precision != XF_MIXED) return XF_EINVAL;
+    if (d.precision != XF_PLAIN_FP64 && d.precision != XF_MIXED && d.precision != XF_FUSED_FP64) return XF_EINVAL;
     if (d.chem_driver != XF_CHEM_PER_CELL && d.chem_driver != XF_CHEM_QUEUE) return XF_EINVAL;
     if (d.scheme != XF_WENO5JS && d.scheme != XF_WENO5Z && d.scheme != XF_WENOCU6) return XF_EINVAL;
     // the viscous terms need the species transport data of a mixture, and

@@ -165,6 +165,7 @@ template <int NS, bool COP> struct Launch {
     static void sweep_d(xf_ctx *c, const double *Ug, int mode, int stage, const double *Ua, const double *Ub, double *Uo)
     {
         if (c->d.precision == XF_MIXED) sweep_p<DIR, XF_MIXED>(c, Ug, mode, stage, Ua, Ub, Uo);
+        else if (c->d.precision == XF_FUSED_FP64) sweep_p<DIR, XF_FUSED_FP64>(c, Ug, mode, stage, Ua, Ub, Uo);
         else sweep_p<DIR, XF_PLAIN_FP64>(c, Ug, mode, stage, Ua, Ub, Uo);
     }
     static void sweep(xf_ctx *c, int dir, const double *Ug, int mode, int stage, const double *Ua, const double *Ub,

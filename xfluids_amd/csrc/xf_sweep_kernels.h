@@ -9,7 +9,15 @@
 // (Eigen_callback.h:179-230), weno5old_GPU / weno5Z_{P,M}
 // (schemes/WENO5s_schemes.hpp), UpdateFluidLU (Reconstruction_kernels.hpp:201-234).
 //
-// GPU structure (plain FP64 form):
+// Three arithmetic forms share the kernel (template PREC):
+//  * XF_PLAIN_FP64: every product and sum rounded in the reference's
+//    expression order (uf + U*l, 0.5 * (ff + a*uf), w * poly / 6, fluxl += f*r);
+//  * XF_FUSED_FP64: the same FP64 operations with each multiply-add pair of
+//    the characteristic projections, the GLF split, the back-projection and
+//    the WENO5-Z indicators and candidate polynomials rounded once (fma);
+//  * XF_MIXED: FP32 projections and WENO weights, FP64 elsewhere.
+//
+// GPU structure (FP64 forms):
 //  * a block owns a tile of TS faces along the sweep axis x TW pencils; the
 //    TS+5 cells of each pencil are staged once into LDS as conserved U and
 //    physical flux F (F built from U, p and velocity exactly as GetPhysFlux);
@@ -81,6 +89,14 @@ __device__ __forceinline__ double weno5js_body(double v1, double v2, double v3, 
     return (s1 + s2 + s3);
 }
 
+// a*b + c as the form computes it: one rounding (fused form) or the
+// reference's two, product first
+template <bool FUSED> __device__ __forceinline__ double madd(double a, double b, double c)
+{
+    if (FUSED) return fma(a, b, c);
+    return c + a * b;
+}
+
 // Correctly rounded quotients without the full division sequence.
 // Markstein (IBM J. Res. Dev. 34, 1990): if y = RN(1/b) and q0 = RN(a y) is
 // within one ulp of a/b, the remainder r = a - b q0 is exact (one fma) and
@@ -104,18 +120,36 @@ __device__ __forceinline__ double div6_fast(double x, bool &slow)
 // WENO5-Z body (weno5Z_P / weno5Z_M).  The three weights share the divisor
 // S = a1 + a2 + a3 >= 1 and have numerators >= 0.1, so they come from one
 // correctly rounded reciprocal RN(1/S); the divisions by 6 use div6_fast.  Every
-// quotient is the IEEE-rounded one, bit for bit.  The indicators and the
-// candidate polynomials fuse each multiply-add pair of the reference's
-// expressions (same association, one rounding per pair).
+// quotient is the IEEE-rounded one, bit for bit.  The plain form rounds the
+// indicators and candidate polynomials as the reference writes them (each
+// indicator difference, written twice there, is formed once: both copies
+// round alike); the fused form rounds each multiply-add pair once.
+template <bool FUSED>
 __device__ __forceinline__ double weno5z_body(double v1, double v2, double v3, double v4, double v5)
 {
     const double epsilon = 1.0e-6;
-    const double d1 = fma(-2.0, v2, v1) + v3, e1 = fma(3.0, v3, fma(-4.0, v2, v1));
-    const double d2 = fma(-2.0, v3, v2) + v4, e2 = v2 - v4;
-    const double d3 = fma(-2.0, v4, v3) + v5, e3 = fma(-4.0, v4, 3.0 * v3) + v5;
-    const double s1 = fma(13.0 * d1, d1, (3.0 * e1) * e1);
-    const double s2 = fma(13.0 * d2, d2, (3.0 * e2) * e2);
-    const double s3 = fma(13.0 * d3, d3, (3.0 * e3) * e3);
+    double s1, s2, s3, p1, p2, p3;
+    if (FUSED) {
+        const double d1 = fma(-2.0, v2, v1) + v3, e1 = fma(3.0, v3, fma(-4.0, v2, v1));
+        const double d2 = fma(-2.0, v3, v2) + v4, e2 = v2 - v4;
+        const double d3 = fma(-2.0, v4, v3) + v5, e3 = fma(-4.0, v4, 3.0 * v3) + v5;
+        s1 = fma(13.0 * d1, d1, (3.0 * e1) * e1);
+        s2 = fma(13.0 * d2, d2, (3.0 * e2) * e2);
+        s3 = fma(13.0 * d3, d3, (3.0 * e3) * e3);
+        p1 = fma(11.0, v3, fma(-7.0, v2, 2.0 * v1));
+        p2 = fma(2.0, v4, fma(5.0, v3, -v2));
+        p3 = fma(5.0, v4, 2.0 * v3) - v5;
+    } else {
+        const double d1 = v1 - 2.0 * v2 + v3, e1 = v1 - 4.0 * v2 + 3.0 * v3;
+        const double d2 = v2 - 2.0 * v3 + v4, e2 = v2 - v4;
+        const double d3 = v3 - 2.0 * v4 + v5, e3 = 3.0 * v3 - 4.0 * v4 + v5;
+        s1 = 13.0 * d1 * d1 + 3.0 * e1 * e1;
+        s2 = 13.0 * d2 * d2 + 3.0 * e2 * e2;
+        s3 = 13.0 * d3 * d3 + 3.0 * e3 * e3;
+        p1 = 2.0 * v1 - 7.0 * v2 + 11.0 * v3;
+        p2 = -v2 + 5.0 * v3 + 2.0 * v4;
+        p3 = 2.0 * v3 + 5.0 * v4 - v5;
+    }
     const double s5 = fabs(s1 - s3);
     const double a1 = 0.1 * (1.0 + s5 / (s1 + epsilon));
     const double a2 = 0.6 * (1.0 + s5 / (s2 + epsilon));
@@ -125,9 +159,9 @@ __device__ __forceinline__ double weno5z_body(double v1, double v2, double v3, d
     const double w1 = quot_from_recip(a1, S, rS);
     const double w2 = quot_from_recip(a2, S, rS);
     const double w3 = quot_from_recip(a3, S, rS);
-    const double x1 = w1 * fma(11.0, v3, fma(-7.0, v2, 2.0 * v1));
-    const double x2 = w2 * fma(2.0, v4, fma(5.0, v3, -v2));
-    const double x3 = w3 * (fma(5.0, v4, 2.0 * v3) - v5);
+    const double x1 = w1 * p1;
+    const double x2 = w2 * p2;
+    const double x3 = w3 * p3;
     bool slow = false;
     double q1 = div6_fast(x1, slow), q2 = div6_fast(x2, slow), q3 = div6_fast(x3, slow);
     if (slow) q1 = x1 / 6.0, q2 = x2 / 6.0, q3 = x3 / 6.0;
@@ -180,7 +214,7 @@ __device__ __forceinline__ double wenocu6_body(double v1, double v2, double v3, 
 
 // pp, mm hold the split values of points i-2..i+3 in [1..6] (WENO5 reads
 // pp[1..5] and mm[2..6]; WENO-CU6 reads all six on both sides)
-template <int SCHEME>
+template <int SCHEME, bool FUSED>
 __device__ __forceinline__ double weno_face(const double *pp, const double *mm, double delta)
 {
     if (SCHEME == XF_WENOCU6) {
@@ -190,7 +224,8 @@ __device__ __forceinline__ double weno_face(const double *pp, const double *mm, 
         return (temf + temm) * (1.0 / 6.0);
     }
     if (SCHEME == XF_WENO5Z)
-        return weno5z_body(pp[1], pp[2], pp[3], pp[4], pp[5]) + weno5z_body(mm[6], mm[5], mm[4], mm[3], mm[2]);
+        return weno5z_body<FUSED>(pp[1], pp[2], pp[3], pp[4], pp[5]) +
+               weno5z_body<FUSED>(mm[6], mm[5], mm[4], mm[3], mm[2]);
     const double six = 1.0 / 6.0;
     const double temf = weno5js_body(pp[1], pp[2], pp[3], pp[4], pp[5]);
     const double temm = weno5js_body(mm[6], mm[5], mm[4], mm[3], mm[2]);
@@ -460,6 +495,7 @@ k_sweep(Geom g, const DevThermo *__restrict__ th, const double *__restrict__ Ug,
     using SS = SweepShape<DIR, E, SHAPE>;
     constexpr int TS = SS::TS, TW = SS::TW, NC = SS::NC;
     constexpr bool MIX = PREC == XF_MIXED;
+    constexpr bool FUSED = PREC == XF_FUSED_FP64;
     __shared__ double sU[E * NC];
     __shared__ double sF[MIX ? 1 : E * NC];   // FP64 flux: plain form only
     __shared__ float sUf[MIX ? E * NC : 1];   // FP32 copies: mixed form only
@@ -618,10 +654,10 @@ k_sweep(Geom g, const DevThermo *__restrict__ th, const double *__restrict__ Ug,
         auto split_weno = [&](double av, const double *uf, const double *ff) -> double {
             double pp[7], mm[7];
 #pragma unroll
-            for (int m = 1; m <= PHI; m++) pp[m] = 0.5 * fma(av, uf[m], ff[m]);
+            for (int m = 1; m <= PHI; m++) pp[m] = FUSED ? 0.5 * fma(av, uf[m], ff[m]) : 0.5 * (ff[m] + av * uf[m]);
 #pragma unroll
-            for (int m = PLO; m <= 6; m++) mm[m] = 0.5 * fma(-av, uf[m], ff[m]);
-            return weno_face<SCHEME>(pp, mm, dl);
+            for (int m = PLO; m <= 6; m++) mm[m] = FUSED ? 0.5 * fma(-av, uf[m], ff[m]) : 0.5 * (ff[m] - av * uf[m]);
+            return weno_face<SCHEME, FUSED>(pp, mm, dl);
         };
         auto split_weno_f32 = [&](float av, const float *uf, const float *ff) -> double {
             float pp[7], mm[7];
@@ -661,8 +697,8 @@ k_sweep(Geom g, const DevThermo *__restrict__ th, const double *__restrict__ Ug,
                 for (int m = 0; m < NCOP; m++) L[5 + m] = rf.z[m];
             }
             // projections, component-outer: every accumulator still sums
-            // over n1 in increasing order, one fused multiply-add per term
-            // (plain form); the empty asm is a compiler-only
+            // over n1 in increasing order, product then sum (plain form) or
+            // one fused multiply-add per term (fused form); the empty asm is a compiler-only
             // fence that keeps the LDS loads of one component from being
             // hoisted above the previous one (register pressure)
             double fl;
@@ -693,33 +729,33 @@ k_sweep(Geom g, const DevThermo *__restrict__ th, const double *__restrict__ Ug,
 #pragma unroll
                     for (int m = 1; m <= 6; m++) {
                         const int li = lds_ix<DIR, TS, TW>(fidx + m - 1, tcol);
-                        uf[m] = fma(sU[n1 * NC + li], Ln, uf[m]);
-                        ff[m] = fma(sF[n1 * NC + li], Ln, ff[m]);
+                        uf[m] = madd<FUSED>(sU[n1 * NC + li], Ln, uf[m]);
+                        ff[m] = madd<FUSED>(sF[n1 * NC + li], Ln, ff[m]);
                     }
                     asm volatile("" ::: "memory");
                 }
                 fl = split_weno(glf_alpha<DIR, E>(f.red, n), uf, ff);
             }
             // right column n, accumulated in the reference's summation order
-            // with one fused multiply-add per entry
+            // (fluxl += f_flux * r per entry, fused in the fused form)
             if (n == 0 || n == E - 1) {
                 flux[0] += fl;
 #pragma unroll
                 for (int k = 0; k < 3; k++) {
                     const double Rk = (k == DIR) ? (n == 0 ? vr3[k] - _c : vr3[k] + _c) : vr3[k];
-                    flux[1 + k] = fma(fl, Rk, flux[1 + k]);
+                    flux[1 + k] = madd<FUSED>(fl, Rk, flux[1 + k]);
                 }
-                flux[4] = fma(fl, n == 0 ? _H - un * _c : _H + un * _c, flux[4]);
+                flux[4] = madd<FUSED>(fl, n == 0 ? _H - un * _c : _H + un * _c, flux[4]);
 #pragma unroll
-                for (int m = 0; m < NCOP; m++) flux[5 + m] = fma(fl, rf._yi[m], flux[5 + m]);
+                for (int m = 0; m < NCOP; m++) flux[5 + m] = madd<FUSED>(fl, rf._yi[m], flux[5 + m]);
             } else {
-                flux[0] = fma(fl, b1, flux[0]);
-                flux[1] = fma(fl, _u * b1, flux[1]);
-                flux[2] = fma(fl, _v * b1, flux[2]);
-                flux[3] = fma(fl, _w * b1, flux[3]);
-                flux[4] = fma(fl, _H * b1 - 1.0, flux[4]);
+                flux[0] = madd<FUSED>(fl, b1, flux[0]);
+                flux[1] = madd<FUSED>(fl, _u * b1, flux[1]);
+                flux[2] = madd<FUSED>(fl, _v * b1, flux[2]);
+                flux[3] = madd<FUSED>(fl, _w * b1, flux[3]);
+                flux[4] = madd<FUSED>(fl, _H * b1 - 1.0, flux[4]);
 #pragma unroll
-                for (int m = 0; m < NCOP; m++) flux[5 + m] = fma(fl, b1 * rf._yi[m], flux[5 + m]);
+                for (int m = 0; m < NCOP; m++) flux[5 + m] = madd<FUSED>(fl, b1 * rf._yi[m], flux[5 + m]);
             }
         };
 
@@ -729,7 +765,7 @@ k_sweep(Geom g, const DevThermo *__restrict__ th, const double *__restrict__ Ug,
         // loop; contributions go to flux[kk] through a masked add (+0.0 on
         // the other components is exact: no accumulator is ever -0.0).
         // A product by an exact 0 or +-1 rounds the same fused or unfused,
-        // so these rows equal the full fused dot products of the oracle.
+        // so these rows equal the oracle's full dot products in either form.
 #pragma unroll 1
         for (int n = 1; n <= E - 2; n++) {
             if (n == 1 + DIR) {
@@ -786,7 +822,7 @@ k_sweep(Geom g, const DevThermo *__restrict__ th, const double *__restrict__ Ug,
 #pragma unroll
             for (int k = 1; k < E; k++)
                 if (k != 4) flux[k] += (k == kk) ? t : 0.0;
-            flux[4] = fma(fl, R4, flux[4]);
+            flux[4] = madd<FUSED>(fl, R4, flux[4]);
         }
         dense_row(E - 1);
         // positivity-preserving limiter on the faces right of interior cells

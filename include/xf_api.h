@@ -145,7 +145,8 @@ int xf_chem_specialization(const xf_ctx *ctx);
 int xf_set_transport(xf_ctx *ctx, const double *fcv, const double *fct, const double *Dkj);
 
 /* ---- state exchange (AoS host <-> SoA device) ---------------------------- */
-/* which: 0 = U, 1 = U1.  T is the per-cell temperature (Newton initial guess).
+/* which: 0 = U, 1 = U1, 2 = U2 (the stage-2 result the fused step keeps apart
+   from U1).  T is the per-cell temperature (Newton initial guess).
    Uploading U (which = 0) also sets U1 = U, as the reference's initial-
    condition kernels do (e.g. 2D-EulerVortex/ini_sample.hpp:84-88). */
 int xf_upload_state(xf_ctx *ctx, int which, const double *U_aos, const double *T);
@@ -160,16 +161,18 @@ int xf_download_lu(xf_ctx *ctx, double *LU_aos);
 int xf_download_maxima(xf_ctx *ctx, double *eigen_block, double *uvw_c_max);
 
 /* ---- the stage seam ---------------------------------------------------- */
-/* FluidBoundaryCondition (src/solver_BCs/BCs_block.cpp:4-218) on U (0) or U1 (1). */
+/* FluidBoundaryCondition (src/solver_BCs/BCs_block.cpp:4-218) on U (0), U1 (1)
+   or U2 (2); the same which codes hold for every stage function below. */
 int xf_boundary(xf_ctx *ctx, int which);
 /* UpdateFluidStateFlux (src/solver_UpdateStates/UpdateStates_block.cpp:7-191):
    rho, Y, u, v, w, p, H, c, gamma, T (Newton), validation.  Also folds the
    GLF maxima (GetLU's eigen/GLF passes, ConVenction_block.hpp:42-172) and the
-   CFL maxima (GetDt, GlobalDt_block.hpp:237-277) into the same pass. */
+   CFL maxima (GetDt, GlobalDt_block.hpp:6-114) into the same pass. */
 int xf_update_states(xf_ctx *ctx, int which);
-/* GetLU (src/solver_Reconstruction/FDM_Method/ConVenction_block.hpp:10-617)
-   without positivity limiter and viscous terms: characteristic WENO face
-   fluxes and LU = -div F. */
+/* GetLU (src/solver_Reconstruction/FDM_Method/ConVenction_block.hpp:10-617):
+   characteristic WENO face fluxes, the positivity-preserving limiter when the
+   case enables it (ConVenction_block.hpp:328-413), the viscous wall fluxes
+   when Visc is on (ConVenction_block.hpp:424-575), and LU = -div F. */
 int xf_compute_lu(xf_ctx *ctx, int which);
 /* UpdateURK3rd (src/solver_UpdateStates/UpdateStates_block.cpp:193-211). */
 int xf_rk3_stage(xf_ctx *ctx, int stage, double dt);
@@ -200,8 +203,41 @@ int xf_synchronize(xf_ctx *ctx);
 /* The context's HIP stream (hipStream_t as void*). */
 void *xf_stream(xf_ctx *ctx);
 
+/* ---- the clock of the time loop ------------------------------------------
+   XFLUIDS::Evolution keeps physicalTime and Iteration and clamps each dt so
+   that physicalTime lands on the next output time (src/XFLUIDS.cpp:170-198:
+   while (physicalTime < target_t) { Iteration++; dt = ComputeTimeStep;
+   if (physicalTime + dt > target_t) dt = target_t - physicalTime;
+   physicalTime += dt; ... }).  Here the clock lives on the device, so a
+   captured step needs no host round trip: each step of xf_steps counts
+   itself, clamps its dt to the target and advances the time; once the time
+   has reached the target, further steps do nothing (reached = 1) until a
+   new target is set.  The default target is +infinity (no clamp).
+   xf_set_time sets the time and the step count (e.g. from a checkpoint
+   header); xf_set_target_time sets the output time and clears reached. */
+int xf_set_time(xf_ctx *ctx, double time, int step);
+int xf_set_target_time(xf_ctx *ctx, double target);
+int xf_get_time(xf_ctx *ctx, double *time, int *step, int *reached);
+
 /* ---- diagnostics --------------------------------------------------------- */
+/* The last physical error (XF_ERR_*) and its cell: the lowest padded-grid
+   (i, j, k) whose failure raised that code, in the first pass that failed.
+   The reference prints the cell of its error check (UpdateStates_block.cpp:
+   80-90, 152-160; Fluids.cpp:1015-1029) and stops the time loop at that
+   stage (src/XFLUIDS.cpp:211-225, 285-287, 377-389, 441-525).  In xf_steps
+   the failing pass halts the rest of its step and every later step on the
+   device, so the state stays as that pass left it (a failure in the fused
+   third-stage update has already written U there).  Codes 1 and 2 come
+   from UpdateStates, 3 from EstimateNAN (before each stage's update, and
+   after each Reaction), 5 from the chemistry. */
 int xf_last_error(xf_ctx *ctx, int *code, int *i, int *j, int *k);
+/* Step and stage of the last physical error: step = the Iteration counter of
+   the device clock (1 for the first step after xf_set_time(ctx, t, 0));
+   where = 1, 2, 3 for the RK stage (its UpdateStates or its EstimateNAN,
+   the reference's "_RK1".."_RK3" and flags 1-3), 4 for a Reaction ("_React",
+   the chemistry and EstimateNAN flag 4).  Zeros for a failure in a stage
+   function called on its own. */
+int xf_last_error_step(xf_ctx *ctx, int *step, int *where);
 /* Average device time (ms) per launch of each kernel class since the last
    reset, measured with HIP events on the context stream when profiling is on. */
 int xf_set_profiling(xf_ctx *ctx, int on);

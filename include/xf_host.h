@@ -81,7 +81,8 @@ int xf_checkpoint_read(const char *path, const xf_case_desc *desc, double *U_aos
                        float *time_consumption);
 /* CopyToUbak + Output_Ubak / Read_Ubak + CopyToU for a context: download U and
    write it; read and upload it (which = 0, so U1 = U as after
-   initialisation), keeping the context's current T as the Newton guess. */
+   initialisation), keeping the context's current T as the Newton guess; the
+   load also sets the device clock (xf_set_time) from the header. */
 int xf_save_checkpoint(xf_ctx *ctx, const char *path, int step, double time, float time_consumption);
 int xf_load_checkpoint(xf_ctx *ctx, const char *path, int *step, double *time, float *time_consumption);
 

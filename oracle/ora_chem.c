@@ -210,7 +210,8 @@ void ora_chemistry(ora_fluid *f, double dt)
     const ora_config *c = &f->cfg;
     const int E = f->E, NS = c->NS;
     int bad = 0;
-#pragma omp parallel for collapse(2) schedule(dynamic, 1) reduction(| : bad)
+    long bad_id = -1;
+#pragma omp parallel for collapse(2) schedule(dynamic, 1) reduction(| : bad) reduction(max : bad_id)
     for (int k = c->Bwidth_Z; k < c->Zmax - c->Bwidth_Z; k++)
         for (int j = c->Bwidth_Y; j < c->Ymax - c->Bwidth_Y; j++)
             for (int i = c->Bwidth_X; i < c->Xmax - c->Bwidth_X; i++) {
@@ -221,8 +222,11 @@ void ora_chemistry(ora_fluid *f, double dt)
                 for (int n = 0; n < NS; n++)
                     _yi[n] = yi[n];
                 double e0 = ora_mix_e(c, _yi, _y[0]);
-                if (ora_chemeq2(c, _y, dt, _rho, f->p[id]) < 0)
+                if (ora_chemeq2(c, _y, dt, _rho, f->p[id]) < 0) {
                     bad = 1;
+                    /* the lowest failing id, kept as the largest -id */
+                    if (-(long)id - 1 > bad_id) bad_id = -(long)id - 1;
+                }
                 double *_U = f->U + E * id + 5;
                 double e1 = ora_mix_e(c, _yi, _y[0]);
                 for (int n = 0; n < NS - 1; n++)
@@ -230,6 +234,7 @@ void ora_chemistry(ora_fluid *f, double dt)
                 f->T[id] = _y[0], *(--_U) += (e1 - e0) * _rho;
             }
     chem_status = bad;
+    if (bad) f->err_code = 5, f->err_id = (size_t)(-bad_id - 1);
 }
 
 int ora_chemistry_status(void) { return chem_status; }

@@ -60,6 +60,7 @@ ora_fluid *ora_create(const ora_config *cfg)
     /* eigen_block zeroed once at allocation: src/Fluids.cpp:421-429 */
     memset(f->eigen_block, 0, sizeof(f->eigen_block));
     memset(f->uvw_c_max, 0, sizeof(f->uvw_c_max));
+    f->target = HUGE_VAL;
     ora_visc_alloc(f);
     return f;
 }
@@ -183,7 +184,7 @@ int ora_update_states(ora_fluid *f, int which)
                         double yv = f->y[NS * id + s];
                         bad = bad || isnan(yv) || isinf(yv);
                     }
-                if (bad) err = 1;
+                if (bad && !err) err = 1, f->err_code = 1, f->err_id = id; /* ascending ids: the lowest */
             }
     if (err) return 1;
 
@@ -203,7 +204,7 @@ int ora_update_states(ora_fluid *f, int which)
                 size_t id = IDX(f, i, j, k);
                 double vs[3] = {f->rho[id], f->p[id], f->T[id]};
                 for (int q = 0; q < 3; q++)
-                    if (vs[q] < 0 || isnan(vs[q]) || isinf(vs[q])) err = 2;
+                    if ((vs[q] < 0 || isnan(vs[q]) || isinf(vs[q])) && !err) err = 2, f->err_code = 2, f->err_id = id;
             }
     return err;
 }
@@ -960,7 +961,7 @@ int ora_estimate_nan(ora_fluid *f, int flag)
                 int bad = UI[id] < 0;
                 for (int ii = 0; ii < E; ii++)
                     bad = bad || isnan(UI[id + ii]) || isinf(UI[id + ii]) || isnan(f->LU[id + ii]) || isinf(f->LU[id + ii]);
-                if (bad) err = 3;
+                if (bad && !err) err = 3, f->err_code = 3, f->err_id = id / E;
             }
     return err;
 }

@@ -22,6 +22,15 @@ struct ora_fluid {
        and conductivity, mixture-averaged D_km (species-major), species
        enthalpies (AoS, NS * id + k) */
     double *Vde, *va, *tca, *Dkm, *hi;
+    /* the last physical error: code, lowest failing cell id of that code,
+       and the step (Iteration) and stage (1..3 RK, 4 Reaction) it came in */
+    int err_code, err_step, err_where;
+    size_t err_id;
+    /* the clock of XFLUIDS::Evolution (src/XFLUIDS.cpp:170-198):
+       physicalTime, the output time dt is clamped to, Iteration, and
+       whether the time has reached the target */
+    double time, target;
+    int step, done;
 };
 
 void ora_visc_alloc(ora_fluid *f);

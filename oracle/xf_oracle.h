@@ -116,6 +116,14 @@ int ora_estimate_nan(ora_fluid *f, int flag);               /* Fluid::EstimateFl
 /* whole steps in the reference's Evolution ordering; returns 0 or error */
 int ora_initialize(ora_fluid *f);                           /* main: BC + UpdateStates */
 int ora_steps(ora_fluid *f, int nsteps, double *dt_out);
+/* the clock of Evolution: physicalTime, Iteration, the output time each dt
+   is clamped to (default +inf), and whether the time has reached it */
+void ora_set_time(ora_fluid *f, double time, int step);
+void ora_set_target_time(ora_fluid *f, double target);
+void ora_get_time(const ora_fluid *f, double *time, int *step, int *reached);
+/* the last physical error code (0 none), its lowest failing cell and the
+   step and stage (1..3 RK, 4 Reaction) of ora_steps it came in */
+int ora_last_error(const ora_fluid *f, int *i, int *j, int *k, int *step, int *where);
 
 /* per-function entry points used by the pin tests */
 double ora_exp(double x);

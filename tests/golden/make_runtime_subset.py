@@ -8,13 +8,13 @@ H2/O2 19-reaction mechanisms and the transport tables (transport_data.dat,
 collision_integral.dat), and writes the mixture lists of this project's own
 workloads (air/H2 shock-bubble, H2/N2 vs air mixing layer).
 
-    python xfluids_amd/data/make_runtime_subset.py [reference_runtime_dat]
+    python tests/golden/make_runtime_subset.py [reference_runtime_dat]
 """
 import os
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-OUT = os.path.join(HERE, "runtime.dat")
+OUT = os.path.join(os.path.dirname(os.path.dirname(HERE)), "xfluids_amd", "data", "runtime.dat")
 SPECIES = ["NCOP", "H", "O", "H2O", "OH", "O2", "H2", "H2O2", "HO2", "N2", "AR"]
 
 

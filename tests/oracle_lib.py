@@ -40,6 +40,7 @@ class OraConfig(C.Structure):
 
 _lib = None
 _DP = C.POINTER(C.c_double)
+_IPT = C.POINTER(C.c_int)
 
 
 def load(path):
@@ -71,6 +72,9 @@ def load(path):
         "ora_evalcv": (None, [P, _DP, _DP, _DP, C.c_double, C.c_double]),
         "ora_chemeq2": (C.c_int, [P, _DP, C.c_double, C.c_double, C.c_double]),
         "ora_face_c2": (C.c_double, [v, C.c_size_t, C.c_size_t, _DP, _DP]),
+        "ora_set_time": (None, [v, C.c_double, C.c_int]), "ora_set_target_time": (None, [v, C.c_double]),
+        "ora_get_time": (None, [v, _DP, _IPT, _IPT]),
+        "ora_last_error": (C.c_int, [v, _IPT, _IPT, _IPT, _IPT, _IPT]),
     }
     for n, (r, a) in sig.items():
         f = getattr(L, n)
@@ -205,3 +209,20 @@ class Oracle:
 
     def reaction(self, dt):
         return self.L.ora_reaction(self._h, dt)
+
+    def set_time(self, time, step=0):
+        self.L.ora_set_time(self._h, time, step)
+
+    def set_target_time(self, target):
+        self.L.ora_set_target_time(self._h, target)
+
+    def time(self):
+        t, s, r = C.c_double(), C.c_int(), C.c_int()
+        self.L.ora_get_time(self._h, C.byref(t), C.byref(s), C.byref(r))
+        return t.value, s.value, bool(r.value)
+
+    def last_error(self):
+        """(code, (i, j, k), step, stage) of the last physical error."""
+        i, j, k, s, w = (C.c_int() for _ in range(5))
+        code = self.L.ora_last_error(self._h, C.byref(i), C.byref(j), C.byref(k), C.byref(s), C.byref(w))
+        return code, (i.value, j.value, k.value), s.value, w.value

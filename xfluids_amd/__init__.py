@@ -63,12 +63,21 @@ class CaseDesc(C.Structure):
 
 
 class XfError(RuntimeError):
-    def __init__(self, status, where="", cell=None):
+    """A non-zero status.  For a physical error (status > 0): cell is the
+    padded-grid (i, j, k) of the failure, step the Iteration and stage the RK
+    stage (1..3) or 4 for a Reaction it happened in (0, 0 for a stage function
+    called on its own)."""
+
+    def __init__(self, status, where="", cell=None, step=0, stage=0):
         self.status = status
         self.cell = cell
+        self.step = step
+        self.stage = stage
         msg = "%s returned %d (%s)" % (where, status, ERRORS.get(status, "?"))
         if cell is not None:
             msg += " at cell (i,j,k)=%s" % (cell,)
+        if step:
+            msg += " in step %d, %s" % (step, "Reaction" if stage == 4 else "RK stage %d" % stage)
         super().__init__(msg)
 
 
@@ -112,6 +121,10 @@ _SIGS = {
     "xf_synchronize": (C.c_int, [_V]),
     "xf_stream": (_V, [_V]),
     "xf_last_error": (C.c_int, [_V, _IP, _IP, _IP, _IP]),
+    "xf_last_error_step": (C.c_int, [_V, _IP, _IP]),
+    "xf_set_time": (C.c_int, [_V, C.c_double, C.c_int]),
+    "xf_set_target_time": (C.c_int, [_V, C.c_double]),
+    "xf_get_time": (C.c_int, [_V, _DP, _IP, _IP]),
     "xf_set_profiling": (C.c_int, [_V, C.c_int]),
     "xf_set_profile_filter": (C.c_int, [_V, C.c_char_p]),
     "xf_time_kernel": (C.c_int, [_V, C.c_char_p, C.c_int, _DP]),
@@ -325,13 +338,29 @@ class Context:
 
     def _chk(self, st, where):
         if st:
-            cell = None
+            cell, step, stage = None, 0, 0
             if st > 0:
                 code, i, j, k = C.c_int(), C.c_int(), C.c_int(), C.c_int()
                 lib().xf_last_error(self._h, C.byref(code), C.byref(i), C.byref(j), C.byref(k))
                 cell = (i.value, j.value, k.value)
-            raise XfError(st, where, cell)
+                s_, w_ = C.c_int(), C.c_int()
+                lib().xf_last_error_step(self._h, C.byref(s_), C.byref(w_))
+                step, stage = s_.value, w_.value
+            raise XfError(st, where, cell, step, stage)
         return st
+
+    # ---- clock -------------------------------------------------------------
+    def set_time(self, time, step=0):
+        self._chk(lib().xf_set_time(self._h, time, step), "xf_set_time")
+
+    def set_target_time(self, target):
+        self._chk(lib().xf_set_target_time(self._h, target), "xf_set_target_time")
+
+    def time(self):
+        """(physical time, steps counted, target reached) of the device clock."""
+        t, s_, r = C.c_double(), C.c_int(), C.c_int()
+        self._chk(lib().xf_get_time(self._h, C.byref(t), C.byref(s_), C.byref(r)), "xf_get_time")
+        return t.value, s_.value, bool(r.value)
 
     # ---- data --------------------------------------------------------------
     def upload(self, U, T=None, which=0):

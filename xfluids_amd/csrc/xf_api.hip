@@ -85,20 +85,52 @@ int check_launch()
     return 0;
 }
 
-// read and clear the device error word; returns an XF_ERR code or 0
+// clear the error words (bits, cell ids, step, stage, halt); the step
+// counter and the target-reached flag stay
+__global__ void k_err_reset(int *err)
+{
+    const int t = threadIdx.x;
+    if (t < ERR_COUNT) err[t] = (t >= ERR_ID && t < ERR_ID + 4) ? 0x7fffffff : 0;
+}
+
+// the clock: op 0 sets the time and the step count, op 1 the target time
+// (and clears the target-reached flag)
+__global__ void k_clock(double *dt, int *err, int op, double t, int step)
+{
+    if (threadIdx.x != 0) return;
+    if (op == 0) {
+        dt[DT_TIME] = t;
+        err[ERR_COUNT] = step;
+    } else {
+        dt[DT_TARGET] = t;
+        err[ERR_DONE] = 0;
+        err[ERR_HALT] = 0;
+    }
+}
+
+// read and clear the device error words; returns an XF_ERR code or 0.  The
+// pass that failed first halted the step, so every bit comes from that one
+// pass; a pass that finds both bad densities/mass fractions and bad
+// primitives reports code 1, as the reference checks those first
+// (UpdateStates_block.cpp:58-115).  The cell is the lowest id with the
+// reported code's failure.
 int take_error(xf_ctx *c)
 {
-    int h[2] = {0, 0x7fffffff};
-    if (hipMemcpyAsync(h, c->f.err, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream) != hipSuccess) return XF_EHIP;
+    int h[ERR_SIZE];
+    if (hipMemcpyAsync(h, c->f.err, sizeof(h), hipMemcpyDeviceToHost, c->stream) != hipSuccess) return XF_EHIP;
     if (hipStreamSynchronize(c->stream) != hipSuccess) return XF_EHIP;
-    if (h[0] == 0) return 0;
-    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, c->f.err, 2);
-    int code = (h[0] & EBIT_RHO_Y) ? XF_ERR_RHO_Y
-               : (h[0] & EBIT_PRIM) ? XF_ERR_PRIM
-               : (h[0] & EBIT_CHEM) ? XF_ERR_CHEM
-                                    : XF_ERR_NAN;
+    c->step_count = h[ERR_COUNT];
+    c->target_reached = h[ERR_DONE];
+    if (h[ERR_BITS] == 0 && h[ERR_HALT] == 0) return 0;
+    hipLaunchKernelGGL(k_err_reset, dim3(1), dim3(64), 0, c->stream, c->f.err);
+    if (h[ERR_BITS] == 0) return 0; // halted at the target time
+    const int b = h[ERR_BITS];
+    const int code = (b & EBIT_RHO_Y) ? XF_ERR_RHO_Y : (b & EBIT_PRIM) ? XF_ERR_PRIM : (b & EBIT_CHEM) ? XF_ERR_CHEM : XF_ERR_NAN;
+    const int slot = code == XF_ERR_RHO_Y ? 0 : (code == XF_ERR_PRIM ? 1 : (code == XF_ERR_NAN ? 2 : 3));
     c->last_code = code;
-    c->last_id = h[1];
+    c->last_id = h[ERR_ID + slot];
+    c->last_step = h[ERR_STEP];
+    c->last_where = h[ERR_WHERE];
     return code;
 }
 
@@ -115,15 +147,16 @@ void boundary_async(xf_ctx *c, double *U, int reset_red)
 }
 
 // UpdateStates (the pass's GLF and CFL maxima replace the stored ones);
-// fill = 1 also runs FluidBoundaryCondition first, fused into the same pass
-void states_async(xf_ctx *c, double *U, int full, int fill = 0)
+// fill = 1 also runs FluidBoundaryCondition first, fused into the same pass;
+// where: the stage a failure is reported in (1..3 RK stage, 4 Reaction)
+void states_async(xf_ctx *c, double *U, int full, int fill, int where)
 {
     int bcs = 0;
     if (fill) {
         for (int q = 0; q < 6; q++) bcs |= (c->d.BC[q] & 15) << (4 * q);
         bcs |= 1 << 28;
     }
-    c->ops.states(c, U, full, bcs);
+    c->ops.states(c, U, full, bcs, where);
 }
 
 // GetLU: sweeps in x, y, z order (each folds its direction's GLF maxima).
@@ -149,18 +182,12 @@ void lu_async(xf_ctx *c, const double *Ug, int rk_stage, const double *Ua, const
 
 void reaction_prep_async(xf_ctx *c);
 
-void reaction_async(xf_ctx *c, int record)
-{
-    reaction_prep_async(c);
-    c->ops.chem(c, record);
-}
-
 // XFLUIDS::Reaction before its chemistry launch: UpdateStates(U) and the
 // device-side chemistry step (dt/2 for Strang)
 void reaction_prep_async(xf_ctx *c)
 {
-    states_async(c, c->f.U, 0);
-    hipLaunchKernelGGL(k_dt_chem, dim3(1), dim3(64), 0, c->stream, c->f.dt, c->d.strang);
+    states_async(c, c->f.U, 0, 0, 4);
+    hipLaunchKernelGGL(k_dt_chem, dim3(1), dim3(64), 0, c->stream, c->f.dt, c->d.strang, c->f.err);
 }
 
 // A step is: dt; [Strang: Reaction]; three RK stages; [Reaction].  The parts
@@ -168,20 +195,22 @@ void reaction_prep_async(xf_ctx *c)
 // can launch one chemistry kernel for all its cases at each Reaction.
 void step_pre_async(xf_ctx *c)
 {
-    hipLaunchKernelGGL(k_dt, dim3(1), dim3(64), 0, c->stream, c->g, c->f.red, c->f.dt);
+    hipLaunchKernelGGL(k_dt, dim3(1), dim3(64), 0, c->stream, c->g, c->f.red, c->f.dt, c->f.err, 1);
     if (c->d.react && c->d.strang) reaction_prep_async(c);
 }
 
 void step_mid_async(xf_ctx *c)
 {
     Fields &f = c->f;
-    // stage 1: U -> U1 ; stage 2: (U, U1) -> U2 ; stage 3: (U, U2) -> U
+    // stage 1: U -> U1 ; stage 2: (U, U1) -> U2 ; stage 3: (U, U2) -> U.
+    // At stage 1 the sweep's Ub is U2, the previous step's second-stage
+    // result, which the reference's stage-1 NaN check reads as its U1.
     double *in[3] = {f.U, f.U1, c->U2};
     for (int s = 1; s <= 3; s++) {
         double *Ug = in[s - 1];
-        states_async(c, Ug, 0, 1); // ghost fill + UpdateStates
+        states_async(c, Ug, 0, 1, s); // ghost fill + UpdateStates
         double *Uo = (s == 1) ? f.U1 : (s == 2 ? c->U2 : f.U);
-        lu_async(c, Ug, s, f.U, (s == 1) ? f.U1 : Ug, Uo);
+        lu_async(c, Ug, s, f.U, (s == 1) ? c->U2 : Ug, Uo);
     }
     if (c->d.react) reaction_prep_async(c);
 }
@@ -189,9 +218,9 @@ void step_mid_async(xf_ctx *c)
 void step_async(xf_ctx *c)
 {
     step_pre_async(c);
-    if (c->d.react && c->d.strang) c->ops.chem(c, 0);
+    if (c->d.react && c->d.strang) c->ops.chem(c, 0, 1);
     step_mid_async(c);
-    if (c->d.react) c->ops.chem(c, 0);
+    if (c->d.react) c->ops.chem(c, 0, 1);
 }
 
 } // namespace
@@ -264,7 +293,7 @@ int xf_create(const xf_case_desc *desc, xf_ctx **out)
     for (auto pp : dbl)
         if (hipMalloc(pp, n1)) return fail(XF_EHIP);
     if (hipMalloc(&f.hi, (size_t)d.NS * n1) || hipMalloc(&f.red, RED_SIZE * sizeof(double)) ||
-        hipMalloc(&f.dt, 4 * sizeof(double)) || hipMalloc(&f.err, 2 * sizeof(int)) ||
+        hipMalloc(&f.dt, DT_SIZE * sizeof(double)) || hipMalloc(&f.err, ERR_SIZE * sizeof(int)) ||
         hipMalloc(&f.substeps, (size_t)c->Ninner * sizeof(int)) || hipMalloc(&f.queue, 4 * sizeof(unsigned)) ||
         hipMalloc(&c->d_th, sizeof(DevThermo)) || hipMalloc(&c->d_mc, sizeof(DevMech)) ||
         hipMalloc(&f.spart, (size_t)nblk(c->N) * 12 * sizeof(double)))
@@ -276,11 +305,15 @@ int xf_create(const xf_case_desc *desc, xf_ctx **out)
     for (auto pp : dbl)
         if (hipMemset(*pp, 0, n1)) return fail(XF_EHIP);
     if (hipMemset(f.hi, 0, (size_t)d.NS * n1) || hipMemset(f.red, 0, RED_SIZE * sizeof(double)) ||
-        hipMemset(f.dt, 0, 4 * sizeof(double)) || hipMemset(f.substeps, 0, (size_t)c->Ninner * sizeof(int)))
+        hipMemset(f.substeps, 0, (size_t)c->Ninner * sizeof(int)))
         return fail(XF_EHIP);
     if (hipDeviceSynchronize()) return fail(XF_EHIP);
-    int e0[2] = {0, 0x7fffffff};
-    if (hipMemcpy(f.err, e0, sizeof(e0), hipMemcpyHostToDevice)) return fail(XF_EHIP);
+    int e0[ERR_SIZE] = {0};
+    for (int q = 0; q < 4; q++) e0[ERR_ID + q] = 0x7fffffff;
+    double t0[DT_SIZE] = {0.0};
+    t0[DT_TARGET] = HUGE_VAL; // no output time to land on
+    if (hipMemcpy(f.err, e0, sizeof(e0), hipMemcpyHostToDevice) || hipMemcpy(f.dt, t0, sizeof(t0), hipMemcpyHostToDevice))
+        return fail(XF_EHIP);
     c->h_th.thermo = d.thermo;
     c->h_mc.NR = d.NR;
     if (hipMemcpy(c->d_th, &c->h_th, sizeof(DevThermo), hipMemcpyHostToDevice) ||
@@ -577,7 +610,7 @@ int xf_update_states(xf_ctx *c, int which)
 {
     if (!c || which < 0 || which > 2) return XF_EINVAL;
     if (c->d.cop && !c->thermo_set) return XF_EINVAL;
-    states_async(c, which_ptr(c, which), 1);
+    states_async(c, which_ptr(c, which), 1, 0, 0);
     if (int e = check_launch()) return e;
     return take_error(c);
 }
@@ -594,11 +627,11 @@ int xf_compute_lu(xf_ctx *c, int which)
 int xf_rk3_stage(xf_ctx *c, int stage, double dt)
 {
     if (!c || stage < 1 || stage > 3) return XF_EINVAL;
-    hipLaunchKernelGGL(k_set_double, dim3(1), dim3(64), 0, c->stream, c->f.dt + 2, dt);
+    hipLaunchKernelGGL(k_set_double, dim3(1), dim3(64), 0, c->stream, c->f.dt + DT_STAGED, dt);
     {
         ProfScope ps(c, "rk3");
         hipLaunchKernelGGL(k_rk3, dim3(nblk(c->Ninner)), dim3(256), 0, c->stream, c->g, c->E, c->f.U, c->f.U1, c->f.LU,
-                           c->f.dt + 2, stage);
+                           c->f.dt + DT_STAGED, stage);
     }
     if (int e = check_launch()) return e;
     XF_HIP(hipStreamSynchronize(c->stream));
@@ -608,7 +641,7 @@ int xf_rk3_stage(xf_ctx *c, int stage, double dt)
 int xf_get_dt(xf_ctx *c, double *dt_out)
 {
     if (!c || !dt_out) return XF_EINVAL;
-    hipLaunchKernelGGL(k_dt, dim3(1), dim3(64), 0, c->stream, c->g, c->f.red, c->f.dt);
+    hipLaunchKernelGGL(k_dt, dim3(1), dim3(64), 0, c->stream, c->g, c->f.red, c->f.dt, c->f.err, 0);
     if (int e = check_launch()) return e;
     XF_HIP(hipMemcpyAsync(dt_out, c->f.dt, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     XF_HIP(hipStreamSynchronize(c->stream));
@@ -619,8 +652,8 @@ int xf_chemistry(xf_ctx *c, double dt_chem, int *substeps)
 {
     if (!c || !c->d.cop || c->d.NR <= 0) return XF_EINVAL;
     if (!c->thermo_set || !c->mech_set) return XF_EINVAL;
-    hipLaunchKernelGGL(k_set_double, dim3(1), dim3(64), 0, c->stream, c->f.dt + 1, dt_chem);
-    c->ops.chem(c, substeps ? 1 : 0);
+    hipLaunchKernelGGL(k_set_double, dim3(1), dim3(64), 0, c->stream, c->f.dt + DT_CHEM, dt_chem);
+    c->ops.chem(c, substeps ? 1 : 0, 0);
     if (int e = check_launch()) return e;
     if (substeps)
         XF_HIP(hipMemcpyAsync(substeps, c->f.substeps, (size_t)c->Ninner * sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -632,7 +665,7 @@ int xf_estimate_nan(xf_ctx *c, int flag)
     if (!c || flag < 1 || flag > 4) return XF_EINVAL;
     const double *UI = (flag <= 2) ? c->f.U1 : c->f.U;
     hipLaunchKernelGGL(k_estimate_nan, dim3(nblk(c->Ninner)), dim3(256), 0, c->stream, c->g, c->E, UI, c->f.LU,
-                       c->f.err);
+                       c->f.err, flag, 0);
     if (int e = check_launch()) return e;
     return take_error(c);
 }
@@ -723,6 +756,46 @@ int xf_last_error(xf_ctx *c, int *code, int *i, int *j, int *k)
     return XF_OK;
 }
 
+int xf_last_error_step(xf_ctx *c, int *step, int *where)
+{
+    if (!c) return XF_EINVAL;
+    if (step) *step = c->last_code ? c->last_step : 0;
+    if (where) *where = c->last_code ? c->last_where : 0;
+    return XF_OK;
+}
+
+int xf_set_time(xf_ctx *c, double time, int step)
+{
+    if (!c || step < 0 || !(time == time)) return XF_EINVAL;
+    hipLaunchKernelGGL(k_clock, dim3(1), dim3(64), 0, c->stream, c->f.dt, c->f.err, 0, time, step);
+    if (int e = check_launch()) return e;
+    XF_HIP(hipStreamSynchronize(c->stream));
+    return XF_OK;
+}
+
+int xf_set_target_time(xf_ctx *c, double target)
+{
+    if (!c || !(target == target)) return XF_EINVAL;
+    hipLaunchKernelGGL(k_clock, dim3(1), dim3(64), 0, c->stream, c->f.dt, c->f.err, 1, target, 0);
+    if (int e = check_launch()) return e;
+    XF_HIP(hipStreamSynchronize(c->stream));
+    return XF_OK;
+}
+
+int xf_get_time(xf_ctx *c, double *time, int *step, int *reached)
+{
+    if (!c) return XF_EINVAL;
+    double t[DT_SIZE];
+    int e[ERR_SIZE];
+    XF_HIP(hipMemcpyAsync(t, c->f.dt, sizeof(t), hipMemcpyDeviceToHost, c->stream));
+    XF_HIP(hipMemcpyAsync(e, c->f.err, sizeof(e), hipMemcpyDeviceToHost, c->stream));
+    XF_HIP(hipStreamSynchronize(c->stream));
+    if (time) *time = t[DT_TIME];
+    if (step) *step = e[ERR_COUNT];
+    if (reached) *reached = e[ERR_DONE];
+    return XF_OK;
+}
+
 int xf_set_profiling(xf_ctx *c, int on)
 {
     if (!c) return XF_EINVAL;
@@ -776,7 +849,7 @@ int time_launches(xf_ctx *c, int dir, int reps, double *avg_ms)
     XF_HIP(hipEventRecord(a, c->stream));
     for (int r = 0; r < reps; r++) {
         if (dir < 0) {
-            states_async(c, c->f.U, 0, 1); // as in a step: ghost fill + UpdateStates
+            states_async(c, c->f.U, 0, 1, 1); // as in a step: ghost fill + UpdateStates
         } else if (dir == 10) {
             c->ops.visc_prep(c, c->f.U);
         } else if (dir >= 20) {
@@ -784,7 +857,7 @@ int time_launches(xf_ctx *c, int dir, int reps, double *avg_ms)
         } else {
             int mode = (q > 0) ? 1 : 0;
             if (q == nd - 1) mode |= 2;
-            c->ops.sweep(c, dir, c->f.U, mode, 1, c->f.U, c->f.U1, c->f.U1);
+            c->ops.sweep(c, dir, c->f.U, mode, 1, c->f.U, c->U2, c->f.U1);
         }
     }
     XF_HIP(hipEventRecord(b, c->stream));
@@ -971,7 +1044,7 @@ void ens_chem(xf_ensemble *e)
 {
     xf_ctx *c0 = e->cases[0];
     c0->ops.chem_multi(e->stream, e->d_cases, (int)e->cases.size(), e->total, c0->d_th, c0->d_mc, e->d_queue,
-                       e->blocks, 0, c0->mech_spec);
+                       e->blocks, 0, 1, c0->mech_spec);
 }
 
 // one step of every case: each case's stages on its own stream, the

@@ -409,24 +409,44 @@ __device__ __forceinline__ int chem_cell(const DevThermo *__restrict__ th, const
     return passes;
 }
 
-// one lane per interior cell
+// EstimateNAN after the Reaction (flag 4, src/XFLUIDS.cpp:639;
+// EstimateFluidNANKernel, src/Fluids.cpp:47-87) on the cell the chemistry has
+// just written: U[0] < 0 or any component NaN/Inf.  The reference also reads
+// the third stage's LU there, which that stage's own check passed and nothing
+// has changed since.
+__device__ __forceinline__ bool cell_bad(const double *U, long long N, long long id, int E)
+{
+    bool bad = U[id] < 0;
+    for (int n = 0; n < E; n++) {
+        const double a = U[n * N + id];
+        bad = bad || isnan(a) || isinf(a);
+    }
+    return bad;
+}
+
+// one lane per interior cell; check = 1 (a time step's Reaction) also runs
+// the post-Reaction NaN check on the cell
 template <int NS, int SPEC>
 __global__ void __launch_bounds__(256) k_chem(Geom g, const DevThermo *__restrict__ th,
-                                              const DevMech *__restrict__ mc, Fields f, int record)
+                                              const DevMech *__restrict__ mc, Fields f, int record, int check)
 {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long total = (long long)g.X_inner * g.Y_inner * g.Z_inner;
     if (t >= total) return;
+    if (step_halted(f.err, ~EBIT_CHEM)) return;
     const int i = g.Bx + (int)(t % g.X_inner);
     const int j = g.By + (int)((t / g.X_inner) % g.Y_inner);
     const int k = g.Bz + (int)(t / ((long long)g.X_inner * g.Y_inner));
     const long long id = (long long)g.Xmax * g.Ymax * k + (long long)g.Xmax * j + i;
-    const int passes = chem_cell<NS, SPEC>(th, mc, f, g.N, id, f.dt[1]);
+    const int passes = chem_cell<NS, SPEC>(th, mc, f, g.N, id, f.dt[DT_CHEM]);
     if (record) f.substeps[t] = passes;
-    if (passes < 0) {
-        atomicOr(&f.err[0], EBIT_CHEM);
-        atomicMin(&f.err[1], (int)id);
+    int bits = passes < 0 ? EBIT_CHEM : 0;
+    if (passes < 0) report_cell(f.err, EBIT_CHEM, id);
+    if (check && cell_bad(f.U, g.N, id, NS + 4)) {
+        report_cell(f.err, EBIT_NAN, id);
+        bits |= EBIT_NAN;
     }
+    if (bits) report_pass(f.err, bits, 4);
 }
 
 // Interior cell t (0 <= t < total) -> padded-grid id
@@ -462,7 +482,7 @@ struct ChemCase {
 template <int NS, int SPEC, class Cases>
 __device__ __forceinline__ void chem_queue_run(const Cases &cases, long long total, const DevThermo *__restrict__ th,
                                                const DevMech *__restrict__ mc, unsigned long long *queue, int record,
-                                               double *lds)
+                                               int check, double *lds)
 {
     const int lane = threadIdx.x & 63;
     ChemLane<NS, ChemLds<NS + 1>, SPEC> L;
@@ -483,8 +503,9 @@ __device__ __forceinline__ void chem_queue_run(const Cases &cases, long long tot
             if (need) {
                 const int rank = __popcll(want & ((1ull << lane) - 1ull));
                 const long long t = (long long)(base + (unsigned long long)rank);
-                if (t < total) {
-                    cc = cases.find(t);
+                if (t < total) cc = cases.find(t);
+                // the cells of a case whose step has halted are skipped
+                if (t < total && !step_halted(cc.err, ~EBIT_CHEM)) {
                     tl = t - cc.off;
                     const long long N = cc.g.N;
                     id = chem_cell_id(cc.g, tl);
@@ -498,9 +519,9 @@ __device__ __forceinline__ void chem_queue_run(const Cases &cases, long long tot
                         L.y[NS] += -L.y[1 + k];
                     }
                     e0 = mix_e<NS>(th, L.y + 1, L.y[0]);
-                    L.begin(cc.dt[1], rho, cc.p[id]);
+                    L.begin(cc.dt[DT_CHEM], rho, cc.p[id]);
                     active = true;
-                } else {
+                } else if (t >= total) {
                     done = true;
                 }
             }
@@ -518,10 +539,13 @@ __device__ __forceinline__ void chem_queue_run(const Cases &cases, long long tot
                 cc.T[id] = L.y[0];
                 cc.U[4 * N + id] += (e1 - e0) * rho;
                 if (record) cc.substeps[tl] = st < 0 ? -1 : L.passes;
-                if (st < 0) {
-                    atomicOr(&cc.err[0], EBIT_CHEM);
-                    atomicMin(&cc.err[1], (int)id);
+                int bits = st < 0 ? EBIT_CHEM : 0;
+                if (st < 0) report_cell(cc.err, EBIT_CHEM, id);
+                if (check && cell_bad(cc.U, N, id, NS + 4)) {
+                    report_cell(cc.err, EBIT_NAN, id);
+                    bits |= EBIT_NAN;
                 }
+                if (bits) report_pass(cc.err, bits, 4);
                 active = false;
             }
         }
@@ -562,7 +586,8 @@ template <int NS> struct ChemQueueLds {
 
 template <int NS, int SPEC>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ChemOccupancy<NS>::waves)))
-k_chem_queue(Geom g, const DevThermo *__restrict__ th, const DevMech *__restrict__ mc, Fields f, int record)
+k_chem_queue(Geom g, const DevThermo *__restrict__ th, const DevMech *__restrict__ mc, Fields f, int record,
+             int check)
 {
     OneCase one;
     one.c.off = 0;
@@ -571,18 +596,18 @@ k_chem_queue(Geom g, const DevThermo *__restrict__ th, const DevMech *__restrict
     one.c.substeps = f.substeps, one.c.err = f.err;
     __shared__ double lds[ChemQueueLds<NS>::doubles];
     chem_queue_run<NS, SPEC>(one, (long long)g.X_inner * g.Y_inner * g.Z_inner, th, mc, (unsigned long long *)f.queue,
-                       record, lds);
+                             record, check, lds);
 }
 
 // one queue over the cells of every case of an ensemble (shared mechanism)
 template <int NS, int SPEC>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ChemOccupancy<NS>::waves)))
 k_chem_queue_multi(const ChemCase *__restrict__ cases, int ncases, long long total, const DevThermo *__restrict__ th,
-                   const DevMech *__restrict__ mc, unsigned long long *queue, int record)
+                   const DevMech *__restrict__ mc, unsigned long long *queue, int record, int check)
 {
     ManyCases many{cases, ncases};
     __shared__ double lds[ChemQueueLds<NS>::doubles];
-    chem_queue_run<NS, SPEC>(many, total, th, mc, queue, record, lds);
+    chem_queue_run<NS, SPEC>(many, total, th, mc, queue, record, check, lds);
 }
 
 // zero the work-queue counter (a kernel, so the step graph holds no memset)

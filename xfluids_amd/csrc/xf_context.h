@@ -22,16 +22,18 @@ struct EvPair {
 };
 
 struct Ops {
-    // UpdateStates on U; bcs != 0 (packed BC codes) also fills the ghost cells
-    void (*states)(xf_ctx *, double *U, int full, int bcs);
+    // UpdateStates on U; bcs != 0 (packed BC codes) also fills the ghost cells;
+    // where: the stage a failure is reported in (1..3 RK stage, 4 Reaction)
+    void (*states)(xf_ctx *, double *U, int full, int bcs, int where);
     void (*sweep)(xf_ctx *, int dir, const double *Ug, int mode, int stage, const double *Ua, const double *Ub,
                   double *Uo);
-    void (*chem)(xf_ctx *, int record);
+    // check = 1: the post-Reaction NaN check of a time step on each cell
+    void (*chem)(xf_ctx *, int record, int check);
     void (*prepare)(xf_ctx *);
     // shared queue over the cells of several cases (ensembles)
     // spec: compiled mechanism whose records the tables hold, or -1
     void (*chem_multi)(hipStream_t, const ChemCase *, int ncases, long long total, const DevThermo *,
-                       const DevMech *, unsigned long long *queue, int blocks, int record, int spec);
+                       const DevMech *, unsigned long long *queue, int blocks, int record, int check, int spec);
     int (*multi_blocks)(long long total, int spec);
     // viscous terms of GetLU: transport coefficients and centre gradients of
     // the state Ug, then the wall flux of one direction (before its sweep)
@@ -72,6 +74,8 @@ struct xf_ctx {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int last_code = 0, last_id = -1;
+    int last_step = 0, last_where = 0; // step (Iteration) and stage of the last failure
+    int step_count = 0, target_reached = 0; // device clock as of the last error read
     bool prof = false;
     std::string prof_filter; // ",name,name," or empty for all
     std::vector<EvPair> evs;
@@ -115,12 +119,13 @@ struct ProfScope {
 };
 
 template <int NS, bool COP> struct Launch {
-    static void states(xf_ctx *c, double *U, int full, int bcs)
+    static void states(xf_ctx *c, double *U, int full, int bcs, int where)
     {
         ProfScope ps(c, "states");
         hipLaunchKernelGGL((k_states<NS, COP>), dim3(nblk(c->N)), dim3(256), 0, c->stream, c->g, c->d_th, U, c->f,
-                           c->d.ncop_gamma, full, bcs);
-        hipLaunchKernelGGL(k_states_fold, dim3(12), dim3(1024), 0, c->stream, c->f.spart, (int)nblk(c->N), c->f.red);
+                           c->d.ncop_gamma, full, bcs, where);
+        hipLaunchKernelGGL(k_states_fold, dim3(12), dim3(1024), 0, c->stream, c->f.spart, (int)nblk(c->N), c->f.red,
+                           c->f.err);
     }
     template <int DIR, int SCH, int PREC, int SHAPE>
     static void sweep_t(xf_ctx *c, const double *Ug, int mode, int stage, const double *Ua, const double *Ub, double *Uo)
@@ -192,7 +197,7 @@ template <int NS, bool COP> struct Launch {
             f(std::integral_constant<int, -1>{});
         }
     }
-    static void chem(xf_ctx *c, int record)
+    static void chem(xf_ctx *c, int record, int check)
     {
         if constexpr (COP) {
             ProfScope ps(c, "chem");
@@ -201,10 +206,10 @@ template <int NS, bool COP> struct Launch {
                 if (c->d.chem_driver == XF_CHEM_QUEUE) {
                     hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, c->stream, c->f.queue);
                     hipLaunchKernelGGL((k_chem_queue<NS, S>), dim3(c->chem_blocks), dim3(64), 0, c->stream, c->g,
-                                       c->d_th, c->d_mc, c->f, record);
+                                       c->d_th, c->d_mc, c->f, record, check);
                 } else {
                     hipLaunchKernelGGL((k_chem<NS, S>), dim3(nblk(c->Ninner)), dim3(256), 0, c->stream, c->g,
-                                       c->d_th, c->d_mc, c->f, record);
+                                       c->d_th, c->d_mc, c->f, record, check);
                 }
             };
             spec_switch<0>(c->mech_spec, launch);
@@ -230,14 +235,14 @@ template <int NS, bool COP> struct Launch {
         }
     }
     static void chem_multi(hipStream_t st, const ChemCase *cases, int ncases, long long total, const DevThermo *th,
-                           const DevMech *mc, unsigned long long *queue, int blocks, int record, int spec)
+                           const DevMech *mc, unsigned long long *queue, int blocks, int record, int check, int spec)
     {
         if constexpr (COP) {
             auto launch = [&](auto K) {
                 constexpr int S = decltype(K)::value;
                 hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, st, (unsigned int *)queue);
                 hipLaunchKernelGGL((k_chem_queue_multi<NS, S>), dim3(blocks), dim3(64), 0, st, cases, ncases, total,
-                                   th, mc, queue, record);
+                                   th, mc, queue, record, check);
             };
             spec_switch<0>(spec, launch);
         }

@@ -196,10 +196,12 @@ static __global__ void __launch_bounds__(256) k_boundary_all(Geom g, double *__r
 // the sources are interior cells or Inflow ghosts, which no thread writes)
 template <int NS, bool COP>
 __global__ void __launch_bounds__(256) k_states(Geom g, const DevThermo *__restrict__ th, double *Ug, Fields f,
-                                                double ncop_gamma, int full, int bcs)
+                                                double ncop_gamma, int full, int bcs, int where)
 {
     constexpr int E = COP ? NS + 4 : 5;
     constexpr int NCOP = COP ? NS - 1 : 0;
+    // block-uniform: the words tested are not written by this launch
+    if (step_halted(f.err, ~(EBIT_RHO_Y | EBIT_PRIM))) return;
     const long long N = g.N;
     const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = id < N;
@@ -362,7 +364,8 @@ __global__ void __launch_bounds__(256) k_states(Geom g, const DevThermo *__restr
     }
     if (ebits) {
         atomicOr(&serr, ebits);
-        atomicMin(&f.err[1], (int)id);
+        if (ebits & EBIT_RHO_Y) report_cell(f.err, EBIT_RHO_Y, id);
+        if (ebits & EBIT_PRIM) report_cell(f.err, EBIT_PRIM, id);
     }
     __syncthreads();
     // the block's maxima go to spart; k_states_fold combines the blocks (one
@@ -373,16 +376,19 @@ __global__ void __launch_bounds__(256) k_states(Geom g, const DevThermo *__restr
         for (int q = 1; q < nw; q++) m = sred[q][threadIdx.x] > m ? sred[q][threadIdx.x] : m;
         f.spart[(long long)blockIdx.x * 12 + threadIdx.x] = m;
     }
-    if (threadIdx.x == 0 && serr) atomicOr(&f.err[0], serr);
+    if (threadIdx.x == 0 && serr) report_pass(f.err, serr, where);
 }
 
 // the per-block GLF and CFL maxima of k_states into the reduction slots: one
 // block per slot (12), each the maximum of +0 and every positive block value
 // (the pass's maxima, combined with the 0 the pass starts from)
-static __global__ void __launch_bounds__(1024) k_states_fold(const double *__restrict__ spart, int nblocks, double *red)
+static __global__ void __launch_bounds__(1024) k_states_fold(const double *__restrict__ spart, int nblocks, double *red,
+                                                             int *err)
 {
     __shared__ unsigned long long s[1024];
     const int q = blockIdx.x;
+    // latch a failure of the UpdateStates pass (or of any earlier one)
+    if (q == 0 && threadIdx.x == 0 && err[ERR_BITS]) err[ERR_HALT] = 1;
     unsigned long long v = 0ULL; // +0.0
     for (int b = threadIdx.x; b < nblocks; b += 1024) {
         const double x = spart[(long long)b * 12 + q];
@@ -402,24 +408,48 @@ static __global__ void __launch_bounds__(1024) k_states_fold(const double *__res
 // small control kernels
 // ---------------------------------------------------------------------------
 // zero the per-pass maxima (pending GLF, CFL) before an UpdateStates pass
-// GetDt: dt = CFL / sum_d max_d / Delta_d, written to device memory
-static __global__ void k_dt(Geom g, double *red, double *dt)
+// GetDt: dt = CFL / sum_d max_d / Delta_d, written to device memory.
+// step = 1 is the head of a time step (XFLUIDS::Evolution,
+// src/XFLUIDS.cpp:172-198): nothing happens once the step is halted or the
+// clock has reached the target time (the loop condition physicalTime <
+// target_t); otherwise the step is counted (Iteration++), dt is clamped so
+// that the clock lands on the target (physicalTime + dt > target_t ->
+// dt = target_t - physicalTime) and the clock advances.  It also latches a
+// failure of the previous pass into ERR_HALT.  step = 0 is GetDt alone.
+static __global__ void k_dt(Geom g, double *red, double *dt, int *err, int step)
 {
     if (threadIdx.x == 0) {
+        if (step) {
+            if (err[ERR_BITS]) err[ERR_HALT] = 1;
+            if (err[ERR_HALT]) return;
+            if (!(dt[DT_TIME] < dt[DT_TARGET])) {
+                err[ERR_DONE] = 1;
+                err[ERR_HALT] = 1;
+                dt[DT_STEP] = 0.0; // no step taken
+                return;
+            }
+            err[ERR_COUNT] = err[ERR_COUNT] + 1;
+        }
         const double mx = g.DimX ? red[RED_DT + 0] : 0.0;
         const double my = g.DimY ? red[RED_DT + 1] : 0.0;
         const double mz = g.DimZ ? red[RED_DT + 2] : 0.0;
         const double dtref = mx * g._dx + my * g._dy + mz * g._dz;
-        dt[0] = g.CFL / dtref;
+        double d = g.CFL / dtref;
+        if (step) {
+            const double t = dt[DT_TIME], te = dt[DT_TARGET];
+            if (t + d > te) d = te - t;
+            dt[DT_TIME] = t + d;
+        }
+        dt[DT_STEP] = d;
         // uvw_c_max as GetDt leaves it (the positivity limiter's lambda_0)
         red[RED_LAM + 0] = mx, red[RED_LAM + 1] = my, red[RED_LAM + 2] = mz;
     }
 }
 
 // Strang half step on device: dt[1] = dt[0] * 0.5 (or dt[0] for Lie)
-static __global__ void k_dt_chem(double *dt, int strang)
+static __global__ void k_dt_chem(double *dt, int strang, const int *err)
 {
-    if (threadIdx.x == 0) dt[1] = strang ? dt[0] * 0.5 : dt[0];
+    if (threadIdx.x == 0 && !step_halted(err, ~0)) dt[DT_CHEM] = strang ? dt[DT_STEP] * 0.5 : dt[DT_STEP];
 }
 
 // ---------------------------------------------------------------------------
@@ -454,25 +484,34 @@ static __global__ void __launch_bounds__(256) k_rk3(Geom g, int E, double *__res
     }
 }
 
-// EstimateFluidNANKernel (src/Fluids.cpp:47-87) on interior cells
+// EstimateFluidNANKernel (src/Fluids.cpp:47-87) on interior cells.  LU may
+// be null: after the Reaction (flag 4) the reference checks the LU of the
+// third stage again, which that stage's own check already passed and nothing
+// has changed since.  step = 1: a pass of a time step (returns at entry once
+// an earlier pass failed).
 static __global__ void __launch_bounds__(256) k_estimate_nan(Geom g, int E, const double *__restrict__ UI,
-                                                      const double *__restrict__ LU, int *err)
+                                                      const double *__restrict__ LU, int *err, int where, int step)
 {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long total = (long long)g.X_inner * g.Y_inner * g.Z_inner;
     if (t >= total) return;
+    if (step && step_halted(err, ~EBIT_NAN)) return;
     const int i = g.Bx + (int)(t % g.X_inner);
     const int j = g.By + (int)((t / g.X_inner) % g.Y_inner);
     const int k = g.Bz + (int)(t / ((long long)g.X_inner * g.Y_inner));
     const long long id = (long long)g.Xmax * g.Ymax * k + (long long)g.Xmax * j + i;
     bool bad = UI[id] < 0;
     for (int n = 0; n < E; n++) {
-        const double a = UI[n * g.N + id], b = LU[n * g.N + id];
-        bad = bad || isnan(a) || isinf(a) || isnan(b) || isinf(b);
+        const double a = UI[n * g.N + id];
+        bad = bad || isnan(a) || isinf(a);
+        if (LU) {
+            const double b = LU[n * g.N + id];
+            bad = bad || isnan(b) || isinf(b);
+        }
     }
     if (bad) {
-        atomicOr(&err[0], EBIT_NAN);
-        atomicMin(&err[1], (int)id);
+        report_cell(err, EBIT_NAN, id);
+        report_pass(err, EBIT_NAN, where);
     }
 }
 

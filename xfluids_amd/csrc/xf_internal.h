@@ -69,8 +69,8 @@ struct Fields {
     double *hi;                        // NS * N species enthalpy at T (COP)
     double *Cp, *gcv, *R;              // mixture Cp, Cp/Cv, R (COP)
     double *red;                       // reduction slots, see RED_*
-    double *dt;                        // device-resident dt
-    int *err;                          // [0] code bits, [1] min cell id
+    double *dt;                        // device-resident dt and clock, see DT_*
+    int *err;                          // error and progress words, see ERR_*
     int *substeps;                     // chemistry predictor passes per interior cell
     unsigned int *queue;               // chemistry work-queue counter
     // viscous terms: Wilke viscosity and conductivity, mixture-averaged
@@ -100,11 +100,57 @@ constexpr int RED_YMIN = 32;
 constexpr int RED_YMAX = RED_YMIN + MAXNS;
 constexpr int RED_SIZE = RED_YMAX + MAXNS;
 
-// error code bits in err[0]
+// error code bits in err[ERR_BITS]
 constexpr int EBIT_RHO_Y = 1;
 constexpr int EBIT_PRIM = 2;
 constexpr int EBIT_NAN = 4;
 constexpr int EBIT_CHEM = 8;
+
+// error and progress words (ints).  A pass that finds a bad cell sets its
+// bit in ERR_BITS, the lowest failing cell id of that bit in ERR_ID + b
+// (b = 0 rho/Y, 1 primitives, 2 NaN check, 3 chemistry), and the step and
+// stage it ran in.  Every kernel of a step returns at entry once an earlier
+// launch has failed (ERR_HALT, latched by the one-lane and fold kernels
+// between passes, or a bit that the kernel itself never sets), so the state
+// stays as the failing pass left it, as the reference stops its time loop
+// there (src/XFLUIDS.cpp:211-225, 285-287).  A kernel never consults bits its
+// own launch may set, so the recorded cell does not depend on block order.
+constexpr int ERR_BITS = 0;
+constexpr int ERR_ID = 1;    // 4 slots
+constexpr int ERR_STEP = 5;  // step counter (Iteration) of the failing pass
+constexpr int ERR_WHERE = 6; // 1..3 RK stage, 4 Reaction
+constexpr int ERR_HALT = 7;  // the step kernels return at entry
+constexpr int ERR_COUNT = 8; // steps begun, counted by k_dt in a step
+constexpr int ERR_DONE = 9;  // the target time was reached
+constexpr int ERR_SIZE = 16;
+
+// device-resident dt and clock (doubles)
+constexpr int DT_STEP = 0;   // dt of the current step
+constexpr int DT_CHEM = 1;   // chemistry step (dt/2 Strang, dt Lie)
+constexpr int DT_STAGED = 2; // dt of xf_rk3_stage
+constexpr int DT_TIME = 4;   // physical time after the current step's dt
+constexpr int DT_TARGET = 5; // output time the step's dt is clamped to (+inf: none)
+constexpr int DT_SIZE = 8;
+
+// the step kernels' entry test: a halted step, or a failure recorded by an
+// earlier launch (bits in mask are those the calling kernel never sets)
+__device__ __forceinline__ bool step_halted(const int *err, int mask)
+{
+    return err[ERR_HALT] != 0 || (err[ERR_BITS] & mask) != 0;
+}
+
+// record a failing cell: the bit, the lowest id per bit, the step and stage
+__device__ __forceinline__ void report_cell(int *err, int bit, long long id)
+{
+    const int slot = bit == EBIT_RHO_Y ? 0 : (bit == EBIT_PRIM ? 1 : (bit == EBIT_NAN ? 2 : 3));
+    atomicMin(&err[ERR_ID + slot], (int)id);
+}
+__device__ __forceinline__ void report_pass(int *err, int bits, int where)
+{
+    atomicOr(&err[ERR_BITS], bits);
+    atomicExch(&err[ERR_STEP], err[ERR_COUNT]);
+    atomicExch(&err[ERR_WHERE], where);
+}
 
 // universal gas constant (src/include/global_setup.h:48-54), folded at
 // compile time exactly as the reference's const expressions

@@ -501,6 +501,8 @@ k_sweep(Geom g, const DevThermo *__restrict__ th, const double *__restrict__ Ug,
     __shared__ float sUf[MIX ? E * NC : 1];   // FP32 copies: mixed form only
     __shared__ float sFf[MIX ? E * NC : 1];
     __shared__ double sP[NC], sR1[NC];
+    // block-uniform: the words tested are not written by this launch
+    if (step_halted(f.err, ~EBIT_NAN)) return;
 
     const long long N = g.N;
     const int tid = threadIdx.x;
@@ -878,20 +880,26 @@ k_sweep(Geom g, const DevThermo *__restrict__ th, const double *__restrict__ Ug,
             const double d = (sFw[n * 256 + fwm] - sFw[n * 256 + fw]) * dl;
             const double LUt = (mode & 1) ? f.LU[n * N + id] + d : 0.0 + d;
             if (mode & 2) {
-                // fused SSP-RK3 stage: Ua = U, Ub = stage input U1 (stage 2, 3), Uo = output
-                double a = Ua[n * N + id], b = (stage == 1) ? 0.0 : Ub[n * N + id];
-                bad = bad || isnan(LUt) || isinf(LUt);
+                // fused SSP-RK3 stage: Ua = U, Ub = stage input (stage 2, 3),
+                // Uo = output.  Before the update, EstimateNAN (flags 1-3,
+                // src/XFLUIDS.cpp:460-466, Fluids.cpp:963-1040) checks LU and
+                // the stage's UI: U1 at stages 1 and 2 (at stage 1 still the
+                // previous step's second-stage result, which is what Ub holds
+                // then), U at stage 3
+                const double a0 = Ua[n * N + id], b0 = Ub[n * N + id];
+                const double ui = (stage == 3) ? a0 : b0;
+                bad = bad || isnan(LUt) || isinf(LUt) || isnan(ui) || isinf(ui) || (n == 0 && ui < 0);
+                double a = a0, b = (stage == 1) ? 0.0 : b0;
                 rk3_point(stage, dt, a, b, LUt);
                 const double o = (stage == 3) ? a : b;
-                bad = bad || isnan(o) || isinf(o) || (n == 0 && o < 0);
                 Uo[n * N + id] = o;
             } else {
                 f.LU[n * N + id] = LUt;
             }
         }
         if (bad) {
-            atomicOr(&f.err[0], EBIT_NAN);
-            atomicMin(&f.err[1], (int)id);
+            report_cell(f.err, EBIT_NAN, id);
+            report_pass(f.err, EBIT_NAN, stage);
         }
     }
 }

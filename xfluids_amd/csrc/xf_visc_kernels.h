@@ -67,6 +67,7 @@ __global__ void __launch_bounds__(256) k_transport(Geom g, const DevThermo *__re
 {
     constexpr int E = NS + 4;
     constexpr int NCOP = NS - 1;
+    if (step_halted(f.err, ~0)) return; // block-uniform (no bit is set here)
     const long long N = g.N;
     const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = id < N;
@@ -250,6 +251,7 @@ static __global__ void __launch_bounds__(256) k_vel_deriv(Geom g, Fields f, int 
 {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long long)nx * ny * nz) return;
+    if (step_halted(f.err, ~0)) return;
     const int c[3] = {i0 + (int)(t % nx), j0 + (int)((t / nx) % ny), k0 + (int)(t / ((long long)nx * ny))};
     const long long N = g.N;
     const long long sy = g.Xmax, sz = (long long)g.Xmax * g.Ymax;
@@ -314,6 +316,7 @@ __global__ void __launch_bounds__(256) k_visc_flux(Geom g, const double *__restr
 {
     constexpr int E = NS + 4;
     constexpr int NCOP = NS - 1;
+    if (step_halted(f.err, ~0)) return;
     const int nx = g.X_inner + (DIR == 0), ny = g.Y_inner + (DIR == 1), nz = g.Z_inner + (DIR == 2);
     // y faces: XCD-aware block order, so the stencil rows that neighbouring
     // blocks share meet in one L2 (2.95 -> 2.13 ms at 256^3).  x needs no

@@ -324,8 +324,15 @@ int xf_load_checkpoint(xf_ctx *ctx, const char *path, int *step, double *time, f
     xf_case_desc d;
     if (!ctx || xf_get_desc(ctx, &d)) return XF_EINVAL;
     std::vector<double> U(state_doubles(d));
-    if (int e = xf_checkpoint_read(path, &d, U.data(), step, time, time_consumption)) return e;
-    return xf_upload_state(ctx, 0, U.data(), nullptr);
+    int st = 0;
+    double tm = 0.0;
+    if (int e = xf_checkpoint_read(path, &d, U.data(), &st, &tm, time_consumption)) return e;
+    if (step) *step = st;
+    if (time) *time = tm;
+    if (int e = xf_upload_state(ctx, 0, U.data(), nullptr)) return e;
+    // the device clock continues from the header (Read_Ubak restores
+    // Iteration and physicalTime, src/XFLUIDS.cpp:616-622)
+    return xf_set_time(ctx, tm, st);
 }
 
 } // extern "C"

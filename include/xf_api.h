@@ -179,6 +179,18 @@ int xf_rk3_stage(xf_ctx *ctx, int stage, double dt);
 /* GetDt (src/solver_GetDt/GlobalDt_block.hpp:6-114) from the maxima of the
    last xf_update_states. */
 int xf_get_dt(xf_ctx *ctx, double *dt_out);
+/* GetDt (src/solver_GetDt/GlobalDt_block.hpp:6-114) with the reference's
+   own arguments instead of a context: the FlowData arrays rho, p, u, v, w
+   (device pointers, one double per padded cell, Xmax*Ymax*Zmax of them, the
+   FlowData layout of src/include/global_setup.h:231-250; the array of an
+   inactive direction may be NULL) and the Block geometry and CFLnumber of
+   desc.  uvw_c_max[3] (device, may be NULL) receives max |u_d| +
+   sqrt(1.4 p / rho) over all padded cells per active direction (0 for the
+   others) and dt_out (device) CFLnumber / (sum_d max_d / Delta_d).  Runs on
+   stream (a hipStream_t; NULL = the default stream) with a workspace of one
+   maximum triple per reduction block, and returns after the stream drained. */
+int xf_get_dt_flow(const xf_case_desc *desc, const double *rho, const double *p, const double *u, const double *v,
+                   const double *w, double *uvw_c_max, double *dt_out, void *stream);
 /* ChemeODEQ2Solver (src/solver_Reaction/Reaction_block.hpp:4-38): CHEMEQ2
    over dt_chem on every interior cell, using the primitives of the last
    xf_update_states(0).  substeps (optional, NULL ok) receives the per-cell

@@ -112,6 +112,7 @@ _SIGS = {
     "xf_compute_lu": (C.c_int, [_V, C.c_int]),
     "xf_rk3_stage": (C.c_int, [_V, C.c_int, C.c_double]),
     "xf_get_dt": (C.c_int, [_V, _DP]),
+    "xf_get_dt_flow": (C.c_int, [C.POINTER(CaseDesc), _V, _V, _V, _V, _V, _V, _V, _V]),
     "xf_chemistry": (C.c_int, [_V, C.c_double, _IP]),
     "xf_reaction": (C.c_int, [_V, C.c_double]),
     "xf_estimate_nan": (C.c_int, [_V, C.c_int]),

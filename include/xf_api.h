@@ -191,6 +191,18 @@ int xf_get_dt(xf_ctx *ctx, double *dt_out);
    maximum triple per reduction block, and returns after the stream drained. */
 int xf_get_dt_flow(const xf_case_desc *desc, const double *rho, const double *p, const double *u, const double *v,
                    const double *w, double *uvw_c_max, double *dt_out, void *stream);
+/* xf_get_dt_flow without any allocation, for callers that cannot allocate
+   device memory (code under HIP graph capture, tight serving loops).  No
+   counterpart in the reference.  The same kernels keep their per-block
+   maxima in a fixed 96 KiB device array of the library (4096 triples; the
+   reduction never uses more blocks), which each GPU holds until the process
+   ends.  Cells: at most INT_MAX (Xmax*Ymax*Zmax); a larger grid returns
+   XF_EINVAL.  Enqueues on stream and returns without waiting: results land
+   in uvw_c_max and dt_out when the stream reaches them.  Calls that overlap
+   on one GPU (several streams or host threads) share that array and race;
+   serialise them. */
+int xf_get_dt_flow_noalloc(const xf_case_desc *desc, const double *rho, const double *p, const double *u,
+                           const double *v, const double *w, double *uvw_c_max, double *dt_out, void *stream);
 /* ChemeODEQ2Solver (src/solver_Reaction/Reaction_block.hpp:4-38): CHEMEQ2
    over dt_chem on every interior cell, using the primitives of the last
    xf_update_states(0).  substeps (optional, NULL ok) receives the per-cell

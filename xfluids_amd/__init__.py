@@ -30,6 +30,7 @@ XF_MAXNR = 40
 XF_WENO5JS, XF_WENO5Z, XF_WENOCU6 = 0, 1, 2
 XF_PLAIN_FP64, XF_MIXED, XF_FUSED_FP64 = 0, 1, 2
 XF_CHEM_PER_CELL, XF_CHEM_QUEUE = 0, 1
+XF_EINVAL, XF_EHIP, XF_ENOKERNEL = -1, -2, -3
 ERRORS = {1: "XF_ERR_RHO_Y", 2: "XF_ERR_PRIM", 3: "XF_ERR_NAN", 5: "XF_ERR_CHEM",
           -1: "XF_EINVAL", -2: "XF_EHIP", -3: "XF_ENOKERNEL"}
 
@@ -113,6 +114,7 @@ _SIGS = {
     "xf_rk3_stage": (C.c_int, [_V, C.c_int, C.c_double]),
     "xf_get_dt": (C.c_int, [_V, _DP]),
     "xf_get_dt_flow": (C.c_int, [C.POINTER(CaseDesc), _V, _V, _V, _V, _V, _V, _V, _V]),
+    "xf_get_dt_flow_noalloc": (C.c_int, [C.POINTER(CaseDesc), _V, _V, _V, _V, _V, _V, _V, _V]),
     "xf_chemistry": (C.c_int, [_V, C.c_double, _IP]),
     "xf_reaction": (C.c_int, [_V, C.c_double]),
     "xf_estimate_nan": (C.c_int, [_V, C.c_int]),

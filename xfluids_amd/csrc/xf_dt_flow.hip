@@ -6,12 +6,45 @@
 #include <hip/hip_runtime.h>
 #include <climits>
 #include <cstdio>
+#include <mutex>
 #include "../../include/xf_api.h"
 
 namespace {
-
 constexpr int DT_BLOCKS = 4096; // blocks of the reduction pass, at most
 constexpr int DT_THREADS = 256;
+} // namespace
+
+// Workspace of xf_get_dt_flow_noalloc: one maximum triple per reduction
+// block, DT_BLOCKS * 3 doubles (96 KiB), held by every GPU the library runs
+// on until the process ends.  Host code never uses this name as an address:
+// the device address comes from hipGetSymbolAddress, once per device.
+static __device__ __align__(256) double scratch[DT_BLOCKS * 3];
+
+namespace {
+
+constexpr int MAX_DEVICES = 64;
+double *g_scratch[MAX_DEVICES] = {};
+std::mutex g_scratch_mu;
+
+// this device's address of `scratch`, looked up on its first use
+int scratch_ptr(double **out)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEVICES) return XF_EHIP;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            fprintf(stderr, "xfluids_amd: hipGetSymbolAddress(scratch) failed on device %d: %s\n", dev,
+                    hipGetErrorString(e));
+            return XF_EHIP;
+        }
+        g_scratch[dev] = (double *)ptr;
+    }
+    *out = g_scratch[dev];
+    return XF_OK;
+}
 
 // maxima of non-negative finite values, as the reference's sycl maximum
 // reduction: a NaN never replaces a value (v > m is false)
@@ -117,6 +150,21 @@ int xf_get_dt_flow(const xf_case_desc *d, const double *rho, const double *p, co
     if (hipStreamSynchronize(st) != hipSuccess) rc = XF_EHIP;
     (void)hipFree(part);
     return rc;
+}
+
+int xf_get_dt_flow_noalloc(const xf_case_desc *d, const double *rho, const double *p, const double *u,
+                           const double *v, const double *w, double *uvw_c_max, double *dt_out, void *stream)
+{
+    long long N = 0;
+    if (int e = dt_flow_check(d, rho, p, u, v, w, dt_out, N)) return e;
+    if (N > (long long)INT_MAX) {
+        fprintf(stderr, "xfluids_amd: xf_get_dt_flow_noalloc: %lld cells exceed the limit of %d\n", N, INT_MAX);
+        return XF_EINVAL;
+    }
+    double *part = nullptr;
+    if (int e = scratch_ptr(&part)) return e;
+    // every block writes its triple before the fold reads it: nothing to clear
+    return dt_flow_enqueue(d, N, rho, p, u, v, w, part, uvw_c_max, dt_out, (hipStream_t)stream);
 }
 
 } // extern "C"

@@ -55,3 +55,18 @@ def test_bench_step_constants_are_the_counted_figures():
     # per face, plus states, LU, RK3 and dt
     assert 23500 < sr["flops_per_cell_update"] < 25000
     assert abs(sr["achieved_tflops"] - sr["flops_per_cell_update"] * 800e6 / 1e12) < 1e-9
+
+
+def test_bench_headline_constants_are_the_counted_figures():
+    """The 13-variable face flux (inviscid part, x = y = z) and the
+    chemistry's per-cell and per-pass counts of the mixing-layer workload."""
+    sys.path.insert(0, ROOT)
+    import bench
+    r = count_flops.measure("mixing-layer", nz=8, inviscid=True)
+    assert r["solve_max_residual"] < 1e-6
+    assert r["face_x_flops"] == r["face_y_flops"] == r["face_z_flops"] == bench.FACE_FLOPS_E13
+    c = count_flops.measure_chem("mixing-layer")
+    assert c["fit_max_rel_residual_flops"] < 1e-3
+    assert abs(c["per_pass_flops"] - bench.CHEM_PASS_FLOPS) < 1e-3
+    assert abs(c["per_cell_flops"] - bench.CHEM_CELL_FLOPS) < 1e-3
+    assert c["per_pass"][4:6] == [bench.CHEM_PASS_EXP, bench.CHEM_PASS_LOG]

@@ -17,8 +17,15 @@
  * Gibbs-energy closed form; mass (sum Y = 1) and element (H, O) conservation;
  * constant-volume energy conservation e(T, Y) = const; 0-D H2/O2/Ar ignition
  * against scipy solve_ivp(Radau) of an independent numpy restatement of the
- * RHS (committed fixture, tolerance-level pin: CHEMEQ2 step control itself
- * is parity unpinned).
+ * RHS (committed fixture).  The integrator body and its step control
+ * (tests/test_oracle_pins.py), run on model systems with closed-form
+ * solutions through ora_chemeq2_model: alpha(r) against the exact
+ * quasi-steady weight (fifth order), the first substep sqreps / k, exact
+ * substep products on linear decay, the relaxation step against the exact
+ * exponential, the Riccati error band of epscl, and the step-size law on
+ * every pass of stiff Riccati runs (accept / reject, 1/sqrt(eps) + 0.005 by
+ * two Heron iterations, the 0.34 cap); each constant changed on its own
+ * (ORA_MUTANT) fails one of them.
  */
 #include "ora_internal.h"
 
@@ -107,35 +114,49 @@ void ora_evalcv(const ora_config *c, const double *y, double *q, double *p, doub
 /* xfrsign(a, b): ProductionRates.h:227-241 */
 static double xfrsign2(double a, double b) { return (b >= 0.0 ? 1.0 : -1.0) * fabs(a); }
 
+/* A planted mistake for the pin tests' mutation checks: each ORA_MUTANT
+   value > 0 changes one constant (the default build has none). */
+#ifndef ORA_MUTANT
+#define ORA_MUTANT 0
+#endif
+
 /* Alpha(rtaui): ProductionRates.h:244 */
 static double alpha_of(double r)
 {
-    return (180.0 + r * (60.0 + r * (11.0 + r))) / (360.0 + r * (60.0 + r * (12.0 + r)));
+    const double a2 = ORA_MUTANT == 1 ? 12.0 : 11.0;
+    return (180.0 + r * (60.0 + r * (a2 + r))) / (360.0 + r * (60.0 + r * (12.0 + r)));
 }
+double ora_alpha(double r) { return alpha_of(r); }
 
-/* Chemeq2<NS,NR> (y[0] = T form): ProductionRates.h:488-638.  Returns the
-   number of predictor passes (accepted + rejected substeps), or -1 when the
-   substep bound is reached. */
-int ora_chemeq2(const ora_config *c, double *y, double dtg, double rho, double m_p)
+/* The right-hand side of the CHEMEQ2 system: production q and loss d of
+   every component at y */
+typedef void (*chem_rhs)(const void *ctx, const double *y, double *q, double *d);
+
+/* The body of Chemeq2 (ProductionRates.h:488-638) after the mass-fraction
+   normalisation: the initial step, predictor, corrector, error measure and
+   step control over dtg, for NL components with right-hand side rhs.
+   Returns the predictor passes (accepted + rejected substeps), or -1 at
+   the substep bound; dt_first (may be NULL) receives the first substep, and
+   trace (may be NULL) for each of the first cap passes its substep, its
+   error measure times epscl, the threshold it is compared with and the
+   time the substep ends at. */
+static int chemeq2_core(int NL, double *y, double dtg, chem_rhs rhs, const void *ctx, double *dt_first,
+                        double *trace, int cap)
 {
-    const int NL = c->NS + 1;
     double ymin = 1.0e-20, dtmin = 1.0e-7;
     double eps, epsmin = 1.0e-4, scrtch = 1e-25;
-    double tfd = 1.0 + 1.0e-10, sqreps = 0.05, epsmax = 1.0, epscl = 1.0e4;
+    const double sqreps = ORA_MUTANT == 2 ? 0.06 : 0.05;
+    const double epscl = ORA_MUTANT == 3 ? 1.0e3 : 1.0e4;
+    const double shrink = ORA_MUTANT == 4 ? 0.5 : 0.34;
+    const double grow = ORA_MUTANT == 5 ? 0.05 : 0.005;
+    double tfd = 1.0 + 1.0e-10, epsmax = 1.0;
     double ys[ORA_MAXNS + 1], y1[ORA_MAXNS + 1];
     double rtau[ORA_MAXNS + 1], scrarray[ORA_MAXNS + 1];
     double qs[ORA_MAXNS + 1], ds[ORA_MAXNS + 1], q[ORA_MAXNS + 1], d[ORA_MAXNS + 1];
     double dt = 0.0, ts, tn = 0.0;
     int passes = 0;
 
-    double sumy = 0.0;
-    for (int i = 1; i < NL; i++)
-        y[i] = ora_max(y[i], ymin), sumy += y[i];
-    sumy = 1.0 / sumy;
-    for (int i = 1; i < NL; i++)
-        y[i] *= sumy;
-
-    ora_evalcv(c, y, q, d, rho, m_p);
+    rhs(ctx, y, q, d);
     for (int i = 0; i < NL; i++) {
         const double ascr = fabs(q[i]);
         const double scr2 = xfrsign2(1.0 / y[i], 0.1 * epsmin * ascr - d[i]);
@@ -144,6 +165,7 @@ int ora_chemeq2(const ora_config *c, double *y, double dtg, double rho, double m
     }
     dt = ora_min(sqreps / scrtch, dtg);
     dtmin *= dt;
+    if (dt_first) *dt_first = dt;
 
     for (;;) {
         ts = tn;
@@ -163,7 +185,7 @@ int ora_chemeq2(const ora_config *c, double *y, double dtg, double rho, double m
         tn = ts + dt;
         for (int i = 0; i < NL; i++)
             y1[i] = y[i];
-        ora_evalcv(c, y, q, d, rho, m_p);
+        rhs(ctx, y, q, d);
         for (int i = 0; i < NL; i++) {
             const double rtaub = 0.5 * (rtau[i] + dt * d[i] / y[i]);
             const double alpha = alpha_of(rtaub);
@@ -181,6 +203,10 @@ int ora_chemeq2(const ora_config *c, double *y, double dtg, double rho, double m
             }
         }
         eps = eps * epscl;
+        if (trace && passes <= cap) {
+            double *t = trace + 4 * (passes - 1);
+            t[0] = dt, t[1] = eps, t[2] = epsmax, t[3] = tn;
+        }
         if (eps < epsmax) {
             if (dtg <= (tn * tfd))
                 return passes;
@@ -191,15 +217,67 @@ int ora_chemeq2(const ora_config *c, double *y, double dtg, double rho, double m
         double rteps = 0.5 * (eps + 1.0);
         rteps = 0.5 * (rteps + eps / rteps);
         rteps = 0.5 * (rteps + eps / rteps);
-        dt = ora_min(dt * (1.0 / rteps + 0.005), tfd * (dtg - tn));
+        dt = ora_min(dt * (1.0 / rteps + grow), tfd * (dtg - tn));
         if (eps > epsmax) {
-            dt = ora_min(dt, 0.34 * dto);
+            dt = ora_min(dt, shrink * dto);
             if (dt <= dtmin)
                 epsmax *= 10.0;
             goto apredictor;
         }
         epsmax = 1.0;
     }
+}
+
+struct mech_ctx {
+    const ora_config *c;
+    double rho, p;
+};
+static void mech_rhs(const void *v, const double *y, double *q, double *d)
+{
+    const struct mech_ctx *m = (const struct mech_ctx *)v;
+    ora_evalcv(m->c, y, q, d, m->rho, m->p);
+}
+
+/* Chemeq2<NS,NR> (y[0] = T form): ProductionRates.h:488-638: the mass
+   fractions floored at ymin and normalised, then the integration with the
+   mechanism's rates (evalcv).  Returns the predictor passes (accepted +
+   rejected substeps), or -1 when the substep bound is reached. */
+int ora_chemeq2(const ora_config *c, double *y, double dtg, double rho, double m_p)
+{
+    const int NL = c->NS + 1;
+    const double ymin = 1.0e-20;
+    double sumy = 0.0;
+    for (int i = 1; i < NL; i++)
+        y[i] = ora_max(y[i], ymin), sumy += y[i];
+    sumy = 1.0 / sumy;
+    for (int i = 1; i < NL; i++)
+        y[i] *= sumy;
+    struct mech_ctx m = {c, rho, m_p};
+    return chemeq2_core(NL, y, dtg, mech_rhs, &m, NULL, NULL, 0);
+}
+
+/* A model system for the pin tests: constant production q_i and loss
+   d_i = k_i * y_i^power (power 1: dy/dt = q - k y, exponential relaxation;
+   power 2 with q = 0: dy/dt = -k y^2), integrated by the same CHEMEQ2
+   body as the mechanism. */
+struct model_ctx {
+    const double *q, *k;
+    int NL, power;
+};
+static void model_rhs(const void *v, const double *y, double *q, double *d)
+{
+    const struct model_ctx *m = (const struct model_ctx *)v;
+    for (int i = 0; i < m->NL; i++) {
+        q[i] = m->q[i];
+        d[i] = m->power == 2 ? m->k[i] * y[i] * y[i] : m->k[i] * y[i];
+    }
+}
+int ora_chemeq2_model(int NL, double *y, double dtg, const double *q, const double *k, int power, double *dt_first,
+                      double *trace, int cap)
+{
+    if (NL < 1 || NL > ORA_MAXNS + 1) return -2;
+    struct model_ctx m = {q, k, NL, power};
+    return chemeq2_core(NL, y, dtg, model_rhs, &m, dt_first, trace, cap);
 }
 
 /* ChemeODEQ2SolverKernel1 over interior cells: Reaction_kernels.hpp:46-73,

@@ -23,10 +23,14 @@
  *   - conservation of sum(U) to round-off on periodic grids;
  *   - dt: brute-force max over the padded grid (numpy);
  *   - ghost fill: numpy restatement of each BC type.
- *   ReconstructSoundSpeed / SoundSpeedMultiSpecies (multi-species Roe c^2)
- *   is pinned in part (tests/test_oracle_recon.py): between equal states it
- *   is (gamma - 1)(q^2/2 + p/rho + c_p T - h), and it is symmetric in the two
- *   cells; the Roe weighting of the jump terms is "parity unpinned".
+ *   ReconstructSoundSpeed / SoundSpeedMultiSpecies (multi-species Roe c^2):
+ *   between equal states it is (gamma - 1)(q^2/2 + p/rho + c_p T - h), and it
+ *   is symmetric in the two cells (tests/test_oracle_recon.py); its
+ *   dp/drho, dp/drho_n and dp/d(rho E) equal central finite differences of
+ *   the equation of state through the Newton temperature, and on faces with
+ *   real jumps the corrected derivatives reproduce the pressure jump, with
+ *   the exact residual the energy term's rho_bar carries
+ *   (tests/test_oracle_pins.py; planted mistakes in either fail).
  */
 #include "ora_internal.h"
 #include <stdlib.h>
@@ -621,17 +625,23 @@ static double dpdrho(double hN, double RN, double q2, double Cp, double R, doubl
     double RNT = RN * T;
     return (gamma - 1.0) * (0.5 * q2 - hN + Cp * RNT / R);
 }
+#ifndef ORA_MUTANT
+#define ORA_MUTANT 0 /* > 0: a planted mistake for the pin tests' mutation checks */
+#endif
 static double dpdrhoi(double hin, double Rin, double hiN, double RiN, double T, double Cp, double R, double gamma)
 {
     double hN_minus_hi = -hin + hiN;
-    double Ri_minus_RN = (Rin - RiN);
+    double Ri_minus_RN = ORA_MUTANT == 7 ? (RiN - Rin) : (Rin - RiN);
     return (gamma - 1.0) * (hN_minus_hi + Cp * Ri_minus_RN * T / R);
 }
 
 /* ReconstructSoundSpeed + SoundSpeedMultiSpecies:
    positive-definite_eigen/Utils_device.hpp:42-79, 102-140 */
+/* terms (may be NULL) receives the intermediate quantities, for the pin
+   tests: see ora_face_c2_terms */
 static double reconstruct_c2(const ora_fluid *f, size_t id_l, size_t id_r, double D, double D1, double _rho,
-                             double _P, double *_yi, double *z, double *b1, double *b3, double *Gamma)
+                             double _P, double *_yi, double *z, double *b1, double *b3, double *Gamma,
+                             double *terms)
 {
     const ora_config *c = &f->cfg;
     const int NS = c->NS, NCOP = f->NCOP;
@@ -678,7 +688,7 @@ static double reconstruct_c2(const ora_fluid *f, size_t id_l, size_t id_r, doubl
         Sum_drhoi += drhoi[n] * drhoi[n];
         Sum_Yidpdrhoi += _yi[n] * _dpdrhoi[n];
     }
-    double _dpdE_new = _dpdE + _dpdE * _dpdE * de * _rho * temp;
+    double _dpdE_new = ORA_MUTANT == 6 ? _dpdE + _dpdE * _dpdE * de * temp : _dpdE + _dpdE * _dpdE * de * _rho * temp;
     double _dpdrho_new = _dpdrho + _dpdrho * _dpdrho * drho * temp;
     for (int n = 0; n < NCOP; n++)
         _dpdrhoi_new[n] = _dpdrhoi[n] + _dpdrhoi[n] * _dpdrhoi[n] * drhoi[n] * temp;
@@ -689,6 +699,17 @@ static double reconstruct_c2(const ora_fluid *f, size_t id_l, size_t id_r, doubl
         *b3 += _yi[n] * z[n];
     }
     *b3 *= *b1;
+    if (terms) {
+        const double t[14] = {_dpdrho, _dpde, _dpdE, dp, drho, de, _rho, _prho, temp1, temp, _dpdrho_new, _dpdE_new,
+                              csqr, (double)NCOP};
+        for (int q = 0; q < 14; q++) terms[q] = t[q];
+        for (int n = 0; n < NCOP; n++) {
+            terms[14 + n] = _dpdrhoi[n];
+            terms[14 + NCOP + n] = drhoi[n];
+            terms[14 + 2 * NCOP + n] = _dpdrhoi_new[n];
+            terms[14 + 3 * NCOP + n] = _yi[n];
+        }
+    }
     /* c2 fallback when c2 <= 0 (sycl::step(c2, 0)) */
     double c2 = csqr;
     double c2w = (c2 <= 0.0) ? 1.0 : 0.0;
@@ -705,10 +726,26 @@ double ora_face_c2(ora_fluid *f, size_t id_l, size_t id_r, double *b1_out, doubl
     double _P = (f->p[id_l] + D * f->p[id_r]) * D1;
     double _rho = sqrt(f->rho[id_r] * f->rho[id_l]);
     double _yi[ORA_MAXNS] = {0}, z[ORA_MAXNS] = {0}, b1 = 0.0, b3 = 0.0, Gamma0;
-    double c2 = reconstruct_c2(f, id_l, id_r, D, D1, _rho, _P, _yi, z, &b1, &b3, &Gamma0);
+    double c2 = reconstruct_c2(f, id_l, id_r, D, D1, _rho, _P, _yi, z, &b1, &b3, &Gamma0, NULL);
     if (b1_out) *b1_out = b1;
     if (b3_out) *b3_out = b3;
     return c2;
+}
+
+/* The same face, with the quantities of ReconstructSoundSpeed and
+   SoundSpeedMultiSpecies in terms[14 + 4 NCOP]: the Roe-averaged dp/drho,
+   dp/de, dp/dE, the jumps dp, drho, de, rho_bar, (p/rho)_bar, temp1, temp,
+   the corrected dp/drho and dp/dE, c^2 before the fallback, NCOP; then per
+   species n < NCOP the averaged dp/drho_n, the jump drho_n, the corrected
+   dp/drho_n and the averaged Y_n. */
+double ora_face_c2_terms(ora_fluid *f, size_t id_l, size_t id_r, double *terms)
+{
+    double D = sqrt(f->rho[id_r] / f->rho[id_l]);
+    double D1 = 1.0 / (D + 1.0);
+    double _P = (f->p[id_l] + D * f->p[id_r]) * D1;
+    double _rho = sqrt(f->rho[id_r] * f->rho[id_l]);
+    double _yi[ORA_MAXNS] = {0}, z[ORA_MAXNS] = {0}, b1 = 0.0, b3 = 0.0, Gamma0;
+    return reconstruct_c2(f, id_l, id_r, D, D1, _rho, _P, _yi, z, &b1, &b3, &Gamma0, terms);
 }
 
 /* One characteristic face flux, MARCO_FLUXWALL_WENO5 (EIGEN_ALLOC == 0):
@@ -732,7 +769,7 @@ static void face_flux(ora_fluid *f, int dir, size_t id_l, size_t id_r, long stri
     if (c->cop) {
         /* MARCO_COPC2: Eigen_callback.h:7-11 */
         Gamma0 = 1.4;
-        c2 = reconstruct_c2(f, id_l, id_r, D, D1, _rho, _P, _yi, z, &b1, &b3, &Gamma0);
+        c2 = reconstruct_c2(f, id_l, id_r, D, D1, _rho, _P, _yi, z, &b1, &b3, &Gamma0, NULL);
     } else {
         /* MARCO_NOCOPC2: Eigen_callback.h:87-91 */
         _yi[0] = 1.0;

@@ -149,6 +149,14 @@ void ora_eigen_lr(const ora_config *c, int dir, int n, double *eig_l, double *ei
                   double *eigen_value, const double *z, const double *yi, double c2,
                   double rho, double u, double v, double w, double H, double b1,
                   double b3);
+/* the face sound speed with its intermediate quantities (pin tests) */
+double ora_face_c2_terms(ora_fluid *f, size_t id_l, size_t id_r, double *terms);
+/* CHEMEQ2's alpha(r) and its integrator on a model system (pin tests):
+   constant production q, loss k y^power; trace (NULL ok): per pass the
+   substep, error measure * epscl, threshold and end time, cap passes */
+double ora_alpha(double r);
+int ora_chemeq2_model(int NL, double *y, double dtg, const double *q, const double *k, int power, double *dt_first,
+                      double *trace, int cap);
 /* production/loss rates and the CHEMEQ2 integrator on one cell */
 void ora_evalcv(const ora_config *c, const double *y, double *q, double *d,
                 double rho, double p);

@@ -294,11 +294,13 @@ int xf_autotune(xf_ctx *ctx, int reps, int *shape3, double *ms);
 /* ---- ensembles ------------------------------------------------------------
    Independent cases (any grid sizes; one species set, mechanism and thermo
    tables) advanced together.  One ensemble step is every case's own step
-   (dt, Strang Reaction, three RK stages, Reaction), with each case's stages on
-   its own stream and the chemistry of all cases in one work-queue launch at
-   each Reaction; xf_ensemble_steps(..., use_graph = 1) captures the whole
-   step as one hipGraph.  Every case's result is bit-identical to stepping
-   it alone.  The contexts stay owned by the caller and must outlive the
+   (dt, Strang Reaction, three RK stages, Reaction).  Cases that agree in
+   their kernel configuration run each kernel class once per stage over the
+   blocks of all cases (a device table maps each block to its case); other
+   ensembles run each case's stages on its own stream.  The chemistry of all
+   cases is one work-queue launch at each Reaction either way;
+   xf_ensemble_steps(..., use_graph = 1) captures the whole step as one
+   hipGraph.  Every case's result is bit-identical to stepping it alone.  The contexts stay owned by the caller and must outlive the
    ensemble; read results with xf_download_state on each.  No counterpart in
    the reference, which runs one case per process. */
 typedef struct xf_ensemble xf_ensemble;
@@ -307,6 +309,12 @@ int xf_ensemble_steps(xf_ensemble *ens, int nsteps, int use_graph);
 /* Waits for every case; returns the first case's physical error code. */
 int xf_ensemble_synchronize(xf_ensemble *ens);
 long long xf_ensemble_cells(const xf_ensemble *ens);
+/* 1 when each ensemble step is one launch per kernel class over the cells of
+   every case (the cases agree in species count, scheme, precision, active
+   directions and the limiter, and none is viscous; the tile shapes are the
+   first case's at creation), 0 when each case's stages run on its own
+   stream with one shared chemistry launch. */
+int xf_ensemble_one_launch(const xf_ensemble *ens);
 void xf_ensemble_destroy(xf_ensemble *ens);
 /* Evaluate the device exp and log (the fixed-sequence FP64 functions every
    kernel uses) on n host values; for bit-level checks against the CPU. */

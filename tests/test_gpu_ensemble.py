@@ -1,6 +1,7 @@
 """Ensembles: independent cases of different sizes stepped together (one
-graph launch per step, one chemistry launch for all cases) give every case
-the bits it gets when stepped alone."""
+graph launch per step, one chemistry launch for all cases; one launch per
+kernel class over all cases when their configuration agrees) give every case
+the bits it gets when stepped alone, in every precision form."""
 import numpy as np
 import pytest
 
@@ -15,17 +16,22 @@ VISCOUS = [dict(nx=64, ny=32, seed=1, viscous=(1, 1, 1)), dict(nx=48, ny=24, see
            dict(nx=40, ny=40, seed=3)]
 
 
+POSITIVE = [dict(nx=64, ny=32, seed=1, positivity=1), dict(nx=40, ny=24, seed=2, positivity=1)]
+THREE_D = [dict(nx=24, ny=16, nz=16, seed=1), dict(nx=16, ny=16, nz=8, seed=2), dict(nx=8, ny=8, nz=8, seed=3)]
+
+
 def _cases(case, specs):
     out = []
     for sp in specs:
-        s = xf.Setup(xf.case_path(case), nx=sp["nx"], ny=sp["ny"], viscous=sp.get("viscous"))
-        U, T = s.initial(case, seed=sp["seed"])
+        kw = {k: sp[k] for k in ("nx", "ny", "nz", "positivity") if k in sp}
+        s = xf.Setup(xf.case_path(case), viscous=sp.get("viscous"), **kw)
+        U, T = s.initial(case if case != "shock-bubble-3d" else "shock-bubble", seed=sp["seed"])
         out.append((s, U, T))
     return out
 
 
-def _alone(s, U, T, n):
-    ctx = s.context()
+def _alone(s, U, T, n, precision=xf.XF_PLAIN_FP64):
+    ctx = s.context(precision=precision)
     ctx.upload(U, T)
     ctx.initialize()
     ctx.steps(n, graph=True)
@@ -34,20 +40,23 @@ def _alone(s, U, T, n):
     return Ug, Tg
 
 
-@pytest.mark.parametrize("case,specs,nsteps", [("reactive-channel", REACTIVE, 3), ("shock-bubble", INERT, 4),
-                                                ("shock-bubble", VISCOUS, 3)])
+@pytest.mark.parametrize("case,specs,nsteps,one_launch", [
+    ("reactive-channel", REACTIVE, 3, True), ("shock-bubble", INERT, 4, True),
+    ("shock-bubble", VISCOUS, 3, False), ("shock-bubble", POSITIVE, 4, True), ("shock-bubble-3d", THREE_D, 3, True)])
 @pytest.mark.parametrize("graph", [True, False])
-def test_ensemble_matches_single_cases(case, specs, nsteps, graph):
-    announce("ensemble %s graph=%s" % (case, graph))
+@pytest.mark.parametrize("precision", [xf.XF_PLAIN_FP64, xf.XF_MIXED])
+def test_ensemble_matches_single_cases(case, specs, nsteps, one_launch, graph, precision):
+    announce("ensemble %s graph=%s precision=%d" % (case, graph, precision))
     cases = _cases(case, specs)
     ctxs = []
     for s, U, T in cases:
-        ctx = s.context()
+        ctx = s.context(precision=precision)
         ctx.upload(U, T)
         ctx.initialize()
         ctxs.append(ctx)
     ens = xf.Ensemble(ctxs)
     assert ens.cells == sum(s.desc.ninner for s, _, _ in cases)
+    assert ens.one_launch == one_launch
     ens.steps(nsteps, graph=graph)
     ens.synchronize()
     got = [c.download(0) for c in ctxs]
@@ -55,7 +64,7 @@ def test_ensemble_matches_single_cases(case, specs, nsteps, graph):
     for c in ctxs:
         c.close()
     for (s, U, T), (Ug, Tg) in zip(cases, got):
-        Ua, Ta = _alone(s, U, T, nsteps)
+        Ua, Ta = _alone(s, U, T, nsteps, precision)
         assert np.array_equal(Ua.view(np.uint64), Ug.view(np.uint64))
         assert np.array_equal(Ta.view(np.uint64), Tg.view(np.uint64))
 

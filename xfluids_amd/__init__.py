@@ -163,6 +163,7 @@ _SIGS = {
     "xf_ensemble_steps": (C.c_int, [_V, C.c_int, C.c_int]),
     "xf_ensemble_synchronize": (C.c_int, [_V]),
     "xf_ensemble_cells": (C.c_longlong, [_V]),
+    "xf_ensemble_one_launch": (C.c_int, [_V]),
     "xf_ensemble_destroy": (None, [_V]),
 }
 
@@ -549,6 +550,11 @@ class Ensemble:
         if st:
             raise XfError(st, "xf_ensemble_create")
         self._h = h
+
+    @property
+    def one_launch(self):
+        """True when each step is one launch per kernel class over all cases."""
+        return lib().xf_ensemble_one_launch(self._h) == 1
 
     @property
     def cells(self):

@@ -1022,6 +1022,12 @@ struct xf_ensemble {
     bool react = false, strang = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
+    // one launch per kernel class over all cases (the cases agree in species
+    // count, scheme, precision, dimensions and switches, and are inviscid)
+    bool multi = false;
+    EnsCase *d_ens = nullptr;
+    long long state_blocks = 0, tiles[3] = {0, 0, 0};
+    int shape[3] = {0, 0, 0};
 };
 
 namespace {
@@ -1047,10 +1053,50 @@ void ens_chem(xf_ensemble *e)
                        e->blocks, 0, 1, c0->mech_spec);
 }
 
+// one step of every case with one launch per kernel class: dt, [Reaction],
+// three stages of UpdateStates (with the ghost fill) and the sweeps (the
+// last fusing the RK update), [Reaction]; the launches of a single case's
+// step, each over the blocks of every case
+void ens_step_multi(xf_ensemble *e)
+{
+    xf_ctx *c0 = e->cases[0];
+    const int n = (int)e->cases.size();
+    hipStream_t st = e->stream;
+    hipLaunchKernelGGL(k_dt_multi, dim3(1), dim3(64), 0, st, e->d_ens, n);
+    auto reaction = [&]() {
+        c0->ops.states_multi(c0, st, e->d_ens, n, e->state_blocks, 0, 0, 4, 0);
+        hipLaunchKernelGGL(k_dt_chem_multi, dim3(1), dim3(64), 0, st, e->d_ens, n, c0->d.strang);
+        ens_chem(e);
+    };
+    if (e->react && e->strang) reaction();
+    int dirs[3], nd = 0;
+    if (c0->g.DimX) dirs[nd++] = 0;
+    if (c0->g.DimY) dirs[nd++] = 1;
+    if (c0->g.DimZ) dirs[nd++] = 2;
+    // stage 1: U -> U1; stage 2: (U, U1) -> U2; stage 3: (U, U2) -> U, as
+    // step_mid_async; at stage 1 the sweep's Ub is U2 (the NaN check's U1)
+    const int in_sel[3] = {0, 1, 2}, out_sel[3] = {1, 2, 0};
+    for (int s = 1; s <= 3; s++) {
+        c0->ops.states_multi(c0, st, e->d_ens, n, e->state_blocks, 0, 1, s, in_sel[s - 1]);
+        for (int q = 0; q < nd; q++) {
+            int mode = (q > 0) ? 1 : 0;
+            if (q == nd - 1) mode |= 2;
+            const int d = dirs[q];
+            c0->ops.sweep_multi(c0, st, e->d_ens, n, e->tiles[d], d, e->shape[d], mode, s, in_sel[s - 1], 0,
+                                s == 1 ? 2 : in_sel[s - 1], out_sel[s - 1]);
+        }
+    }
+    if (e->react) reaction();
+}
+
 // one step of every case: each case's stages on its own stream, the
 // chemistry of all cases in one work-queue launch at each Reaction
 void ens_step(xf_ensemble *e)
 {
+    if (e->multi) {
+        ens_step_multi(e);
+        return;
+    }
     ens_fork(e);
     for (xf_ctx *c : e->cases) step_pre_async(c);
     ens_join(e);
@@ -1119,6 +1165,46 @@ int xf_ensemble_create(xf_ctx *const *cases, int ncases, xf_ensemble **out)
     unsigned long long z[2] = {0, 0};
     if (hipMemcpy(e->d_queue, z, sizeof(z), hipMemcpyHostToDevice) != hipSuccess) return fail(XF_EHIP);
     if (e->react) e->blocks = c0->ops.multi_blocks(e->total, c0->mech_spec);
+    // the one-launch form: cases that agree in everything the launches are
+    // specialised on, with the template case's tile shapes
+    bool multi = !c0->d.visc;
+    for (int k = 0; k < ncases && multi; k++) {
+        const xf_case_desc &d = cases[k]->d, &d0 = c0->d;
+        multi = !d.visc && d.scheme == d0.scheme && d.precision == d0.precision && d.DimX == d0.DimX &&
+                d.DimY == d0.DimY && d.DimZ == d0.DimZ && d.positivity == d0.positivity &&
+                d.ncop_gamma == d0.ncop_gamma && cases[k]->ops.states == c0->ops.states &&
+                (!d.react || cases[k]->mech_spec == c0->mech_spec);
+    }
+    if (multi) {
+        std::vector<EnsCase> t(ncases);
+        for (int q = 0; q < 3; q++) e->shape[q] = c0->shape[q];
+        long long sb = 0, tb[3] = {0, 0, 0};
+        for (int k = 0; k < ncases; k++) {
+            xf_ctx *c = cases[k];
+            EnsCase &r = t[k];
+            r.g = c->g;
+            r.f = c->f;
+            r.U2 = c->U2;
+            r.state_off = sb;
+            sb += nblk(c->N);
+            int bcs = 0;
+            for (int q = 0; q < 6; q++) bcs |= (c->d.BC[q] & 15) << (4 * q);
+            r.bcs = bcs | (1 << 28);
+            for (int dd = 0; dd < 3; dd++) {
+                unsigned gr[3];
+                sweep_grid(c->g, dd, e->shape[dd], gr);
+                for (int q = 0; q < 3; q++) r.tiles[dd][q] = (int)gr[q];
+                r.tile_off[dd] = tb[dd];
+                tb[dd] += (long long)gr[0] * gr[1] * gr[2];
+            }
+        }
+        e->state_blocks = sb;
+        for (int q = 0; q < 3; q++) e->tiles[q] = tb[q];
+        if (hipMalloc(&e->d_ens, ncases * sizeof(EnsCase)) != hipSuccess ||
+            hipMemcpy(e->d_ens, t.data(), ncases * sizeof(EnsCase), hipMemcpyHostToDevice) != hipSuccess)
+            return fail(XF_EHIP);
+        e->multi = true;
+    }
     *out = e;
     return XF_OK;
 }
@@ -1156,6 +1242,8 @@ int xf_ensemble_synchronize(xf_ensemble *e)
 
 long long xf_ensemble_cells(const xf_ensemble *e) { return e ? e->total : 0; }
 
+int xf_ensemble_one_launch(const xf_ensemble *e) { return e ? (e->multi ? 1 : 0) : XF_EINVAL; }
+
 void xf_ensemble_destroy(xf_ensemble *e)
 {
     if (!e) return;
@@ -1166,6 +1254,7 @@ void xf_ensemble_destroy(xf_ensemble *e)
     for (auto ev : e->join)
         if (ev) (void)hipEventDestroy(ev);
     if (e->d_cases) (void)hipFree(e->d_cases);
+    if (e->d_ens) (void)hipFree(e->d_ens);
     if (e->d_queue) (void)hipFree(e->d_queue);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;

@@ -39,6 +39,13 @@ struct Ops {
     // the state Ug, then the wall flux of one direction (before its sweep)
     void (*visc_prep)(xf_ctx *, const double *Ug);
     void (*visc_flux)(xf_ctx *, int dir, const double *Ug);
+    // one launch over every case of an ensemble (case table on the device;
+    // sel codes pick U, U1 or U2 per case; the scheme, precision and tile
+    // shape are those of the template case c0)
+    void (*states_multi)(const xf_ctx *c0, hipStream_t, const EnsCase *, int ncases, long long blocks, int full,
+                         int fill, int where, int sel);
+    void (*sweep_multi)(const xf_ctx *c0, hipStream_t, const EnsCase *, int ncases, long long blocks, int dir, int shape,
+                        int mode, int stage, int sel_g, int sel_a, int sel_b, int sel_o);
 };
 
 // dispatch tables, one translation unit each
@@ -95,6 +102,28 @@ namespace xf {
 
 inline unsigned nblk(long long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
 
+// tile grid of a sweep in direction dir with tile shape `shape` (TS faces by
+// TW = 256 / TS pencils, the shapes of SweepShape): x tiles along the sweep
+// axis of x, pencil rows, third axis
+inline void sweep_grid(const Geom &g, int dir, int shape, unsigned out[3])
+{
+    const int TS = dir == 0 ? (shape == 0 ? 64 : 32) : (shape == 0 ? 16 : (shape == 1 ? 32 : 8));
+    const int TW = 256 / TS;
+    if (dir == 0) {
+        out[0] = (unsigned)((g.X_inner + TS - 2) / (TS - 1));
+        out[1] = (unsigned)((g.Y_inner * g.Z_inner + TW - 1) / TW);
+        out[2] = 1;
+    } else if (dir == 1) {
+        out[0] = (unsigned)((g.X_inner + TW - 1) / TW);
+        out[1] = (unsigned)((g.Y_inner + TS - 2) / (TS - 1));
+        out[2] = (unsigned)g.Z_inner;
+    } else {
+        out[0] = (unsigned)((g.X_inner + TW - 1) / TW);
+        out[1] = (unsigned)((g.Z_inner + TS - 2) / (TS - 1));
+        out[2] = (unsigned)g.Y_inner;
+    }
+}
+
 struct ProfScope {
     xf_ctx *c;
     const char *name;
@@ -131,15 +160,12 @@ template <int NS, bool COP> struct Launch {
     static void sweep_t(xf_ctx *c, const double *Ug, int mode, int stage, const double *Ua, const double *Ub, double *Uo)
     {
         constexpr int E = COP ? NS + 4 : 5;
-        using SS = SweepShape<DIR, E, SHAPE>;
-        const Geom &g = c->g;
-        dim3 grid;
-        if (DIR == 0)
-            grid = dim3((unsigned)((g.X_inner + SS::TS - 2) / (SS::TS - 1)), (unsigned)((g.Y_inner * g.Z_inner + SS::TW - 1) / SS::TW), 1);
-        else if (DIR == 1)
-            grid = dim3((unsigned)((g.X_inner + SS::TW - 1) / SS::TW), (unsigned)((g.Y_inner + SS::TS - 2) / (SS::TS - 1)), (unsigned)g.Z_inner);
-        else
-            grid = dim3((unsigned)((g.X_inner + SS::TW - 1) / SS::TW), (unsigned)((g.Z_inner + SS::TS - 2) / (SS::TS - 1)), (unsigned)g.Y_inner);
+        static_assert(SweepShape<DIR, E, SHAPE>::TS == (DIR == 0 ? (SHAPE == 0 ? 64 : 32)
+                                                                 : (SHAPE == 0 ? 16 : (SHAPE == 1 ? 32 : 8))),
+                      "sweep_grid's tile sizes");
+        unsigned gr[3];
+        sweep_grid(c->g, DIR, SHAPE, gr);
+        const dim3 grid(gr[0], gr[1], gr[2]);
         static const char *names[3] = {"sweep_x", "sweep_y", "sweep_z"};
         ProfScope ps(c, names[DIR]);
         hipLaunchKernelGGL((k_sweep<NS, COP, DIR, SCH, PREC, SHAPE>), grid, dim3(256), 0, c->stream, c->g, c->d_th,
@@ -299,9 +325,68 @@ template <int NS, bool COP> struct Launch {
             if (dir == 2) visc_flux_d<2>(c, Ug);
         }
     }
+    static void states_multi(const xf_ctx *c0, hipStream_t st, const EnsCase *cases, int ncases, long long blocks,
+                             int full, int fill, int where, int sel)
+    {
+        hipLaunchKernelGGL((k_states_multi<NS, COP>), dim3((unsigned)blocks), dim3(256), 0, st, cases, ncases, c0->d_th,
+                           c0->d.ncop_gamma, full, fill, where, sel);
+        hipLaunchKernelGGL(k_states_fold_multi, dim3((unsigned)(12 * ncases)), dim3(1024), 0, st, cases);
+    }
+    template <int DIR, int SCH, int PREC, int SHAPE>
+    static void sweep_multi_t(const xf_ctx *c0, hipStream_t st, const EnsCase *cases, int ncases, long long blocks,
+                              int mode, int stage, int sg, int sa, int sb, int so)
+    {
+        hipLaunchKernelGGL((k_sweep_multi<NS, COP, DIR, SCH, PREC, SHAPE>), dim3((unsigned)blocks), dim3(256), 0, st,
+                           cases, ncases, c0->d_th, c0->d.ncop_gamma, mode, stage, sg, sa, sb, so);
+    }
+    template <int DIR, int SCH, int PREC>
+    static void sweep_multi_s(const xf_ctx *c0, hipStream_t st, const EnsCase *cases, int ncases, long long blocks,
+                              int shape, int mode, int stage, int sg, int sa, int sb, int so)
+    {
+        switch (shape) {
+        case 1: sweep_multi_t<DIR, SCH, PREC, 1>(c0, st, cases, ncases, blocks, mode, stage, sg, sa, sb, so); break;
+        case 2:
+            if constexpr (SweepShapes<DIR>::count > 2) {
+                sweep_multi_t<DIR, SCH, PREC, 2>(c0, st, cases, ncases, blocks, mode, stage, sg, sa, sb, so);
+                break;
+            }
+            [[fallthrough]];
+        default: sweep_multi_t<DIR, SCH, PREC, 0>(c0, st, cases, ncases, blocks, mode, stage, sg, sa, sb, so);
+        }
+    }
+    template <int DIR, int PREC>
+    static void sweep_multi_p(const xf_ctx *c0, hipStream_t st, const EnsCase *cases, int ncases, long long blocks,
+                              int shape, int mode, int stage, int sg, int sa, int sb, int so)
+    {
+        if (c0->d.scheme == XF_WENO5Z)
+            sweep_multi_s<DIR, XF_WENO5Z, PREC>(c0, st, cases, ncases, blocks, shape, mode, stage, sg, sa, sb, so);
+        else if (c0->d.scheme == XF_WENOCU6)
+            sweep_multi_s<DIR, XF_WENOCU6, PREC>(c0, st, cases, ncases, blocks, shape, mode, stage, sg, sa, sb, so);
+        else
+            sweep_multi_s<DIR, XF_WENO5JS, PREC>(c0, st, cases, ncases, blocks, shape, mode, stage, sg, sa, sb, so);
+    }
+    template <int DIR>
+    static void sweep_multi_d(const xf_ctx *c0, hipStream_t st, const EnsCase *cases, int ncases, long long blocks,
+                              int shape, int mode, int stage, int sg, int sa, int sb, int so)
+    {
+        if (c0->d.precision == XF_MIXED)
+            sweep_multi_p<DIR, XF_MIXED>(c0, st, cases, ncases, blocks, shape, mode, stage, sg, sa, sb, so);
+        else if (c0->d.precision == XF_FUSED_FP64)
+            sweep_multi_p<DIR, XF_FUSED_FP64>(c0, st, cases, ncases, blocks, shape, mode, stage, sg, sa, sb, so);
+        else
+            sweep_multi_p<DIR, XF_PLAIN_FP64>(c0, st, cases, ncases, blocks, shape, mode, stage, sg, sa, sb, so);
+    }
+    static void sweep_multi(const xf_ctx *c0, hipStream_t st, const EnsCase *cases, int ncases, long long blocks,
+                            int dir, int shape, int mode, int stage, int sg, int sa, int sb, int so)
+    {
+        if (dir == 0) sweep_multi_d<0>(c0, st, cases, ncases, blocks, shape, mode, stage, sg, sa, sb, so);
+        if (dir == 1) sweep_multi_d<1>(c0, st, cases, ncases, blocks, shape, mode, stage, sg, sa, sb, so);
+        if (dir == 2) sweep_multi_d<2>(c0, st, cases, ncases, blocks, shape, mode, stage, sg, sa, sb, so);
+    }
     static Ops ops()
     {
-        return Ops{&states, &sweep, &chem, &prepare, &chem_multi, &multi_blocks, &visc_prep, &visc_flux};
+        return Ops{&states, &sweep, &chem, &prepare, &chem_multi, &multi_blocks, &visc_prep, &visc_flux,
+                   &states_multi, &sweep_multi};
     }
 };
 

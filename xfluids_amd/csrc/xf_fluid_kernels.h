@@ -194,16 +194,18 @@ static __global__ void __launch_bounds__(256) k_boundary_all(Geom g, double *__r
 // bit 28, and every ghost cell first takes its value from its source cell as
 // k_boundary_all forms it (FluidBoundaryCondition, fused into UpdateStates:
 // the sources are interior cells or Inflow ghosts, which no thread writes)
+// the 256 cells of block bid of one case
 template <int NS, bool COP>
-__global__ void __launch_bounds__(256) k_states(Geom g, const DevThermo *__restrict__ th, double *Ug, Fields f,
-                                                double ncop_gamma, int full, int bcs, int where)
+__device__ __forceinline__ void states_block(const Geom &g, const DevThermo *__restrict__ th, double *Ug,
+                                             const Fields &f, double ncop_gamma, int full, int bcs, int where,
+                                             long long bid)
 {
     constexpr int E = COP ? NS + 4 : 5;
     constexpr int NCOP = COP ? NS - 1 : 0;
     // block-uniform: the words tested are not written by this launch
     if (step_halted(f.err, ~(EBIT_RHO_Y | EBIT_PRIM))) return;
     const long long N = g.N;
-    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long id = bid * blockDim.x + threadIdx.x;
     const bool live = id < N;
     double glf[9], dtm[3];
 #pragma unroll
@@ -374,19 +376,37 @@ __global__ void __launch_bounds__(256) k_states(Geom g, const DevThermo *__restr
         double m = sred[0][threadIdx.x];
         const int nw = (blockDim.x + 63) >> 6;
         for (int q = 1; q < nw; q++) m = sred[q][threadIdx.x] > m ? sred[q][threadIdx.x] : m;
-        f.spart[(long long)blockIdx.x * 12 + threadIdx.x] = m;
+        f.spart[bid * 12 + threadIdx.x] = m;
     }
     if (threadIdx.x == 0 && serr) report_pass(f.err, serr, where);
+}
+
+template <int NS, bool COP>
+__global__ void __launch_bounds__(256) k_states(Geom g, const DevThermo *__restrict__ th, double *Ug, Fields f,
+                                                double ncop_gamma, int full, int bcs, int where)
+{
+    states_block<NS, COP>(g, th, Ug, f, ncop_gamma, full, bcs, where, (long long)blockIdx.x);
+}
+
+// UpdateStates of every case of an ensemble in one launch (fill = 1: with
+// each case's ghost fill); sel picks the state (0 U, 1 U1, 2 U2)
+template <int NS, bool COP>
+__global__ void __launch_bounds__(256) k_states_multi(const EnsCase *__restrict__ cases, int ncases,
+                                                      const DevThermo *__restrict__ th, double ncop_gamma, int full,
+                                                      int fill, int where, int sel)
+{
+    const long long b = blockIdx.x;
+    const int k = ens_find(cases, ncases, b, 3);
+    const EnsCase &c = cases[k];
+    states_block<NS, COP>(c.g, th, c.buf(sel), c.f, ncop_gamma, full, fill ? c.bcs : 0, where, b - c.state_off);
 }
 
 // the per-block GLF and CFL maxima of k_states into the reduction slots: one
 // block per slot (12), each the maximum of +0 and every positive block value
 // (the pass's maxima, combined with the 0 the pass starts from)
-static __global__ void __launch_bounds__(1024) k_states_fold(const double *__restrict__ spart, int nblocks, double *red,
-                                                             int *err)
+__device__ __forceinline__ void states_fold(const double *__restrict__ spart, int nblocks, double *red, int *err, int q)
 {
     __shared__ unsigned long long s[1024];
-    const int q = blockIdx.x;
     // latch a failure of the UpdateStates pass (or of any earlier one)
     if (q == 0 && threadIdx.x == 0 && err[ERR_BITS]) err[ERR_HALT] = 1;
     unsigned long long v = 0ULL; // +0.0
@@ -404,10 +424,22 @@ static __global__ void __launch_bounds__(1024) k_states_fold(const double *__res
     if (threadIdx.x == 0) red[q < 9 ? RED_GLFP + q : RED_DT + q - 9] = __longlong_as_double((long long)s[0]);
 }
 
+static __global__ void __launch_bounds__(1024) k_states_fold(const double *__restrict__ spart, int nblocks, double *red,
+                                                             int *err)
+{
+    states_fold(spart, nblocks, red, err, blockIdx.x);
+}
+
+// the folds of every case: 12 blocks per case
+static __global__ void __launch_bounds__(1024) k_states_fold_multi(const EnsCase *__restrict__ cases)
+{
+    const EnsCase &c = cases[blockIdx.x / 12];
+    states_fold(c.f.spart, (int)((c.g.N + 255) / 256), c.f.red, c.f.err, blockIdx.x % 12);
+}
+
 // ---------------------------------------------------------------------------
 // small control kernels
 // ---------------------------------------------------------------------------
-// zero the per-pass maxima (pending GLF, CFL) before an UpdateStates pass
 // GetDt: dt = CFL / sum_d max_d / Delta_d, written to device memory.
 // step = 1 is the head of a time step (XFLUIDS::Evolution,
 // src/XFLUIDS.cpp:172-198): nothing happens once the step is halted or the
@@ -416,40 +448,56 @@ static __global__ void __launch_bounds__(1024) k_states_fold(const double *__res
 // that the clock lands on the target (physicalTime + dt > target_t ->
 // dt = target_t - physicalTime) and the clock advances.  It also latches a
 // failure of the previous pass into ERR_HALT.  step = 0 is GetDt alone.
+__device__ __forceinline__ void dt_case(const Geom &g, double *red, double *dt, int *err, int step)
+{
+    if (step) {
+        if (err[ERR_BITS]) err[ERR_HALT] = 1;
+        if (err[ERR_HALT]) return;
+        if (!(dt[DT_TIME] < dt[DT_TARGET])) {
+            err[ERR_DONE] = 1;
+            err[ERR_HALT] = 1;
+            dt[DT_STEP] = 0.0; // no step taken
+            return;
+        }
+        err[ERR_COUNT] = err[ERR_COUNT] + 1;
+    }
+    const double mx = g.DimX ? red[RED_DT + 0] : 0.0;
+    const double my = g.DimY ? red[RED_DT + 1] : 0.0;
+    const double mz = g.DimZ ? red[RED_DT + 2] : 0.0;
+    const double dtref = mx * g._dx + my * g._dy + mz * g._dz;
+    double d = g.CFL / dtref;
+    if (step) {
+        const double t = dt[DT_TIME], te = dt[DT_TARGET];
+        if (t + d > te) d = te - t;
+        dt[DT_TIME] = t + d;
+    }
+    dt[DT_STEP] = d;
+    // uvw_c_max as GetDt leaves it (the positivity limiter's lambda_0)
+    red[RED_LAM + 0] = mx, red[RED_LAM + 1] = my, red[RED_LAM + 2] = mz;
+}
+
 static __global__ void k_dt(Geom g, double *red, double *dt, int *err, int step)
 {
-    if (threadIdx.x == 0) {
-        if (step) {
-            if (err[ERR_BITS]) err[ERR_HALT] = 1;
-            if (err[ERR_HALT]) return;
-            if (!(dt[DT_TIME] < dt[DT_TARGET])) {
-                err[ERR_DONE] = 1;
-                err[ERR_HALT] = 1;
-                dt[DT_STEP] = 0.0; // no step taken
-                return;
-            }
-            err[ERR_COUNT] = err[ERR_COUNT] + 1;
-        }
-        const double mx = g.DimX ? red[RED_DT + 0] : 0.0;
-        const double my = g.DimY ? red[RED_DT + 1] : 0.0;
-        const double mz = g.DimZ ? red[RED_DT + 2] : 0.0;
-        const double dtref = mx * g._dx + my * g._dy + mz * g._dz;
-        double d = g.CFL / dtref;
-        if (step) {
-            const double t = dt[DT_TIME], te = dt[DT_TARGET];
-            if (t + d > te) d = te - t;
-            dt[DT_TIME] = t + d;
-        }
-        dt[DT_STEP] = d;
-        // uvw_c_max as GetDt leaves it (the positivity limiter's lambda_0)
-        red[RED_LAM + 0] = mx, red[RED_LAM + 1] = my, red[RED_LAM + 2] = mz;
-    }
+    if (threadIdx.x == 0) dt_case(g, red, dt, err, step);
 }
 
 // Strang half step on device: dt[1] = dt[0] * 0.5 (or dt[0] for Lie)
 static __global__ void k_dt_chem(double *dt, int strang, const int *err)
 {
     if (threadIdx.x == 0 && !step_halted(err, ~0)) dt[DT_CHEM] = strang ? dt[DT_STEP] * 0.5 : dt[DT_STEP];
+}
+
+// the same two for every case of an ensemble, one lane per case
+static __global__ void k_dt_multi(const EnsCase *__restrict__ cases, int ncases)
+{
+    for (int k = threadIdx.x; k < ncases; k += blockDim.x) dt_case(cases[k].g, cases[k].f.red, cases[k].f.dt, cases[k].f.err, 1);
+}
+static __global__ void k_dt_chem_multi(const EnsCase *__restrict__ cases, int ncases, int strang)
+{
+    for (int k = threadIdx.x; k < ncases; k += blockDim.x) {
+        const Fields &f = cases[k].f;
+        if (!step_halted(f.err, ~0)) f.dt[DT_CHEM] = strang ? f.dt[DT_STEP] * 0.5 : f.dt[DT_STEP];
+    }
 }
 
 // ---------------------------------------------------------------------------

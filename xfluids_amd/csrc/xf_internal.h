@@ -81,6 +81,36 @@ struct Fields {
     double *spart; // per-block GLF and CFL maxima of k_states (12 per block)
 };
 
+// One case of an ensemble as the ensemble's launches see it: geometry,
+// fields, the stage-2 buffer, the case's first block in a states launch and
+// in a sweep launch of each direction, its tile grid per direction and its
+// packed ghost-fill codes (bit 28 set, as k_states takes them).
+struct EnsCase {
+    Geom g;
+    Fields f;
+    double *U2;
+    long long state_off;
+    long long tile_off[3];
+    int tiles[3][3];
+    int bcs;
+    // 0 = U, 1 = U1, 2 = U2
+    __device__ __forceinline__ double *buf(int sel) const { return sel == 0 ? f.U : (sel == 1 ? f.U1 : U2); }
+};
+
+// the case whose block range holds block b of a states launch (kind 3) or
+// of a sweep launch in direction kind (cases in ascending block order)
+__device__ __forceinline__ int ens_find(const EnsCase *c, int n, long long b, int kind)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        const long long o = kind < 3 ? c[mid].tile_off[kind] : c[mid].state_off;
+        if (o <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 // reduction slot layout (doubles, compared as uint64 bit patterns)
 // GLF running maxima per direction d: RED_GLF + 3*d + {0: |u-c|, 1: |u|, 2: |u+c|}
 //   (never reset: the reference zeroes eigen_block once, src/Fluids.cpp:427,

@@ -484,11 +484,13 @@ template <int NS, int PREC> struct SweepOccupancy {
     static constexpr int waves = NS == 1 ? 4 : (NS <= 3 ? (PREC == XF_MIXED ? 4 : 3) : 2);
 };
 
+// One tile of TS faces x TW pencils of one direction: (tx, ty, tz) is the
+// tile, in units of tiles along x, the pencil rows and the third axis.
 template <int NS, bool COP, int DIR, int SCHEME, int PREC, int SHAPE>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SweepOccupancy<NS, PREC>::waves)))
-k_sweep(Geom g, const DevThermo *__restrict__ th, const double *__restrict__ Ug,
-                                               Fields f, double ncop_gamma, int mode, int stage, const double *Ua,
-                                               const double *Ub, double *Uo)
+__device__ __forceinline__ void sweep_tile(const Geom &g, const DevThermo *__restrict__ th,
+                                           const double *__restrict__ Ug, const Fields &f, double ncop_gamma,
+                                           int mode, int stage, const double *Ua, const double *Ub, double *Uo,
+                                           int tx, int ty, int tz)
 {
     constexpr int E = COP ? NS + 4 : 5;
     constexpr int NCOP = COP ? NS - 1 : 0;
@@ -511,8 +513,6 @@ k_sweep(Geom g, const DevThermo *__restrict__ th, const double *__restrict__ Ug,
     const int B = DIR == 0 ? g.Bx : (DIR == 1 ? g.By : g.Bz);
     const int inner = DIR == 0 ? g.X_inner : (DIR == 1 ? g.Y_inner : g.Z_inner);
     const int Lmax = DIR == 0 ? g.Xmax : (DIR == 1 ? g.Ymax : g.Zmax);
-    int tx, ty, tz;
-    xcd_tile(tx, ty, tz);
     const int c0 = B + (TS - 1) * (DIR == 0 ? tx : ty);
     const long long stride = DIR == 0 ? 1LL : (DIR == 1 ? (long long)g.Xmax : (long long)g.Xmax * g.Ymax);
     const double *vel_n = DIR == 0 ? f.u : (DIR == 1 ? f.v : f.w);
@@ -902,6 +902,35 @@ k_sweep(Geom g, const DevThermo *__restrict__ th, const double *__restrict__ Ug,
             report_pass(f.err, EBIT_NAN, stage);
         }
     }
+}
+
+// tiles of one case: grid (tiles along the sweep's x, rows, third axis), in
+// the XCD-aware order
+template <int NS, bool COP, int DIR, int SCHEME, int PREC, int SHAPE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SweepOccupancy<NS, PREC>::waves)))
+k_sweep(Geom g, const DevThermo *__restrict__ th, const double *__restrict__ Ug, Fields f, double ncop_gamma, int mode,
+        int stage, const double *Ua, const double *Ub, double *Uo)
+{
+    int tx, ty, tz;
+    xcd_tile(tx, ty, tz);
+    sweep_tile<NS, COP, DIR, SCHEME, PREC, SHAPE>(g, th, Ug, f, ncop_gamma, mode, stage, Ua, Ub, Uo, tx, ty, tz);
+}
+
+// the tiles of every case of an ensemble in one launch: block b belongs to
+// the case whose block range [off, off + count) holds it; the stage's
+// buffers are picked per case by code (0 U, 1 U1, 2 U2)
+template <int NS, bool COP, int DIR, int SCHEME, int PREC, int SHAPE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SweepOccupancy<NS, PREC>::waves)))
+k_sweep_multi(const EnsCase *__restrict__ cases, int ncases, const DevThermo *__restrict__ th, double ncop_gamma,
+              int mode, int stage, int sel_g, int sel_a, int sel_b, int sel_o)
+{
+    const long long b = blockIdx.x;
+    const int k = ens_find(cases, ncases, b, DIR);
+    const EnsCase &c = cases[k];
+    const int l = (int)(b - c.tile_off[DIR]);
+    const int gx = c.tiles[DIR][0], gy = c.tiles[DIR][1];
+    sweep_tile<NS, COP, DIR, SCHEME, PREC, SHAPE>(c.g, th, c.buf(sel_g), c.f, ncop_gamma, mode, stage, c.buf(sel_a),
+                                                  c.buf(sel_b), c.buf(sel_o), l % gx, (l / gx) % gy, l / (gx * gy));
 }
 
 } // namespace xf

@@ -103,3 +103,23 @@ def test_clock_restarts_from_checkpoint_header():
     o.set_target_time(1.25)
     st, _ = o.steps(2)
     assert st == 0 and o.time() == (1.25, 40, True)
+
+
+def test_shock_tube_fixture_is_post_reflection():
+    """The committed 170 us state of the 1024-cell tube (made by the oracle,
+    tests/golden/make_shock_tube_fixture.py): the reflected shock has heated
+    the gas at the left wall to about 1200 K, and the oracle continues from
+    it without error."""
+    import os
+    import xfluids_amd as xf
+    z = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "shock_tube_1024_170us.npz"))
+    s = xf.Setup(xf.case_path("reactive-shock-tube"), nx=int(z["nx"]))
+    d = s.desc
+    T = z["T"].reshape(-1)[d.Bwidth_X:d.Bwidth_X + d.X_inner]
+    assert abs(float(z["time"]) - 170e-6) < 1e-12
+    assert 1150.0 < T[:64].max() < 1300.0      # hot gas behind the reflected shock
+    o = oracle_for(s, z["U"], z["T"])
+    assert o.initialize() == 0
+    o.set_time(float(z["time"]), int(z["steps"]))
+    st, _ = o.steps(2)
+    assert st == 0 and o.time()[1] == int(z["steps"]) + 2

@@ -136,6 +136,17 @@ int xf_mech_specialization_index(int NS, int NR, const int *Nu_f, const int *Nu_
                                  const double *Rargus);
 int xf_set_chem_specialization(xf_ctx *ctx, int on);
 int xf_chem_specialization(const xf_ctx *ctx);
+/* Queue chemistry (XF_CHEM_QUEUE): a cell still running after `passes`
+   CHEMEQ2 passes is handed from its lane to a whole wave, which spreads each
+   rate evaluation over its 64 lanes (species and reactions in parallel; the
+   same operations on every value, bit-identical results).  A few very stiff
+   cells then no longer keep single lanes busy long after the rest.  At most
+   65 536 cells per launch are handed over (the rest stay on their lanes);
+   passes = 0 turns it off.  Ensembles use their first case's setting.  No
+   counterpart in the reference (one work-item per cell).
+   Currently switched off: its first GPU run did not finish and the cause has
+   not been found, so the default is 0 and any passes > 0 returns XF_EINVAL. */
+int xf_set_chem_handover(xf_ctx *ctx, int passes);
 
 /* Transport fits in the reference's layout, each a cubic in ln T:
    fitted_coefficients_visc[NS][4] (ln mu_k, Pa s), fitted_coefficients_therm[NS][4]

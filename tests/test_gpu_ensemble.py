@@ -69,6 +69,34 @@ def test_ensemble_matches_single_cases(case, specs, nsteps, one_launch, graph, p
         assert np.array_equal(Ta.view(np.uint64), Tg.view(np.uint64))
 
 
+@pytest.mark.skip(reason="whole-wave chemistry hand-over switched off: its first GPU run did not finish and "
+                         "the cause has not been shown")
+@pytest.mark.parametrize("after", [1, 8])
+def test_ensemble_handover_matches_single_cases(after):
+    """The shared chemistry queue of an ensemble hands stiff cells of any
+    case to whole waves: every case keeps its single-run bits."""
+    announce("ensemble chemistry hand-over after %d passes" % after)
+    cases = _cases("reactive-channel", REACTIVE)
+    ctxs = []
+    for s, U, T in cases:
+        ctx = s.context()
+        ctx.upload(U, T)
+        ctx.initialize()
+        ctxs.append(ctx)
+    ctxs[0].set_chem_handover(after)
+    ens = xf.Ensemble(ctxs)
+    ens.steps(3, graph=True)
+    ens.synchronize()
+    got = [c.download(0) for c in ctxs]
+    ens.close()
+    for c in ctxs:
+        c.close()
+    for (s, U, T), (Ug, Tg) in zip(cases, got):
+        Ua, Ta = _alone(s, U, T, 3)
+        assert np.array_equal(Ua.view(np.uint64), Ug.view(np.uint64))
+        assert np.array_equal(Ta.view(np.uint64), Tg.view(np.uint64))
+
+
 def test_ensemble_rejects_mixed_species():
     a = xf.Setup(xf.case_path("reactive-channel"), nx=8, ny=8).context()
     b = xf.Setup(xf.case_path("shock-bubble"), nx=8, ny=8).context()

@@ -103,6 +103,7 @@ _SIGS = {
     "xf_mech_specialization_index": (C.c_int, [C.c_int, C.c_int, _IP, _IP, _DP, _DP]),
     "xf_set_chem_specialization": (C.c_int, [_V, C.c_int]),
     "xf_chem_specialization": (C.c_int, [_V]),
+    "xf_set_chem_handover": (C.c_int, [_V, C.c_int]),
     "xf_upload_state": (C.c_int, [_V, C.c_int, _DP, _DP]),
     "xf_download_state": (C.c_int, [_V, C.c_int, _DP, _DP]),
     "xf_download_prims": (C.c_int, [_V] + [_DP] * 10),
@@ -476,6 +477,11 @@ class Context:
         tables match one (default), or the generic table loop
         (xf_set_chem_specialization)."""
         self._chk(lib().xf_set_chem_specialization(self._h, 1 if on else 0), "xf_set_chem_specialization")
+
+    def set_chem_handover(self, passes):
+        """Queue chemistry: hand a cell to a whole wave after `passes` passes
+        (0: never).  Switched off in this build: only 0 is accepted."""
+        self._chk(lib().xf_set_chem_handover(self._h, passes), "xf_set_chem_handover")
 
     def chem_specialization(self):
         """Index of the compiled mechanism in use, or -1 (xf_chem_specialization)."""

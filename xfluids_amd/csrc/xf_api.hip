@@ -13,6 +13,13 @@
 
 using namespace xf;
 
+// The whole-wave hand-over of stiff chemistry cells (k_chem_coop): off.  Its
+// one GPU run (reactive channel 24 x 24, 2e-7 s, hand-over after 8 passes)
+// did not finish within 120 s, and reading the queue, the hand-over list and
+// the whole-wave loop has not shown why; until a cause is found no context
+// allocates the list and xf_set_chem_handover accepts only 0.
+constexpr bool CHEM_HANDOVER_ON = false;
+
 namespace {
 
 bool select_ops(int NS, int cop, Ops &o)
@@ -321,6 +328,11 @@ int xf_create(const xf_case_desc *desc, xf_ctx **out)
         return fail(XF_EHIP);
     unsigned qz[4] = {0, 0, 0, 0};
     if (hipMemcpy(f.queue, qz, sizeof(qz), hipMemcpyHostToDevice)) return fail(XF_EHIP);
+    // the queue chemistry's hand-over list (cells that go on as whole waves)
+    if (CHEM_HANDOVER_ON && d.cop && d.react && c->ops.spill_bytes() > 0) {
+        c->spill_cap = (int)std::min<long long>(c->Ninner, 65536);
+        if (hipMalloc(&c->spill, (size_t)c->spill_cap * c->ops.spill_bytes())) return fail(XF_EHIP);
+    }
     if (d.visc) {
         const size_t nv[6] = {n1, n1, (size_t)d.NS * n1, 9 * n1, nE,
                               (size_t)nblk(c->N) * 2 * d.NS * sizeof(double)};
@@ -346,7 +358,7 @@ void xf_destroy(xf_ctx *c)
     void *ptrs[] = {f.U,   f.U1,    f.LU,       c->U2,   f.rho,   f.u,     f.v,   f.w,   f.p,   f.H,
                     f.T,   f.c,     f.gamma,    f.e,     f.Cp,    f.gcv,   f.R,   f.hi,  f.red, f.dt,
                     f.err, f.substeps, f.queue, c->d_th, c->d_mc, c->d_tr, f.mu,  f.kap, f.Dkm, f.Vde,
-                    f.Fv,  f.ypart, f.spart};
+                    f.Fv,  f.ypart, f.spart, c->spill};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -492,6 +504,23 @@ int xf_set_chem_specialization(xf_ctx *c, int on)
 }
 
 int xf_chem_specialization(const xf_ctx *c) { return c ? c->mech_spec : -1; }
+
+int xf_set_chem_handover(xf_ctx *c, int passes)
+{
+    if (!c || passes < 0) return XF_EINVAL;
+    if (passes > 0 && !CHEM_HANDOVER_ON) {
+        fprintf(stderr, "xfluids_amd: the whole-wave chemistry hand-over is switched off in this build (its GPU "
+                        "run did not finish and the cause has not been found); only passes = 0 is accepted\n");
+        return XF_EINVAL;
+    }
+    if (c->spill_after != passes) {
+        c->spill_after = passes;
+        // a captured step graph holds the old launch arguments
+        if (c->gexec) (void)hipGraphExecDestroy(c->gexec), c->gexec = nullptr;
+        if (c->graph) (void)hipGraphDestroy(c->graph), c->graph = nullptr;
+    }
+    return XF_OK;
+}
 
 static double *which_ptr(xf_ctx *c, int which)
 {
@@ -1026,6 +1055,8 @@ struct xf_ensemble {
     // count, scheme, precision, dimensions and switches, and are inviscid)
     bool multi = false;
     EnsCase *d_ens = nullptr;
+    void *spill = nullptr; // hand-over list of the shared chemistry queue
+    int spill_cap = 0;
     long long state_blocks = 0, tiles[3] = {0, 0, 0};
     int shape[3] = {0, 0, 0};
 };
@@ -1050,7 +1081,7 @@ void ens_chem(xf_ensemble *e)
 {
     xf_ctx *c0 = e->cases[0];
     c0->ops.chem_multi(e->stream, e->d_cases, (int)e->cases.size(), e->total, c0->d_th, c0->d_mc, e->d_queue,
-                       e->blocks, 0, 1, c0->mech_spec);
+                       e->blocks, 0, 1, c0->mech_spec, e->spill, e->spill_cap, c0->spill_after);
 }
 
 // one step of every case with one launch per kernel class: dt, [Reaction],
@@ -1164,7 +1195,14 @@ int xf_ensemble_create(xf_ctx *const *cases, int ncases, xf_ensemble **out)
         return fail(XF_EHIP);
     unsigned long long z[2] = {0, 0};
     if (hipMemcpy(e->d_queue, z, sizeof(z), hipMemcpyHostToDevice) != hipSuccess) return fail(XF_EHIP);
-    if (e->react) e->blocks = c0->ops.multi_blocks(e->total, c0->mech_spec);
+    if (e->react) {
+        e->blocks = c0->ops.multi_blocks(e->total, c0->mech_spec);
+        if (CHEM_HANDOVER_ON) {
+            e->spill_cap = (int)std::min<long long>(e->total, 65536);
+            if (hipMalloc(&e->spill, (size_t)e->spill_cap * c0->ops.spill_bytes()) != hipSuccess)
+                return fail(XF_EHIP);
+        }
+    }
     // the one-launch form: cases that agree in everything the launches are
     // specialised on, with the template case's tile shapes
     bool multi = !c0->d.visc;
@@ -1255,6 +1293,7 @@ void xf_ensemble_destroy(xf_ensemble *e)
         if (ev) (void)hipEventDestroy(ev);
     if (e->d_cases) (void)hipFree(e->d_cases);
     if (e->d_ens) (void)hipFree(e->d_ens);
+    if (e->spill) (void)hipFree(e->spill);
     if (e->d_queue) (void)hipFree(e->d_queue);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;

@@ -54,8 +54,8 @@ struct RxnCtx {
 // skips exactly the zero-coefficient terms (each adds +0 or multiplies by 1 in
 // the reference, so skipping changes no bit).
 template <int NS, class Eff>
-__device__ __forceinline__ void rxn_step(const DevRxn R, Eff eff, const RxnCtx &c, const double *gk,
-                                         const double *m_cm, double *qi, double *pi)
+__device__ __forceinline__ void rxn_rates(const DevRxn R, Eff eff, const RxnCtx &c, const double *gk,
+                                          const double *m_cm, double &Kf_out, double &Kck_out)
 {
     const double A = R.A, B = R.B, EaR = R.EaR;
     const unsigned md = R.md, mt = R.mt, mf = R.mf, mb = R.mb;
@@ -105,6 +105,14 @@ __device__ __forceinline__ void rxn_step(const DevRxn R, Eff eff, const RxnCtx &
 #pragma unroll
     for (int it = 0; it < NS; it++)
         if ((mb >> it) & 1u) Kck *= pown_i(m_cm[it], (int)((pb >> (4 * it)) & 15ull));
+    Kf_out = Kf, Kck_out = Kck;
+}
+
+// the reaction's terms of the production and loss rates of every species
+template <int NS>
+__device__ __forceinline__ void rxn_accum(unsigned mf, unsigned mb, unsigned long long pf, unsigned long long pb,
+                                          double Kf, double Kck, double *qi, double *pi)
+{
 #pragma unroll
     for (int n = 0; n < NS; n++) {
         if (((mf | mb) >> n) & 1u) {
@@ -113,6 +121,15 @@ __device__ __forceinline__ void rxn_step(const DevRxn R, Eff eff, const RxnCtx &
             pi[n] += nf * Kf + nb * Kck;
         }
     }
+}
+
+template <int NS, class Eff>
+__device__ __forceinline__ void rxn_step(const DevRxn R, Eff eff, const RxnCtx &c, const double *gk,
+                                         const double *m_cm, double *qi, double *pi)
+{
+    double Kf, Kck;
+    rxn_rates<NS>(R, eff, c, gk, m_cm, Kf, Kck);
+    rxn_accum<NS>(R.mf, R.mb, R.pf, R.pb, Kf, Kck, qi, pi);
 }
 
 // Reaction r of compiled mechanism K: its record as a compile-time constant
@@ -198,6 +215,72 @@ __device__ __forceinline__ void evalcv_dev(const DevThermo *__restrict__ th, con
     double cv;
     if (uni) cv = mix_cv<NS>(th, y + 1, tu);
     else cv = mix_cv<NS>(th, y + 1, t);
+    const double _Cv = 1.0 / cv;
+    q[0] = q0 * _Cv, d[0] = d0 * _Cv;
+}
+
+// evalcv with the work of one cell spread over the 64 lanes of a wave, every
+// input uniform across the wave and every output left uniform: lane n < NS
+// evaluates species n's enthalpy and Gibbs energy, lane r < NR reaction r's
+// forward and reverse rates; both are broadcast, and every lane then sums
+// the species' production and loss over the reactions in increasing order
+// and finishes the temperature terms exactly as evalcv_dev.  The operations
+// on every value are evalcv_dev's, so the results are its bits (the table
+// form of the rates, which equals every compiled mechanism's).
+template <int NS>
+__device__ __forceinline__ void evalcv_coop(const DevThermo *__restrict__ th, const DevMech *__restrict__ mc,
+                                            const double *y, double *q, double *d, double m_dens, double m_p)
+{
+    const int ln = threadIdx.x & 63;
+    const double m_tmp = y[0];
+    const double _dens = 1.0 / m_dens;
+    const TPre t = th->thermo ? tpre<true>(m_tmp, 1) : tpre<true>(m_tmp, 0);
+    double h1[NS], gk[NS];
+    {
+        double h = 0.0, gv = 0.0;
+        if (ln < NS) {
+            h = h_k(th, t, 1.0, ln);
+            double temp = 0.0;
+            temp += s_k(th, t, 1.0, ln);
+            temp += (-h / m_tmp);
+            gv = temp;
+        }
+#pragma unroll
+        for (int n = 0; n < NS; n++) h1[n] = __shfl(h, n), gk[n] = __shfl(gv, n);
+    }
+    const double logStandConc = dlog(m_p / (R0 * m_tmp));
+    const double logT = dlog(m_tmp);
+    const double logP = dlog(m_p * (1.0 / 101325.0));
+    double m_cm[NS];
+#pragma unroll
+    for (int n = 0; n < NS; n++) m_cm[n] = y[1 + n] * th->_Wi[n] * m_dens * 1.0E-3;
+    RxnCtx c;
+    c.m_tmp = m_tmp, c.logT = logT, c.logP = logP, c.logStandConc = logStandConc;
+    c.rT = 1.0 / m_tmp;
+    c.t_ok = m_tmp >= 0x1p-500 && m_tmp <= 0x1p500;
+    const int NR = mc->NR;
+    double Kf = 0.0, Kck = 0.0;
+    if (ln < NR) {
+        const DevRxn R = mc->rx[ln];
+        const double *er = mc->third + ln * NS;
+        rxn_rates<NS>(R, [er](int it) { return er[it]; }, c, gk, m_cm, Kf, Kck);
+    }
+    double *qi = q + 1, *pi = d + 1;
+#pragma unroll
+    for (int n = 0; n < NS; n++) qi[n] = 0.0, pi[n] = 0.0;
+    for (int r = 0; r < NR; r++) {
+        const DevRxn &R = mc->rx[r];
+        rxn_accum<NS>(R.mf, R.mb, R.pf, R.pb, __shfl(Kf, r), __shfl(Kck, r), qi, pi);
+    }
+    double q0 = 0.0, d0 = 0.0;
+#pragma unroll
+    for (int n = 0; n < NS; n++) {
+        qi[n] *= th->Wi[n] * _dens;
+        pi[n] *= th->Wi[n] * _dens;
+        const double ei = (h1[n] - m_tmp) * th->Ri[n];
+        q0 -= qi[n] * ei, d0 -= pi[n] * ei;
+    }
+    const double cv = mix_cv<NS>(th, y + 1, t);
     const double _Cv = 1.0 / cv;
     q[0] = q0 * _Cv, d[0] = d0 * _Cv;
 }
@@ -349,7 +432,58 @@ template <int NS, class St = ChemRegs<NS + 1>, int SPEC = -1> struct ChemLane {
         evalcv_dev<NS, SPEC>(th, mc, y, q, d, rho, m_p);
         return after();
     }
+    // the same advance by a whole wave on one cell (state uniform across
+    // the wave): the rate evaluation spread over the lanes
+    __device__ __forceinline__ int advance_coop(const DevThermo *__restrict__ th, const DevMech *__restrict__ mc)
+    {
+        if (before() < 0) return -1;
+        evalcv_coop<NS>(th, mc, y, q, d, rho, m_p);
+        return after();
+    }
 };
+
+// Everything of a cell's CHEMEQ2 run between two passes, so that a cell the
+// queue lanes hand over continues on a whole wave (k_chem_coop) from exactly
+// where it stopped: y, q, d, ys, qs, ds, rtau, the step state, the start
+// energy, the cell and its case.
+template <int NS> struct ChemSpill {
+    static constexpr int NL = NS + 1;
+    static constexpr int NV = 7 * NL + 9;
+    double v[NV];
+    long long id, tl;
+    int passes, started, k, pad;
+};
+
+template <int NS, class St, int SPEC>
+__device__ __forceinline__ void chem_spill_save(ChemSpill<NS> &r, ChemLane<NS, St, SPEC> &L, double e0, long long id,
+                                                long long tl, int k)
+{
+    constexpr int NL = NS + 1;
+#pragma unroll
+    for (int i = 0; i < NL; i++) {
+        r.v[i] = L.y[i], r.v[NL + i] = L.q[i], r.v[2 * NL + i] = L.d[i];
+        r.v[3 * NL + i] = L.ys(i), r.v[4 * NL + i] = L.qs(i), r.v[5 * NL + i] = L.ds(i), r.v[6 * NL + i] = L.rtau(i);
+    }
+    double *w = r.v + 7 * NL;
+    w[0] = L.dt, w[1] = L.dtg, w[2] = L.dtmin, w[3] = L.epsmax, w[4] = L.ts, w[5] = L.tn, w[6] = L.rho, w[7] = L.m_p;
+    w[8] = e0;
+    r.id = id, r.tl = tl, r.passes = L.passes, r.started = L.started ? 1 : 0, r.k = k;
+}
+
+template <int NS, class St, int SPEC>
+__device__ __forceinline__ double chem_spill_load(const ChemSpill<NS> &r, ChemLane<NS, St, SPEC> &L)
+{
+    constexpr int NL = NS + 1;
+#pragma unroll
+    for (int i = 0; i < NL; i++) {
+        L.y[i] = r.v[i], L.q[i] = r.v[NL + i], L.d[i] = r.v[2 * NL + i];
+        L.ys(i) = r.v[3 * NL + i], L.qs(i) = r.v[4 * NL + i], L.ds(i) = r.v[5 * NL + i], L.rtau(i) = r.v[6 * NL + i];
+    }
+    const double *w = r.v + 7 * NL;
+    L.dt = w[0], L.dtg = w[1], L.dtmin = w[2], L.epsmax = w[3], L.ts = w[4], L.tn = w[5], L.rho = w[6], L.m_p = w[7];
+    L.passes = r.passes, L.started = r.started != 0;
+    return w[8];
+}
 
 // Chemeq2 on one cell.  Returns predictor passes, or -1 at the safety bound.
 template <int NS, int SPEC>
@@ -468,6 +602,41 @@ struct ChemCase {
     int *substeps, *err;
 };
 
+// Write-back of a finished cell (ChemeODEQ2SolverKernel1's tail,
+// Reaction_kernels.hpp:60-73): Ns-1 partial densities, T, the energy
+// change; the pass count; the pass bound and post-Reaction NaN check.
+template <int NS, class St, int SPEC>
+__device__ __forceinline__ void chem_store(const DevThermo *__restrict__ th, const ChemCase &cc, long long id,
+                                           long long tl, ChemLane<NS, St, SPEC> &L, double e0, int st, int record,
+                                           int check)
+{
+    const long long N = cc.g.N;
+    const double rho = L.rho;
+    const double e1 = mix_e<NS>(th, L.y + 1, L.y[0]);
+#pragma unroll
+    for (int n = 0; n < NS - 1; n++) cc.U[(5 + n) * N + id] = L.y[1 + n] * rho;
+    cc.T[id] = L.y[0];
+    cc.U[4 * N + id] += (e1 - e0) * rho;
+    if (record) cc.substeps[tl] = st < 0 ? -1 : L.passes;
+    int bits = st < 0 ? EBIT_CHEM : 0;
+    if (st < 0) report_cell(cc.err, EBIT_CHEM, id);
+    if (check && cell_bad(cc.U, N, id, NS + 4)) {
+        report_cell(cc.err, EBIT_NAN, id);
+        bits |= EBIT_NAN;
+    }
+    if (bits) report_pass(cc.err, bits, 4);
+}
+
+// Cells that the queue's lanes hand to whole waves: records, their count
+// (count[0], may exceed cap: the lanes that found the list full kept their
+// cells) and the waves' pick counter (count[1]); a cell is handed over once
+// it has run `after` passes.  cap = 0: no hand-over.
+struct ChemSpillBuf {
+    void *rec;
+    unsigned *count;
+    int cap, after;
+};
+
 // Work-queue driver: persistent lanes draining the interior cells of one or
 // several cases.  Cells of very different stiffness take from 1 to ~10^3
 // passes; with one lane per cell a wave runs as long as its stiffest cell
@@ -475,22 +644,29 @@ struct ChemCase {
 // the next one from a device counter (one atomic per wave, handed out by lane
 // rank), so the lanes of a wave stay busy with the same code until the queue
 // is empty; a cell that starts and the cells that continue share one rate
-// evaluation per iteration.  Each cell's arithmetic is exactly that of the
-// per-cell driver.  Termination: the counter only grows, a wave stops once
-// every lane has drawn an index >= total and holds no cell, and a cell is
-// bounded by CHEM_MAXSUB passes.  The counter must be zero at launch.
+// evaluation per iteration.  A cell still running after sp.after passes is
+// handed to the whole-wave kernel (k_chem_coop) while the list has
+// room: a few very stiff cells would otherwise keep single lanes busy, one
+// serial rate evaluation after another, long after the rest.  Each cell's
+// arithmetic is exactly that of the per-cell driver.  Termination: the
+// counter only grows, a wave stops once every lane has drawn an index >=
+// total and holds no cell, and a cell is bounded by CHEM_MAXSUB passes.  The
+// counters must be zero at launch.
 template <int NS, int SPEC, class Cases>
 __device__ __forceinline__ void chem_queue_run(const Cases &cases, long long total, const DevThermo *__restrict__ th,
                                                const DevMech *__restrict__ mc, unsigned long long *queue, int record,
-                                               int check, double *lds)
+                                               int check, double *lds, ChemSpillBuf sp)
 {
     const int lane = threadIdx.x & 63;
     ChemLane<NS, ChemLds<NS + 1>, SPEC> L;
     L.st.p = lds + lane;
-    ChemCase cc{};
+    // a lane keeps only its cell's case index, id and interior index across
+    // passes; the case record is read again where it is needed (for an
+    // ensemble it differs per lane, and holding it would cost ~40 registers)
     long long tl = -1, id = 0;
+    int ck = 0;
     double e0 = 0.0;
-    bool active = false, done = false;
+    bool active = false, done = false, can_spill = sp.cap > 0;
     for (;;) {
         const bool need = !active && !done;
         const unsigned long long want = __ballot(need);
@@ -503,7 +679,11 @@ __device__ __forceinline__ void chem_queue_run(const Cases &cases, long long tot
             if (need) {
                 const int rank = __popcll(want & ((1ull << lane) - 1ull));
                 const long long t = (long long)(base + (unsigned long long)rank);
-                if (t < total) cc = cases.find(t);
+                ChemCase cc{};
+                if (t < total) {
+                    ck = cases.find_k(t);
+                    cc = cases.get(ck);
+                }
                 // the cells of a case whose step has halted are skipped
                 if (t < total && !step_halted(cc.err, ~EBIT_CHEM)) {
                     tl = t - cc.off;
@@ -531,37 +711,57 @@ __device__ __forceinline__ void chem_queue_run(const Cases &cases, long long tot
             // starting cells and running cells share this one evaluation
             const int st = L.advance(th, mc);
             if (st != 0) {
-                const long long N = cc.g.N;
-                const double rho = L.rho;
-                const double e1 = mix_e<NS>(th, L.y + 1, L.y[0]);
-#pragma unroll
-                for (int n = 0; n < NS - 1; n++) cc.U[(5 + n) * N + id] = L.y[1 + n] * rho;
-                cc.T[id] = L.y[0];
-                cc.U[4 * N + id] += (e1 - e0) * rho;
-                if (record) cc.substeps[tl] = st < 0 ? -1 : L.passes;
-                int bits = st < 0 ? EBIT_CHEM : 0;
-                if (st < 0) report_cell(cc.err, EBIT_CHEM, id);
-                if (check && cell_bad(cc.U, N, id, NS + 4)) {
-                    report_cell(cc.err, EBIT_NAN, id);
-                    bits |= EBIT_NAN;
-                }
-                if (bits) report_pass(cc.err, bits, 4);
+                chem_store<NS>(th, cases.get(ck), id, tl, L, e0, st, record, check);
                 active = false;
+            } else if (can_spill && L.passes >= sp.after) {
+                const unsigned idx = atomicAdd(sp.count, 1u);
+                if (idx < (unsigned)sp.cap) {
+                    chem_spill_save(((ChemSpill<NS> *)sp.rec)[idx], L, e0, id, tl, ck);
+                    active = false;
+                } else {
+                    can_spill = false; // the list is full: keep this cell here
+                }
             }
         }
     }
 }
 
+// The cells the queue handed over, one whole wave per cell (evalcv_coop):
+// persistent one-wave blocks take records from the list until it is
+// drained, and write each cell back as the queue would have.
+template <int NS, class Cases>
+__device__ __forceinline__ void chem_coop_run(const Cases &cases, const DevThermo *__restrict__ th,
+                                              const DevMech *__restrict__ mc, ChemSpillBuf sp, int record, int check)
+{
+    const int ln = threadIdx.x & 63;
+    const ChemSpill<NS> *rec = (const ChemSpill<NS> *)sp.rec;
+    const unsigned n = min(sp.count[0], (unsigned)sp.cap);
+    for (;;) {
+        unsigned idx = 0;
+        if (ln == 0) idx = atomicAdd(sp.count + 1, 1u);
+        idx = __shfl(idx, 0);
+        if (idx >= n) break;
+        ChemLane<NS, ChemRegs<NS + 1>, -1> L;
+        const double e0 = chem_spill_load(rec[idx], L);
+        const ChemCase cc = cases.get(rec[idx].k);
+        int st;
+        while ((st = L.advance_coop(th, mc)) == 0) {
+        }
+        if (ln == 0) chem_store<NS>(th, cc, rec[idx].id, rec[idx].tl, L, e0, st, record, check);
+    }
+}
+
 struct OneCase {
     ChemCase c;
-    __device__ __forceinline__ ChemCase find(long long) const { return c; }
+    __device__ __forceinline__ int find_k(long long) const { return 0; }
+    __device__ __forceinline__ ChemCase get(int) const { return c; }
 };
 
 // cases[k].off ascending; the case holding cell t by binary search
 struct ManyCases {
     const ChemCase *cases;
     int n;
-    __device__ __forceinline__ ChemCase find(long long t) const
+    __device__ __forceinline__ int find_k(long long t) const
     {
         int lo = 0, hi = n - 1;
         while (lo < hi) {
@@ -569,8 +769,9 @@ struct ManyCases {
             if (cases[mid].off <= t) lo = mid;
             else hi = mid - 1;
         }
-        return cases[lo];
+        return lo;
     }
+    __device__ __forceinline__ ChemCase get(int k) const { return cases[k]; }
 };
 
 // two waves per SIMD for the 9-species state (<= 256 registers per lane)
@@ -584,30 +785,60 @@ template <int NS> struct ChemQueueLds {
     static constexpr int doubles = 4 * (NS + 1) * 64;
 };
 
-template <int NS, int SPEC>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ChemOccupancy<NS>::waves)))
-k_chem_queue(Geom g, const DevThermo *__restrict__ th, const DevMech *__restrict__ mc, Fields f, int record,
-             int check)
+__device__ __forceinline__ OneCase one_case(const Geom &g, const Fields &f)
 {
     OneCase one;
     one.c.off = 0;
     one.c.g = g;
     one.c.U = f.U, one.c.T = f.T, one.c.p = f.p, one.c.dt = f.dt;
     one.c.substeps = f.substeps, one.c.err = f.err;
+    return one;
+}
+
+// the queue counter is queue[0..1] (64 bits); the hand-over counters follow
+// (queue[2] records, queue[3] waves' picks)
+template <int NS, int SPEC>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ChemOccupancy<NS>::waves)))
+k_chem_queue(Geom g, const DevThermo *__restrict__ th, const DevMech *__restrict__ mc, Fields f, int record,
+             int check, void *spill, int spill_cap, int spill_after)
+{
+    const OneCase one = one_case(g, f);
     __shared__ double lds[ChemQueueLds<NS>::doubles];
+    const ChemSpillBuf sp{spill, f.queue + 2, spill_cap, spill_after};
     chem_queue_run<NS, SPEC>(one, (long long)g.X_inner * g.Y_inner * g.Z_inner, th, mc, (unsigned long long *)f.queue,
-                             record, check, lds);
+                             record, check, lds, sp);
 }
 
 // one queue over the cells of every case of an ensemble (shared mechanism)
 template <int NS, int SPEC>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ChemOccupancy<NS>::waves)))
 k_chem_queue_multi(const ChemCase *__restrict__ cases, int ncases, long long total, const DevThermo *__restrict__ th,
-                   const DevMech *__restrict__ mc, unsigned long long *queue, int record, int check)
+                   const DevMech *__restrict__ mc, unsigned long long *queue, int record, int check, void *spill,
+                   int spill_cap, int spill_after)
 {
     ManyCases many{cases, ncases};
     __shared__ double lds[ChemQueueLds<NS>::doubles];
-    chem_queue_run<NS, SPEC>(many, total, th, mc, queue, record, check, lds);
+    const ChemSpillBuf sp{spill, (unsigned *)queue + 2, spill_cap, spill_after};
+    chem_queue_run<NS, SPEC>(many, total, th, mc, queue, record, check, lds, sp);
+}
+
+// the handed-over cells of one case / of an ensemble, one wave per cell
+template <int NS>
+__global__ void __launch_bounds__(64) k_chem_coop(Geom g, const DevThermo *__restrict__ th,
+                                                  const DevMech *__restrict__ mc, Fields f, int record, int check,
+                                                  void *spill, int spill_cap)
+{
+    const OneCase one = one_case(g, f);
+    chem_coop_run<NS>(one, th, mc, ChemSpillBuf{spill, f.queue + 2, spill_cap, 0}, record, check);
+}
+template <int NS>
+__global__ void __launch_bounds__(64) k_chem_coop_multi(const ChemCase *__restrict__ cases, int ncases,
+                                                        const DevThermo *__restrict__ th,
+                                                        const DevMech *__restrict__ mc, unsigned long long *queue,
+                                                        int record, int check, void *spill, int spill_cap)
+{
+    ManyCases many{cases, ncases};
+    chem_coop_run<NS>(many, th, mc, ChemSpillBuf{spill, (unsigned *)queue + 2, spill_cap, 0}, record, check);
 }
 
 // zero the work-queue counter (a kernel, so the step graph holds no memset)

@@ -33,7 +33,10 @@ struct Ops {
     // shared queue over the cells of several cases (ensembles)
     // spec: compiled mechanism whose records the tables hold, or -1
     void (*chem_multi)(hipStream_t, const ChemCase *, int ncases, long long total, const DevThermo *,
-                       const DevMech *, unsigned long long *queue, int blocks, int record, int check, int spec);
+                       const DevMech *, unsigned long long *queue, int blocks, int record, int check, int spec,
+                       void *spill, int spill_cap, int spill_after);
+    // bytes of one hand-over record (ChemSpill<NS>)
+    size_t (*spill_bytes)();
     int (*multi_blocks)(long long total, int spec);
     // viscous terms of GetLU: transport coefficients and centre gradients of
     // the state Ug, then the wall flux of one direction (before its sweep)
@@ -73,6 +76,12 @@ struct xf_ctx {
     Ops ops{};
     bool has_ops = false;
     int chem_blocks = 0; // resident one-wave blocks of the work-queue chemistry kernel
+    // cells the queue hands to whole waves after spill_after passes (at most
+    // spill_cap of them per launch; 0 = never).  Switched off: the one run of
+    // the hand-over on the GPU did not finish and no cause has been shown, so
+    // xf_set_chem_handover refuses any other value and nothing is allocated.
+    void *spill = nullptr;
+    int spill_cap = 0, spill_after = 0;
     // compiled mechanism the device tables match (MechSpec<mech_spec>), or -1;
     // spec_allowed = false keeps the generic loop (xf_set_chem_specialization)
     int mech_spec = -1;
@@ -230,9 +239,13 @@ template <int NS, bool COP> struct Launch {
             auto launch = [&](auto K) {
                 constexpr int S = decltype(K)::value;
                 if (c->d.chem_driver == XF_CHEM_QUEUE) {
+                    const int cap = c->spill_after > 0 ? c->spill_cap : 0;
                     hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, c->stream, c->f.queue);
                     hipLaunchKernelGGL((k_chem_queue<NS, S>), dim3(c->chem_blocks), dim3(64), 0, c->stream, c->g,
-                                       c->d_th, c->d_mc, c->f, record, check);
+                                       c->d_th, c->d_mc, c->f, record, check, c->spill, cap, c->spill_after);
+                    if (cap > 0)
+                        hipLaunchKernelGGL((k_chem_coop<NS>), dim3(coop_blocks(cap)), dim3(64), 0, c->stream, c->g,
+                                           c->d_th, c->d_mc, c->f, record, check, c->spill, cap);
                 } else {
                     hipLaunchKernelGGL((k_chem<NS, S>), dim3(nblk(c->Ninner)), dim3(256), 0, c->stream, c->g,
                                        c->d_th, c->d_mc, c->f, record, check);
@@ -261,18 +274,27 @@ template <int NS, bool COP> struct Launch {
         }
     }
     static void chem_multi(hipStream_t st, const ChemCase *cases, int ncases, long long total, const DevThermo *th,
-                           const DevMech *mc, unsigned long long *queue, int blocks, int record, int check, int spec)
+                           const DevMech *mc, unsigned long long *queue, int blocks, int record, int check, int spec,
+                           void *spill, int spill_cap, int spill_after)
     {
         if constexpr (COP) {
+            const int cap = spill_after > 0 ? spill_cap : 0;
             auto launch = [&](auto K) {
                 constexpr int S = decltype(K)::value;
                 hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, st, (unsigned int *)queue);
                 hipLaunchKernelGGL((k_chem_queue_multi<NS, S>), dim3(blocks), dim3(64), 0, st, cases, ncases, total,
-                                   th, mc, queue, record, check);
+                                   th, mc, queue, record, check, spill, cap, spill_after);
             };
             spec_switch<0>(spec, launch);
+            if (cap > 0)
+                hipLaunchKernelGGL((k_chem_coop_multi<NS>), dim3(coop_blocks(cap)), dim3(64), 0, st, cases, ncases, th,
+                                   mc, queue, record, check, spill, cap);
         }
     }
+    // persistent one-wave blocks of the hand-over kernel: a wave per cell
+    // while the list lasts, no more than 2048
+    static unsigned coop_blocks(int cap) { return (unsigned)std::min(cap, 2048); }
+    static size_t spill_bytes() { return COP ? sizeof(ChemSpill<NS>) : 0; }
     static int multi_blocks(long long total, int spec)
     {
         int per_cu = 0, ncu = 0, dev = 0;
@@ -385,8 +407,8 @@ template <int NS, bool COP> struct Launch {
     }
     static Ops ops()
     {
-        return Ops{&states, &sweep, &chem, &prepare, &chem_multi, &multi_blocks, &visc_prep, &visc_flux,
-                   &states_multi, &sweep_multi};
+        return Ops{&states, &sweep, &chem, &prepare, &chem_multi, &spill_bytes, &multi_blocks, &visc_prep,
+                   &visc_flux, &states_multi, &sweep_multi};
     }
 };
 

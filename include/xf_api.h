@@ -93,6 +93,9 @@ typedef struct xf_case_desc {
     int visc_diffu;     /* Visc_Diffu: species diffusion (CMakeLists.txt:57) */
     double Yil_limiter; /* BlSz.Yil_limiter and BlSz.Dim_limiter */
     double Dim_limiter; /*   (src/read_ini/src/iniset.cpp:358-359) */
+    int ghost;          /* GhostSpecies: the last species is a ghost with Y = 0; UpdateStates
+                           renormalises the others and writes them back into the state
+                           (Getrhoyi, src/solver_UpdateStates/Update_device.hpp:15-21) */
 } xf_case_desc;
 
 typedef struct xf_ctx xf_ctx;

@@ -98,12 +98,23 @@ double *ora_field(ora_fluid *f, const char *name)
 /* UpdateStates                                                       */
 /* ------------------------------------------------------------------ */
 
-/* Getrhoyi (non-GhostSpecies branch): src/solver_UpdateStates/Update_device.hpp:7-29 */
-static void getrhoyi(const ora_fluid *f, const double *UI, double *rho, double *yi)
+/* Getrhoyi: src/solver_UpdateStates/Update_device.hpp:7-29.  Default: the
+   last species is the remainder 1 - sum.  GhostSpecies (:15-21): the last
+   species is a ghost with Y = 0, the others are renormalised to sum 1 and
+   written back into the state. */
+static void getrhoyi(const ora_fluid *f, double *UI, double *rho, double *yi)
 {
     *rho = UI[0];
     double rho1 = 1.0 / *rho;
-    if (f->cfg.cop) {
+    if (f->cfg.cop && f->cfg.ghost) {
+        yi[f->NCOP] = 0.0;
+        double sum_yi = 0.0;
+        for (int ii = 0; ii < f->NCOP; ii++)
+            yi[ii] = UI[ii + 5] * rho1, sum_yi += yi[ii];
+        sum_yi = 1.0 / sum_yi;
+        for (int ii = 0; ii < f->NCOP; ii++)
+            yi[ii] *= sum_yi, UI[ii + 5] = *rho * yi[ii];
+    } else if (f->cfg.cop) {
         yi[f->NCOP] = 1.0;
         for (int ii = 5; ii < f->E; ii++)
             yi[ii - 5] = UI[ii] * rho1, yi[f->NCOP] += -yi[ii - 5];

@@ -88,6 +88,10 @@ typedef struct {
        1: its multiply-add pairs fused, the arithmetic of the fused FP64
        device form */
     int fused;
+    /* GhostSpecies (a mixture's case_setup.h, e.g. runtime.dat/Inert-SBI):
+       the last species is a ghost with Y = 0 and Getrhoyi renormalises the
+       others, writing them back into the state */
+    int ghost;
 } ora_config;
 
 typedef struct ora_fluid ora_fluid;

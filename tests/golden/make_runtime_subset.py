@@ -6,7 +6,9 @@ the build container.  Copies the numeric blocks of the species it needs from
 thermal_dynamics.dat (NASA-9) and thermal_dynamics_janaf.dat (NASA-7), the
 H2/O2 19-reaction mechanisms and the transport tables (transport_data.dat,
 collision_integral.dat), and writes the mixture lists of this project's own
-workloads (air/H2 shock-bubble, H2/N2 vs air mixing layer).
+workloads (air/H2 shock-bubble, H2/N2 vs air mixing layer) and copies the
+reference's own shock-bubble mixtures (Inert-SBI, Inert-SBI-without-fuel,
+RSBI-19REA) and the 21-reaction H2O-N2 mechanism.
 
     python tests/golden/make_runtime_subset.py [reference_runtime_dat]
 """
@@ -15,7 +17,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 OUT = os.path.join(os.path.dirname(os.path.dirname(HERE)), "xfluids_amd", "data", "runtime.dat")
-SPECIES = ["NCOP", "H", "O", "H2O", "OH", "O2", "H2", "H2O2", "HO2", "N2", "AR"]
+SPECIES = ["NCOP", "H", "O", "H2O", "OH", "O2", "H2", "H2O2", "HO2", "N2", "AR", "Xe"]
 
 
 def blocks(path):
@@ -59,6 +61,14 @@ def copy_mech(src_root, name):
             fo.write(fi.read())
 
 
+def copy_mixture(src_root, name):
+    d = os.path.join(OUT, name)
+    os.makedirs(d, exist_ok=True)
+    with open(os.path.join(src_root, name, "species_list.dat")) as fi, \
+            open(os.path.join(d, "species_list.dat"), "w") as fo:
+        fo.write(fi.read())
+
+
 def copy_root_file(src_root, fn):
     with open(os.path.join(src_root, fn)) as fi, open(os.path.join(OUT, fn), "w") as fo:
         fo.write(fi.read())
@@ -90,6 +100,18 @@ def main():
     # mixing layer: air outside (upper stream), H2:N2 = 1:1 inside (lower stream)
     write_mixture("Reaction/H2O-N2_19_reaction", ["H2", "O2", "O", "H", "OH", "HO2", "H2O2", "H2O", "N2"],
                   [0, 21, 0, 0, 0, 0, 0, 0, 79], [1, 0, 0, 0, 0, 0, 0, 0, 1])
+    # the reference's own shock-bubble mixtures, whose last species is a
+    # GhostSpecies copy of N2: Inert-SBI (H2/O2/Xe bubble in N2, NS = 5),
+    # Inert-SBI-without-fuel (O2/Xe bubble, NS = 4), and the reacting
+    # RSBI-19REA (NS = 11, 19 reactions, Xe in place of AR)
+    copy_mixture(src, "Inert-SBI")
+    copy_mixture(src, "Inert-SBI-without-fuel")
+    copy_mech(src, "RSBI-19REA")
+    # H2O-N2_21_reaction (NS = 10 with the ghost N2, 21 reactions): the same
+    # non-premixed streams as the 19-reaction mixing layer
+    copy_mech(src, "H2O-N2_21_reaction")
+    write_mixture("Reaction/H2O-N2_21_reaction", ["H2", "O2", "O", "H", "OH", "HO2", "H2O2", "H2O", "N2", "N2"],
+                  [0, 21, 0, 0, 0, 0, 0, 0, 79, 0], [1, 0, 0, 0, 0, 0, 0, 0, 1, 0])
     print("wrote", OUT)
 
 

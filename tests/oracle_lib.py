@@ -35,7 +35,7 @@ class OraConfig(C.Structure):
                 ("visc", C.c_int), ("visc_heat", C.c_int), ("visc_diffu", C.c_int),
                 ("Yil_limiter", C.c_double), ("Dim_limiter", C.c_double),
                 ("fcv", C.c_double * (MAXNS * 4)), ("fct", C.c_double * (MAXNS * 4)),
-                ("Dkj", C.c_double * (MAXNS * MAXNS * 4)), ("fused", C.c_int)]
+                ("Dkj", C.c_double * (MAXNS * MAXNS * 4)), ("fused", C.c_int), ("ghost", C.c_int)]
 
 
 _lib = None

@@ -36,6 +36,15 @@ CASES = {
     "mixing_layer_3d_cu6": ("mixing-layer", "mixing-layer", dict(nx=16, ny=16, nz=16, scheme=2), 2),
     "shock_bubble_cu6_pp_visc": ("shock-bubble", "shock-bubble",
                                  dict(nx=96, ny=48, scheme=2, positivity=1, viscous=(1, 1, 1)), 6),
+    # the reference's own shock-bubble mixtures, GhostSpecies (the last N2
+    # carries no mass; Getrhoyi renormalises and writes back): Inert-SBI
+    # (NS = 5, WENO-CU6, limiter, viscous, 3-D as cmake/init_sample.cmake
+    # configures it), without fuel (NS = 4, WENO5-Z, 2-D), the reacting
+    # RSBI-19REA (NS = 11), and the 21-reaction H2O-N2 mixing layer (NS = 10)
+    "inert_sbi_3d": ("inert-sbi", "shock-bubble", dict(nx=32, ny=16, nz=16), 4),
+    "inert_sbi_no_fuel": ("inert-sbi-without-fuel", "shock-bubble", dict(nx=64, ny=32), 6),
+    "rsbi": ("rsbi", "shock-bubble", dict(nx=64, ny=32), 4),
+    "mixing_layer_21": ("mixing-layer-21", "mixing-layer", dict(nx=16, ny=16, nz=16), 2),
 }
 
 

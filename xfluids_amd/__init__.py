@@ -48,7 +48,7 @@ class CaseDesc(C.Structure):
                 ("scheme", C.c_int), ("react", C.c_int), ("strang", C.c_int),
                 ("ncop_gamma", C.c_double), ("precision", C.c_int), ("chem_driver", C.c_int),
                 ("positivity", C.c_int), ("visc", C.c_int), ("visc_heat", C.c_int), ("visc_diffu", C.c_int),
-                ("Yil_limiter", C.c_double), ("Dim_limiter", C.c_double)]
+                ("Yil_limiter", C.c_double), ("Dim_limiter", C.c_double), ("ghost", C.c_int)]
 
     @property
     def E(self):

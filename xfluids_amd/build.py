@@ -27,7 +27,8 @@ HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-ffp-contract=off",
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-I", os.path.join(ROOT, "include")]
 
 HIP_SOURCES = ["csrc/xf_api.hip", "csrc/xf_ops_ns1.hip", "csrc/xf_ops_ns2.hip", "csrc/xf_ops_ns3.hip",
-               "csrc/xf_ops_ns9.hip", "csrc/xf_dt_flow.hip"]
+               "csrc/xf_ops_ns4.hip", "csrc/xf_ops_ns5.hip", "csrc/xf_ops_ns9.hip", "csrc/xf_ops_ns10.hip",
+               "csrc/xf_ops_ns11.hip", "csrc/xf_dt_flow.hip"]
 HOST_SOURCES = ["host/xf_json.cpp", "host/xf_setup.cpp", "host/xf_initial.cpp", "host/xf_transport.cpp",
                 "host/xf_host.cpp"]
 

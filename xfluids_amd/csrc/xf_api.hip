@@ -32,7 +32,11 @@ bool select_ops(int NS, int cop, Ops &o)
     switch (NS) {
     case 2: o = ops_ns2(); return true;
     case 3: o = ops_ns3(); return true;
+    case 4: o = ops_ns4(); return true;
+    case 5: o = ops_ns5(); return true;
     case 9: o = ops_ns9(); return true;
+    case 10: o = ops_ns10(); return true;
+    case 11: o = ops_ns11(); return true;
     default: return false;
     }
 }
@@ -287,6 +291,7 @@ int xf_create(const xf_case_desc *desc, xf_ctx **out)
     g.dx = d.dx, g.dy = d.dy, g.dz = d.dz;
     g.positivity = d.positivity ? 1 : 0;
     g.visc = d.visc ? 1 : 0;
+    g.ghost = d.ghost ? 1 : 0;
     g.N = c->N;
     auto fail = [&](int code) {
         xf_destroy(c);
@@ -328,6 +333,12 @@ int xf_create(const xf_case_desc *desc, xf_ctx **out)
         return fail(XF_EHIP);
     unsigned qz[4] = {0, 0, 0, 0};
     if (hipMemcpy(f.queue, qz, sizeof(qz), hipMemcpyHostToDevice)) return fail(XF_EHIP);
+    // GhostSpecies: the renormalised mass fractions of UpdateStates
+    if (d.ghost) {
+        if (!d.cop || d.NS < 2) return fail(XF_EINVAL);
+        if (hipMalloc(&f.y, (size_t)(d.NS - 1) * n1) || hipMemset(f.y, 0, (size_t)(d.NS - 1) * n1))
+            return fail(XF_EHIP);
+    }
     // the queue chemistry's hand-over list (cells that go on as whole waves)
     if (CHEM_HANDOVER_ON && d.cop && d.react && c->ops.spill_bytes() > 0) {
         c->spill_cap = (int)std::min<long long>(c->Ninner, 65536);
@@ -358,7 +369,7 @@ void xf_destroy(xf_ctx *c)
     void *ptrs[] = {f.U,   f.U1,    f.LU,       c->U2,   f.rho,   f.u,     f.v,   f.w,   f.p,   f.H,
                     f.T,   f.c,     f.gamma,    f.e,     f.Cp,    f.gcv,   f.R,   f.hi,  f.red, f.dt,
                     f.err, f.substeps, f.queue, c->d_th, c->d_mc, c->d_tr, f.mu,  f.kap, f.Dkm, f.Vde,
-                    f.Fv,  f.ypart, f.spart, c->spill};
+                    f.Fv,  f.ypart, f.spart, c->spill, f.y};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -576,7 +587,19 @@ int xf_download_prims(xf_ctx *c, double *rho, double *p, double *u, double *v, d
     double *src[] = {f.rho, f.p, f.u, f.v, f.w, f.H, f.c, f.gamma, f.e};
     for (int q = 0; q < 9; q++)
         if (dst[q]) XF_HIP(hipMemcpy(dst[q], src[q], n1, hipMemcpyDeviceToHost));
-    if (y_aos) {
+    if (y_aos && c->d.ghost) {
+        // GhostSpecies: the renormalised fractions UpdateStates stored, the
+        // ghost's 0 (Update_device.hpp:15-21)
+        const int NS = c->d.NS;
+        const long long N = c->N;
+        std::vector<double> pl((size_t)(NS - 1) * N);
+        XF_HIP(hipMemcpy(pl.data(), f.y, pl.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (long long id = 0; id < N; id++) {
+            double *y = y_aos + (size_t)NS * id;
+            for (int k = 0; k < NS - 1; k++) y[k] = pl[(size_t)k * N + id];
+            y[NS - 1] = 0.0;
+        }
+    } else if (y_aos) {
         // Y of every padded cell from U, exactly as Getrhoyi (Update_device.hpp:7-29)
         const int E = c->E, NS = c->d.NS;
         const long long N = c->N;
@@ -1151,7 +1174,7 @@ int xf_ensemble_create(xf_ctx *const *cases, int ncases, xf_ensemble **out)
     for (int k = 0; k < ncases; k++) {
         xf_ctx *c = cases[k];
         if (!c || c->d.NS != c0->d.NS || c->d.cop != c0->d.cop || c->d.NR != c0->d.NR ||
-            c->d.react != c0->d.react || c->d.strang != c0->d.strang)
+            c->d.react != c0->d.react || c->d.strang != c0->d.strang || c->d.ghost != c0->d.ghost)
             return XF_EINVAL;
         if (c->d.cop && !c->thermo_set) return XF_EINVAL;
         if (c->d.react && !c->mech_set) return XF_EINVAL;
@@ -1185,6 +1208,7 @@ int xf_ensemble_create(xf_ctx *const *cases, int ncases, xf_ensemble **out)
         r.g = c->g;
         r.U = c->f.U, r.T = c->f.T, r.p = c->f.p, r.dt = c->f.dt;
         r.substeps = c->f.substeps, r.err = c->f.err;
+        r.y = c->f.y;
         off += c->Ninner;
     }
     e->total = off;
@@ -1205,10 +1229,10 @@ int xf_ensemble_create(xf_ctx *const *cases, int ncases, xf_ensemble **out)
     }
     // the one-launch form: cases that agree in everything the launches are
     // specialised on, with the template case's tile shapes
-    bool multi = !c0->d.visc;
+    bool multi = !c0->d.visc && !c0->d.ghost;
     for (int k = 0; k < ncases && multi; k++) {
         const xf_case_desc &d = cases[k]->d, &d0 = c0->d;
-        multi = !d.visc && d.scheme == d0.scheme && d.precision == d0.precision && d.DimX == d0.DimX &&
+        multi = !d.visc && !d.ghost && d.scheme == d0.scheme && d.precision == d0.precision && d.DimX == d0.DimX &&
                 d.DimY == d0.DimY && d.DimZ == d0.DimZ && d.positivity == d0.positivity &&
                 d.ncop_gamma == d0.ncop_gamma && cases[k]->ops.states == c0->ops.states &&
                 (!d.react || cases[k]->mech_spec == c0->mech_spec);

@@ -516,22 +516,40 @@ __device__ __forceinline__ double mix_e(const DevThermo *__restrict__ th, const 
     return e;
 }
 
+// The mass fractions a cell's chemistry starts from: the y array of the
+// Reaction's UpdateStates (ChemeODEQ2SolverKernel1 reads it,
+// Reaction_kernels.hpp:58-60) -- Y_k = U_{5+k} / rho and the remainder, or for
+// a GhostSpecies mixture the stored renormalised planes and 0
+template <int NS>
+__device__ __forceinline__ void chem_y0(const double *U, const double *ypl, int ghost, long long N, long long id,
+                                        double rho1, double *yi)
+{
+    if (ghost) {
+        yi[NS - 1] = 0.0;
+#pragma unroll
+        for (int k = 0; k < NS - 1; k++) yi[k] = ypl[k * N + id];
+    } else {
+        yi[NS - 1] = 1.0;
+#pragma unroll
+        for (int k = 0; k < NS - 1; k++) {
+            yi[k] = U[(5 + k) * N + id] * rho1;
+            yi[NS - 1] += -yi[k];
+        }
+    }
+}
+
 // One interior cell of ChemeODEQ2SolverKernel1.
 template <int NS, int SPEC>
-__device__ __forceinline__ int chem_cell(const DevThermo *__restrict__ th, const DevMech *__restrict__ mc, Fields &f,
-                                         long long N, long long id, double dtc)
+__device__ __forceinline__ int chem_cell(const Geom &g, const DevThermo *__restrict__ th,
+                                         const DevMech *__restrict__ mc, Fields &f, long long N, long long id,
+                                         double dtc)
 {
     constexpr int E = NS + 4;
     double y[NS + 1];
     y[0] = f.T[id];
     const double rho = f.U[id]; // rho = U[0] (Getrhoyi)
     const double rho1 = 1.0 / rho;
-    y[1 + NS - 1] = 1.0;
-#pragma unroll
-    for (int k = 0; k < NS - 1; k++) {
-        y[1 + k] = f.U[(5 + k) * N + id] * rho1;
-        y[NS] += -y[1 + k];
-    }
+    chem_y0<NS>(f.U, f.y, g.ghost, N, id, rho1, y + 1);
     const double e0 = mix_e<NS>(th, y + 1, y[0]);
     const int passes = chemeq2_dev<NS, SPEC>(th, mc, y, dtc, rho, f.p[id]);
     const double e1 = mix_e<NS>(th, y + 1, y[0]);
@@ -572,7 +590,7 @@ __global__ void __launch_bounds__(256) k_chem(Geom g, const DevThermo *__restric
     const int j = g.By + (int)((t / g.X_inner) % g.Y_inner);
     const int k = g.Bz + (int)(t / ((long long)g.X_inner * g.Y_inner));
     const long long id = (long long)g.Xmax * g.Ymax * k + (long long)g.Xmax * j + i;
-    const int passes = chem_cell<NS, SPEC>(th, mc, f, g.N, id, f.dt[DT_CHEM]);
+    const int passes = chem_cell<NS, SPEC>(g, th, mc, f, g.N, id, f.dt[DT_CHEM]);
     if (record) f.substeps[t] = passes;
     int bits = passes < 0 ? EBIT_CHEM : 0;
     if (passes < 0) report_cell(f.err, EBIT_CHEM, id);
@@ -600,6 +618,7 @@ struct ChemCase {
     Geom g;
     double *U, *T, *p, *dt;
     int *substeps, *err;
+    double *y; // GhostSpecies mass-fraction planes (g.ghost), else unused
 };
 
 // Write-back of a finished cell (ChemeODEQ2SolverKernel1's tail,
@@ -692,12 +711,7 @@ __device__ __forceinline__ void chem_queue_run(const Cases &cases, long long tot
                     L.y[0] = cc.T[id];
                     const double rho = cc.U[id];
                     const double rho1 = 1.0 / rho;
-                    L.y[NS] = 1.0;
-#pragma unroll
-                    for (int k = 0; k < NS - 1; k++) {
-                        L.y[1 + k] = cc.U[(5 + k) * N + id] * rho1;
-                        L.y[NS] += -L.y[1 + k];
-                    }
+                    chem_y0<NS>(cc.U, cc.y, cc.g.ghost, N, id, rho1, L.y + 1);
                     e0 = mix_e<NS>(th, L.y + 1, L.y[0]);
                     L.begin(cc.dt[DT_CHEM], rho, cc.p[id]);
                     active = true;
@@ -792,6 +806,7 @@ __device__ __forceinline__ OneCase one_case(const Geom &g, const Fields &f)
     one.c.g = g;
     one.c.U = f.U, one.c.T = f.T, one.c.p = f.p, one.c.dt = f.dt;
     one.c.substeps = f.substeps, one.c.err = f.err;
+    one.c.y = f.y;
     return one;
 }
 

@@ -55,7 +55,11 @@ struct Ops {
 Ops ops_ns1();
 Ops ops_ns2();
 Ops ops_ns3();
+Ops ops_ns4();
+Ops ops_ns5();
 Ops ops_ns9();
+Ops ops_ns10();
+Ops ops_ns11();
 
 } // namespace xf
 
@@ -162,6 +166,11 @@ template <int NS, bool COP> struct Launch {
         ProfScope ps(c, "states");
         hipLaunchKernelGGL((k_states<NS, COP>), dim3(nblk(c->N)), dim3(256), 0, c->stream, c->g, c->d_th, U, c->f,
                            c->d.ncop_gamma, full, bcs, where);
+        if constexpr (COP) {
+            if (c->g.ghost)
+                hipLaunchKernelGGL((k_rhoyi_writeback<NS>), dim3(nblk(c->N)), dim3(256), 0, c->stream, c->g, U,
+                                   c->f);
+        }
         hipLaunchKernelGGL(k_states_fold, dim3(12), dim3(1024), 0, c->stream, c->f.spart, (int)nblk(c->N), c->f.red,
                            c->f.err);
     }

@@ -244,7 +244,18 @@ __device__ __forceinline__ void states_block(const Geom &g, const DevThermo *__r
         const double rho = U[0];
         const double rho1 = 1.0 / rho;
         double yi[COP ? NS : 1];
-        if (COP) {
+        if (COP && g.ghost) {
+            // GhostSpecies: renormalise the first NS - 1 fractions; the state's
+            // partial densities become rho * Y_k in k_rhoyi_writeback, after
+            // this launch (the ghost fill above reads sources unchanged)
+            yi[NCOP] = 0.0;
+            double sum_yi = 0.0;
+#pragma unroll
+            for (int ii = 0; ii < NCOP; ii++) yi[ii] = U[ii + 5] * rho1, sum_yi += yi[ii];
+            sum_yi = 1.0 / sum_yi;
+#pragma unroll
+            for (int ii = 0; ii < NCOP; ii++) yi[ii] *= sum_yi, f.y[ii * N + id] = yi[ii];
+        } else if (COP) {
             yi[NCOP] = 1.0;
 #pragma unroll
             for (int ii = 5; ii < E; ii++) yi[ii - 5] = U[ii] * rho1, yi[NCOP] += -yi[ii - 5];
@@ -386,6 +397,22 @@ __global__ void __launch_bounds__(256) k_states(Geom g, const DevThermo *__restr
                                                 double ncop_gamma, int full, int bcs, int where)
 {
     states_block<NS, COP>(g, th, Ug, f, ncop_gamma, full, bcs, where, (long long)blockIdx.x);
+}
+
+// GhostSpecies write-back of Getrhoyi (Update_device.hpp:20-21) on every
+// padded cell: U_{5+k} = rho * Y_k with the fractions k_states stored.  A
+// launch of its own after k_states, whose ghost fill reads interior cells as
+// its sources; it runs exactly when k_states did (the same entry test, and
+// before k_states_fold latches this pass's failures).
+template <int NS>
+__global__ void __launch_bounds__(256) k_rhoyi_writeback(Geom g, double *Ug, Fields f)
+{
+    if (step_halted(f.err, ~(EBIT_RHO_Y | EBIT_PRIM))) return;
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= g.N) return;
+    const double rho = Ug[id];
+#pragma unroll
+    for (int k = 0; k < NS - 1; k++) Ug[(5 + k) * g.N + id] = rho * f.y[k * g.N + id];
 }
 
 // UpdateStates of every case of an ensemble in one launch (fill = 1: with

@@ -59,6 +59,7 @@ struct Geom {
     double dx, dy, dz; // cell sizes (WENO-CU6 epsilon = 1e-8 dl^2)
     int positivity;    // positivity-preserving flux limiter on
     int visc;          // viscous wall fluxes (Fv) subtracted from the face fluxes
+    int ghost;         // GhostSpecies mixture: mass fractions from the y planes
 };
 
 // Device pointers of one case
@@ -78,6 +79,10 @@ struct Fields {
     // viscous face flux of the direction being swept (E planes)
     double *mu, *kap, *Dkm, *Vde, *Fv;
     double *ypart; // per-block interior min / max of each mass fraction (2 NS per block)
+    // GhostSpecies mixtures only: the renormalised mass fractions of the
+    // first NS - 1 species (NS - 1 planes) as the last UpdateStates formed
+    // them; the ghost species' own is 0
+    double *y;
     double *spart; // per-block GLF and CFL maxima of k_states (12 per block)
 };
 
@@ -161,6 +166,26 @@ constexpr int DT_STAGED = 2; // dt of xf_rk3_stage
 constexpr int DT_TIME = 4;   // physical time after the current step's dt
 constexpr int DT_TARGET = 5; // output time the step's dt is clamped to (+inf: none)
 constexpr int DT_SIZE = 8;
+
+// Getrhoyi's mass fractions of cell id (Update_device.hpp:7-29) for the
+// kernels that follow an UpdateStates: Y_k = U_{5+k} / rho with the last
+// species the remainder 1 - sum; for a GhostSpecies mixture the renormalised
+// values that UpdateStates stored, and 0 for the ghost.  u5(k) gives U_{5+k}
+// of the cell, r1 = 1 / rho.
+template <int NS, class U5>
+__device__ __forceinline__ void cell_y(const Geom &g, const Fields &f, long long id, double r1, U5 u5, double *y)
+{
+    constexpr int NCOP = NS - 1;
+    if (g.ghost) {
+        y[NCOP] = 0.0;
+#pragma unroll
+        for (int k = 0; k < NCOP; k++) y[k] = f.y[k * g.N + id];
+    } else {
+        y[NCOP] = 1.0;
+#pragma unroll
+        for (int k = 0; k < NCOP; k++) y[k] = u5(k) * r1, y[NCOP] += -y[k];
+    }
+}
 
 // the step kernels' entry test: a halted step, or a failure recorded by an
 // earlier launch (bits in mask are those the calling kernel never sets)

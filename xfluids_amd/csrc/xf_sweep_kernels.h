@@ -238,7 +238,8 @@ __device__ __forceinline__ double weno_face(const double *pp, const double *mm, 
 // epsilon (1e-6 in its 12-beta units) becomes 1e-6 / m^2: the nonlinear
 // weights are those of the FP64 definition up to FP32 rounding, with no
 // overflow for any data magnitude.  WENO5-JS weights are normalised through
-// the ratios (s_min / s_k)^2; WENO5-Z keeps tau5 = |beta_1 - beta_3|.
+// the ratios (s_min / s_k)^2; WENO5-Z keeps tau5 = |beta_1 - beta_3|.  The
+// weights are normalised before they multiply the candidate values.
 // Divisions use the hardware reciprocal (1 ulp).  A stencil below 1e-30 in
 // magnitude takes the ideal weights, as the FP64 form does when epsilon
 // dominates every indicator.
@@ -273,7 +274,11 @@ __device__ __forceinline__ float weno5_body_f32(float v1, float v2, float v3, fl
         const float r1 = smin * rcp_f32(s1), r2 = smin * rcp_f32(s2), r3 = smin * rcp_f32(s3);
         a1 = 0.1f * r1 * r1, a2 = 0.6f * r2 * r2, a3 = 0.3f * r3 * r3;
     }
-    return (a1 * q1 + a2 * q2 + a3 * q3) * rcp_f32(a1 + a2 + a3) * (1.0f / 6.0f);
+    // normalised weights first: a_k reaches ~1e32 where an indicator is 0
+    // and epsilon is small, and a_k * q_k would overflow FP32 at the
+    // magnitudes of strong shocks
+    const float ia = rcp_f32(a1 + a2 + a3);
+    return ((a1 * ia) * q1 + (a2 * ia) * q2 + (a3 * ia) * q3) * (1.0f / 6.0f);
 }
 
 // WENO-CU6 in FP32 with the same rescaling: every indicator (s1..s3, s6)
@@ -310,7 +315,9 @@ __device__ __forceinline__ float wenocu6_body_f32(float v1, float v2, float v3, 
     const float a2 = 0.45f * (20.0f + s5 * rcp_f32(s2 + eps));
     const float a3 = 0.45f * (20.0f + s5 * rcp_f32(s3 + eps));
     const float a4 = 0.05f * (20.0f + s5 * rcp_f32(s6 + eps));
-    return (a1 * c1 + a2 * c2 + a3 * c3 + a4 * c4) * rcp_f32(a1 + a2 + a3 + a4) * (1.0f / 6.0f);
+    // normalised weights first (see weno5_body_f32): a_k up to ~1e34
+    const float ia = rcp_f32(a1 + a2 + a3 + a4);
+    return ((a1 * ia) * c1 + (a2 * ia) * c2 + (a3 * ia) * c3 + (a4 * ia) * c4) * (1.0f / 6.0f);
 }
 
 template <int SCHEME>
@@ -559,8 +566,15 @@ __device__ __forceinline__ void sweep_tile(const Geom &g, const DevThermo *__res
 #pragma unroll
             for (int k = 0; k < 3; k++) F[1 + k] = (k == DIR) ? Um * vel[k] + p : Um * vel[k];
             F[4] = (U[4] + p) * vel[DIR];
+            // GetPhysFlux: UI[1 + DIR] * yi (Y_k = U_{5+k} / rho, or the stored
+            // fractions of a GhostSpecies mixture)
+            if (g.ghost) {
 #pragma unroll
-            for (int ii = 5; ii < E; ii++) F[ii] = Um * (U[ii] * rho1);
+                for (int ii = 5; ii < E; ii++) F[ii] = Um * f.y[(ii - 5) * N + id];
+            } else {
+#pragma unroll
+                for (int ii = 5; ii < E; ii++) F[ii] = Um * (U[ii] * rho1);
+            }
 #pragma unroll
             for (int n = 0; n < E; n++) {
                 sU[n * NC + li] = U[n];
@@ -621,13 +635,8 @@ __device__ __forceinline__ void sweep_tile(const Geom &g, const DevThermo *__res
         rf._rho = sqrt(rr * rl);
         if (COP) {
             double yl[COP ? NS : 1], yr[COP ? NS : 1], hl[COP ? NS : 1], hr[COP ? NS : 1];
-            yl[NCOP] = 1.0;
-            yr[NCOP] = 1.0;
-#pragma unroll
-            for (int k = 0; k < NCOP; k++) {
-                yl[k] = sU[(5 + k) * NC + li_l] * r1l, yl[NCOP] += -yl[k];
-                yr[k] = sU[(5 + k) * NC + li_r] * r1r, yr[NCOP] += -yr[k];
-            }
+            cell_y<COP ? NS : 1>(g, f, id_l, r1l, [&](int k) { return sU[(5 + k) * NC + li_l]; }, yl);
+            cell_y<COP ? NS : 1>(g, f, id_r, r1r, [&](int k) { return sU[(5 + k) * NC + li_r]; }, yr);
 #pragma unroll
             for (int k = 0; k < (COP ? NS : 1); k++) hl[k] = f.hi[k * N + id_l], hr[k] = f.hi[k * N + id_r];
             cop_c2<COP ? NS : 1>(th, rf, D, D1, yl, yr, hl, hr, f.T[id_l], f.T[id_r], f.gcv[id_l], f.gcv[id_r],
@@ -832,7 +841,7 @@ __device__ __forceinline__ void sweep_tile(const Geom &g, const DevThermo *__res
             const double lambda_0 = f.red[RED_LAM + DIR];
             const double lambda = g.CFL / lambda_0;
             double UL[E], UR[E], FL[E], FR[E];
-            auto cell_flux = [&](int li, double *Uc, double *Fc) {
+            auto cell_flux = [&](int li, long long idc, double *Uc, double *Fc) {
 #pragma unroll
                 for (int n = 0; n < E; n++) Uc[n] = sU[n * NC + li];
                 if (!MIX) {
@@ -847,12 +856,17 @@ __device__ __forceinline__ void sweep_tile(const Geom &g, const DevThermo *__res
 #pragma unroll
                     for (int k = 0; k < 3; k++) Fc[1 + k] = (k == DIR) ? Um * vel[k] + p : Um * vel[k];
                     Fc[4] = (Uc[4] + p) * vel[DIR];
+                    if (g.ghost) {
 #pragma unroll
-                    for (int ii = 5; ii < E; ii++) Fc[ii] = Um * (Uc[ii] * rho1);
+                        for (int ii = 5; ii < E; ii++) Fc[ii] = Um * f.y[(ii - 5) * N + idc];
+                    } else {
+#pragma unroll
+                        for (int ii = 5; ii < E; ii++) Fc[ii] = Um * (Uc[ii] * rho1);
+                    }
                 }
             };
-            cell_flux(li_l, UL, FL);
-            cell_flux(li_r, UR, FR);
+            cell_flux(li_l, id_l, UL, FL);
+            cell_flux(li_r, id_r, UR, FR);
             positivity_face<E, NCOP>(flux, UL, UR, FL, FR, lambda_0, lambda);
         }
         // viscous wall flux of this face (after the limiter, as GetLU orders them)

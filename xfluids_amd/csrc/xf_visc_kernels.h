@@ -84,9 +84,7 @@ __global__ void __launch_bounds__(256) k_transport(Geom g, const DevThermo *__re
             for (int n = 0; n < E; n++) U[n] = Ug[n * N + id];
             rho = U[0];
             rho1 = 1.0 / rho;
-            yi[NCOP] = 1.0;
-#pragma unroll
-            for (int ii = 5; ii < E; ii++) yi[ii - 5] = U[ii] * rho1, yi[NCOP] += -yi[ii - 5];
+            cell_y<NS>(g, f, id, rho1, [&](int k) { return U[5 + k]; }, yi);
             f.u[id] = U[1] * rho1;
             f.v[id] = U[2] * rho1;
             f.w[id] = U[3] * rho1;
@@ -400,9 +398,7 @@ __global__ void __launch_bounds__(256) k_visc_flux(Geom g, const double *__restr
             const double r = Ug[cid[c]];
             rw[c] = r;
             const double r1 = 1.0 / r;
-            Y[c][NCOP] = 1.0;
-#pragma unroll
-            for (int ii = 0; ii < NCOP; ii++) Y[c][ii] = Ug[(5 + ii) * N + cid[c]] * r1, Y[c][NCOP] += -Y[c][ii];
+            cell_y<NS>(g, f, cid[c], r1, [&](int ii) { return Ug[(5 + ii) * N + cid[c]]; }, Y[c]);
         }
         const double rho_wall = (9.0 * (rw[2] + rw[1]) - (rw[3] + rw[0])) * SX;
         double CorrectTerm = 0.0, Dim_Yil = 1.0e-20, Yi_wall[NS];

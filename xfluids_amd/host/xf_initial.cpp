@@ -2,9 +2,11 @@
 //
 //   isentropic-vortex    exactly the reference's 2D Euler vortex
 //                        (src/solver_Ini/sample/2D-EulerVortex/ini_sample.hpp:14-89)
-//   shock-bubble         Mach-M planar shock in air hitting a circular bubble
-//                        of the "inside" mixture (radius 0.2 of the domain
-//                        height, tanh interface over 3 cells)
+//   shock-bubble         Mach-M planar shock in the "outside" mixture hitting
+//                        a circular (spherical in 3-D) bubble of the "inside"
+//                        mixture; radius init.bubble_shape_x or 0.2 of the
+//                        domain height, tanh interface over
+//                        init.bubble_boundary_cells or 3 cells
 //   reactive-shock-tube  the reference's 1D reactive shock tube states
 //                        (src/solver_Ini/sample/1D-X-Y-Z/reactive-st/ini_sample.hpp:46-69)
 //   mixing-layer         tanh shear layer, velocity ratio 3, temperature ratio 2,
@@ -95,8 +97,8 @@ bool setup_initial(const Setup &s, const std::string &wl, unsigned long long see
         const double T1 = s.blast_down[2] > 0 ? s.blast_down[2] : 295.0;
         const double xs = s.blast_center[0];
         const double xb = s.cop_center[0], yb = s.cop_center[1];
-        const double R = 0.2 * (d.DimY ? s.Domain_width : s.Domain_length);
-        const double delta = 3.0 * s.dx;
+        const double R = s.bubble_r > 0.0 ? s.bubble_r : 0.2 * (d.DimY ? s.Domain_width : s.Domain_length);
+        const double delta = (s.bubble_cells > 0.0 ? s.bubble_cells : 3.0) * s.dx;
         // Rankine-Hugoniot jump with the air's gamma at T1
         const double Rair = host_mix_R(s, s.Y_out.data());
         const double cp = host_mix_cp(s, s.Y_out.data(), T1);

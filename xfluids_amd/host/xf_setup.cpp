@@ -216,6 +216,14 @@ bool setup_load(const std::string &settings, const std::string &runtime_root, Se
         s.NR = 0;
     }
     d.NR = s.NR;
+    // GhostSpecies: compile-time in the reference, set by a mixture's
+    // case_setup.h (runtime.dat/Inert-SBI, Reaction/RSBI-19REA, ...): the last
+    // species of the list is a ghost copy that carries no mass
+    d.ghost = eq->value("GhostSpecies", 0.0) != 0.0 ? 1 : 0;
+    if (d.ghost && (!d.cop || NS < 2)) {
+        err = "GhostSpecies needs a multi-component mixture";
+        return false;
+    }
     // Sources_React defaults to COP_CHEME: on for reacting mixtures
     d.react = (int)eq->value("Sources_React", (d.cop && s.NR > 0) ? 1.0 : 0.0) && s.NR > 0;
     // equations.PositivityPreserving (settings/*.json): the flux limiter
@@ -277,6 +285,8 @@ bool setup_load(const std::string &settings, const std::string &runtime_root, Se
 
     // init section
     s.blast_mach = ini->value("blast_mach", 0.0);
+    s.bubble_r = ini->value("bubble_shape_x", 0.0);
+    s.bubble_cells = ini->value("bubble_boundary_cells", 0.0);
     s.blast_center = ini->values("blast_center", {0.5, 0.5, 0.5});
     s.cop_center = ini->values("cop_center", {0.0, 0.0, 0.0});
     s.blast_up = ini->values("blast_upstream", {0, 0, 298.15, 0, 0, 0});

@@ -29,6 +29,9 @@ struct Setup {
     std::string mixture, slip_order = "Strang";
     // init section (read_json.cpp:132-148)
     double blast_mach = 0.0;
+    // init.bubble_shape_x (bubble radius) and init.bubble_boundary_cells
+    // (interface width in cells) of settings/shock-bubble.json; 0 = defaults
+    double bubble_r = 0.0, bubble_cells = 0.0;
     std::vector<double> blast_center{0.5, 0.5, 0.5}, cop_center{0, 0, 0};
     std::vector<double> blast_up{0, 0, 298.15, 0, 0, 0}, blast_down{0, 0, 298.15, 0, 0, 0}, cop_in;
     // species (thermal.cpp:6-32): names, molar and mass fractions outside / inside

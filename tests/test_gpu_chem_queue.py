@@ -37,7 +37,7 @@ def test_queue_matches_per_cell_single_call(dt_chem):
     assert np.array_equal(a[1].view(np.uint64), b[1].view(np.uint64))
 
 
-@pytest.mark.parametrize("case", ["reactive_shock_tube", "reactive_channel"])
+@pytest.mark.parametrize("case", ["reactive_shock_tube", "reactive_channel", "rsbi", "mixing_layer_21"])
 def test_queue_matches_per_cell_steps(case):
     announce("chem queue vs per-cell steps " + case)
     s, U, T, nsteps = make(case)
@@ -54,7 +54,8 @@ def test_queue_matches_per_cell_steps(case):
     assert np.array_equal(a[2], b[2])
 
 
-@pytest.mark.parametrize("case", ["reactive_shock_tube", "reactive_channel", "mixing_layer_3d"])
+@pytest.mark.parametrize("case", ["reactive_shock_tube", "reactive_channel", "mixing_layer_3d", "rsbi",
+                                  "mixing_layer_21"])
 @pytest.mark.parametrize("driver", [xf.XF_CHEM_PER_CELL, xf.XF_CHEM_QUEUE])
 def test_compiled_mechanism_matches_table_loop(case, driver):
     """The rate evaluation specialised to a shipped mechanism (records as

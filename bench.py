@@ -102,9 +102,12 @@ def measured_traffic():
     return out
 
 
+CHEM_DRIVERS = {"per-cell": xf.XF_CHEM_PER_CELL, "queue": xf.XF_CHEM_QUEUE, "wave": xf.XF_CHEM_WAVE}
+
+
 def ctx_for(s, prec, drv):
     return s.context(precision=xf.XF_MIXED if prec == "mixed" else xf.XF_PLAIN_FP64,
-                     chem_driver=xf.XF_CHEM_QUEUE if drv == "queue" else xf.XF_CHEM_PER_CELL)
+                     chem_driver=CHEM_DRIVERS[drv])
 
 
 def timed_steps(ctx, warmup, steps):
@@ -408,7 +411,9 @@ def vortex_leg(steps=200, n=256):
 # ---------------------------------------------------------------------------
 # C3: reactive shock tube in its post-reflection window
 # ---------------------------------------------------------------------------
-C3_FORMS = [("plain", "per-cell"), ("plain", "queue"), ("mixed", "queue")]
+# the batched forms of the tube: the queue, and one wave per cell (each pass
+# spread over the lanes), the driver for few cells of uneven stiffness
+C3_FORMS = [("plain", "per-cell"), ("plain", "queue"), ("mixed", "queue"), ("plain", "wave"), ("mixed", "wave")]
 
 
 def shock_tube(nx, t_window, steps, warmup):
@@ -556,12 +561,16 @@ def main():
     ap.add_argument("--ensemble-cases", type=int, default=64)
     ap.add_argument("--ensemble-sides", default="128,256,512,1024")
     ap.add_argument("--c2-steps", type=int, default=500)
+    ap.add_argument("--c3-only", action="store_true", help="only the shock tube window (its own JSON line)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
         sys.exit(2)
     if args.copy_probe:
         print(json.dumps({"copy_probe": copy_probe()}))
+        return
+    if args.c3_only:
+        print(json.dumps({"shock_tube": shock_tube(8192, args.c3_window_us * 1e-6, args.c3_steps, 10)}))
         return
     s, ml = mixing_layer(args.n, args.startup, args.warmup, args.steps)
     d = s.desc

@@ -63,6 +63,9 @@ extern "C" {
 /* chemistry drivers */
 #define XF_CHEM_PER_CELL 0 /* one lane per cell */
 #define XF_CHEM_QUEUE 1    /* persistent lanes draining a device work queue */
+#define XF_CHEM_WAVE 2     /* one wave per cell, each pass spread over its lanes (few
+                              cells, e.g. a 1-D tube: the stiffest cell's chain of passes
+                              sets the launch time) */
 
 /* Case descriptor: the runtime form of the reference's Block/MeshSize
    (src/include/global_setup.h:162-200, derived as Setup::init,
@@ -85,7 +88,7 @@ typedef struct xf_case_desc {
     int strang;         /* 1 = Strang splitting, 0 = Lie */
     double ncop_gamma;  /* NCOP_Gamma */
     int precision;      /* XF_PLAIN_FP64, XF_MIXED or XF_FUSED_FP64 */
-    int chem_driver;    /* XF_CHEM_PER_CELL or XF_CHEM_QUEUE */
+    int chem_driver;    /* XF_CHEM_PER_CELL, XF_CHEM_QUEUE or XF_CHEM_WAVE */
     int positivity;     /* PositivityPreserving flux limiter on the face fluxes
                            (PositivityPreserving_kernels.hpp:5-76) */
     int visc;           /* Visc: viscous stresses in the wall fluxes (CMakeLists.txt:55) */

@@ -86,6 +86,34 @@ HANDOVER_OFF = ("whole-wave hand-over switched off: its first GPU run (reactive 
                 "passes) did not finish and the cause has not been shown")
 
 
+@pytest.mark.parametrize("case", ["reactive_shock_tube", "reactive_channel", "mixing_layer_3d", "rsbi",
+                                  "mixing_layer_21"])
+def test_wave_driver_matches_per_cell(case):
+    """One wave per cell (each pass spread over the lanes, the component
+    sums as ordered chains): a single call's states, temperatures and pass
+    counts, and full steps, bit for bit against one lane per cell."""
+    announce("chem wave driver vs per-cell " + case)
+    s, U, T, nsteps = make(case)
+    res = {}
+    for drv in (xf.XF_CHEM_PER_CELL, xf.XF_CHEM_WAVE):
+        ctx = _ctx(s, U, T, drv)
+        ctx.update_states(0)
+        sub = ctx.chemistry(0.5 * ctx.get_dt(), record=True)
+        U1, T1 = ctx.download(0)
+        ctx.upload(U, T)
+        ctx.initialize()
+        dts = ctx.steps(nsteps, graph=True, record_dt=True)
+        Ug, Tg = ctx.download(0)
+        ctx.close()
+        res[drv] = (sub, U1, T1, Ug, Tg, dts)
+    a, b = res[xf.XF_CHEM_PER_CELL], res[xf.XF_CHEM_WAVE]
+    print("passes: median %g max %d" % (np.median(a[0]), a[0].max()))
+    assert np.array_equal(a[0], b[0])
+    for x, y in zip(a[1:5], b[1:5]):
+        assert np.array_equal(x.view(np.uint64), y.view(np.uint64))
+    assert np.array_equal(a[5], b[5])
+
+
 def test_handover_refused_while_switched_off():
     announce("chem hand-over refused")
     s, U, T, _ = make("reactive_channel")

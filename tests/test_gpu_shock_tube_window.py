@@ -1,8 +1,8 @@
 """The reactive shock tube in the reference's output window (t = 170 us,
 settings/1d-reactive-st.json): from the committed state the CPU oracle
 reached there on 1024 cells (tests/golden/make_shock_tube_fixture.py), the
-plain form with per-cell and with queue chemistry must stay bit-identical to
-the oracle, the queue must equal the per-cell driver pass for pass, and the
+plain form with per-cell, queue and one-wave-per-cell chemistry must stay
+bit-identical to the oracle, the queue must equal the per-cell driver pass for pass, and the
 mixed form must keep L1 of rho, p and T within 1e-6."""
 import os
 
@@ -57,7 +57,7 @@ def _gpu(precision, driver):
     return s, Ug, Tg, dts, clock, passes
 
 
-@pytest.mark.parametrize("driver", [xf.XF_CHEM_PER_CELL, xf.XF_CHEM_QUEUE])
+@pytest.mark.parametrize("driver", [xf.XF_CHEM_PER_CELL, xf.XF_CHEM_QUEUE, xf.XF_CHEM_WAVE])
 def test_plain_bitwise_in_the_window(driver):
     announce("shock tube window, plain, driver %d" % driver)
     Uo, To, dto, clock_o = _oracle()

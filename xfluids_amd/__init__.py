@@ -29,7 +29,7 @@ XF_MAXNS = 16
 XF_MAXNR = 40
 XF_WENO5JS, XF_WENO5Z, XF_WENOCU6 = 0, 1, 2
 XF_PLAIN_FP64, XF_MIXED, XF_FUSED_FP64 = 0, 1, 2
-XF_CHEM_PER_CELL, XF_CHEM_QUEUE = 0, 1
+XF_CHEM_PER_CELL, XF_CHEM_QUEUE, XF_CHEM_WAVE = 0, 1, 2
 XF_EINVAL, XF_EHIP, XF_ENOKERNEL = -1, -2, -3
 ERRORS = {1: "XF_ERR_RHO_Y", 2: "XF_ERR_PRIM", 3: "XF_ERR_NAN", 5: "XF_ERR_CHEM",
           -1: "XF_EINVAL", -2: "XF_EHIP", -3: "XF_ENOKERNEL"}

@@ -265,7 +265,8 @@ int xf_create(const xf_case_desc *desc, xf_ctx **out)
         d.Zmax != d.Z_inner + 2 * d.Bwidth_Z)
         return XF_EINVAL;
     if (d.precision != XF_PLAIN_FP64 && d.precision != XF_MIXED && d.precision != XF_FUSED_FP64) return XF_EINVAL;
-    if (d.chem_driver != XF_CHEM_PER_CELL && d.chem_driver != XF_CHEM_QUEUE) return XF_EINVAL;
+    if (d.chem_driver != XF_CHEM_PER_CELL && d.chem_driver != XF_CHEM_QUEUE && d.chem_driver != XF_CHEM_WAVE)
+        return XF_EINVAL;
     if (d.scheme != XF_WENO5JS && d.scheme != XF_WENO5Z && d.scheme != XF_WENOCU6) return XF_EINVAL;
     // the viscous terms need the species transport data of a mixture, and
     // their centre derivatives reach 4 cells into the ghost layers
@@ -282,6 +283,11 @@ int xf_create(const xf_case_desc *desc, xf_ctx **out)
     c->NCOP = d.cop ? d.NS - 1 : 0;
     c->N = (long long)d.Xmax * d.Ymax * d.Zmax;
     c->Ninner = (long long)d.X_inner * d.Y_inner * d.Z_inner;
+    // one 64-lane block per cell: the grid must stay below 2^32 work-items
+    if (d.chem_driver == XF_CHEM_WAVE && c->Ninner * 64 > 0xffffffffLL) {
+        delete c;
+        return XF_EINVAL;
+    }
     Geom &g = c->g;
     g.Xmax = d.Xmax, g.Ymax = d.Ymax, g.Zmax = d.Zmax;
     g.X_inner = d.X_inner, g.Y_inner = d.Y_inner, g.Z_inner = d.Z_inner;

@@ -180,7 +180,8 @@ __device__ __forceinline__ void evalcv_dev(const DevThermo *__restrict__ th, con
     if (uni) species(tu);
     else species(t);
     const double logStandConc = dlog(m_p / (R0 * m_tmp));
-    const double logT = dlog(m_tmp);
+    // log T: the fit's log max(T, 200) when T >= 200 (the same call on the same value)
+    const double logT = m_tmp >= 200.0 ? t.lT : dlog(m_tmp);
     const double logP = dlog(m_p * (1.0 / 101325.0));
     double m_cm[NS];
 #pragma unroll
@@ -249,7 +250,8 @@ __device__ __forceinline__ void evalcv_coop(const DevThermo *__restrict__ th, co
         for (int n = 0; n < NS; n++) h1[n] = __shfl(h, n), gk[n] = __shfl(gv, n);
     }
     const double logStandConc = dlog(m_p / (R0 * m_tmp));
-    const double logT = dlog(m_tmp);
+    // log T: the fit's log max(T, 200) when T >= 200 (the same call on the same value)
+    const double logT = m_tmp >= 200.0 ? t.lT : dlog(m_tmp);
     const double logP = dlog(m_p * (1.0 / 101325.0));
     double m_cm[NS];
 #pragma unroll
@@ -854,6 +856,374 @@ __global__ void __launch_bounds__(64) k_chem_coop_multi(const ChemCase *__restri
 {
     ManyCases many{cases, ncases};
     chem_coop_run<NS>(many, th, mc, ChemSpillBuf{spill, (unsigned *)queue + 2, spill_cap, 0}, record, check);
+}
+
+// ---------------------------------------------------------------------------
+// One cell per wave (XF_CHEM_WAVE).  With few cells (a 1-D tube) the chemistry
+// launch lasts as long as the stiffest cell's serial chain of passes, about
+// 4500 instructions each on one lane.  Here the wave of a cell spreads every
+// pass over its lanes: lane i < NS + 1 holds solution component i (0 = T,
+// 1 + n = Y_n) and does that component's predictor and corrector; lane 1 + n
+// evaluates species n's enthalpy, Gibbs energy and concentration, lane r < NR
+// reaction r's rates, and lane 1 + n sums its species' production and loss
+// over the reactions in order.  Every sum and maximum the reference forms over
+// components or species (normalisation, step-size scratch, energy terms, Cv,
+// the error norm) runs as an ordered chain over the lanes' values, so each
+// value is rounded exactly as in ChemLane / evalcv_dev and the results are the
+// per-cell driver's bits.  Requires NS + 1 <= 64 and NR <= 64.
+// ---------------------------------------------------------------------------
+
+// Cross-lane exchanges of the cell's wave go through LDS: the lanes that own
+// values write them, one barrier (a single wave: no wait on other waves), and
+// every lane reads them back at the same addresses (LDS broadcasts).  Sums are
+// then formed on every lane in the reference's order.
+template <int NS> struct WaveShared {
+    double K[2 * MAXNR];    // Kf, Kck of each reaction
+    double S[2 * MAXNS];    // Gibbs energy and concentration of each species
+    double V[4 * 64];       // per-lane operands of the ordered sums
+    // species lane ln's nu_f | nu_b << 4 in reaction r at nu[64 * r + ln]
+    // (filled once per cell; each lane reads only its own entries)
+    unsigned char nu[64 * MAXNR];
+};
+
+// ((init + v[lo]) + v[lo+1]) + ... of a lane-indexed LDS array with stride st
+__device__ __forceinline__ double lds_sum(const double *v, int st, int lo, int hi, double init)
+{
+    double s = init;
+    for (int k = lo; k < hi; k++) s += v[k * st];
+    return s;
+}
+// ((init - v[lo]) - v[lo+1]) - ...
+__device__ __forceinline__ double lds_sub(const double *v, int st, int lo, int hi, double init)
+{
+    double s = init;
+    for (int k = lo; k < hi; k++) s -= v[k * st];
+    return s;
+}
+
+// What a lane reads on every pass, loaded once per cell: species lane 1 + n
+// keeps W, 1/W and R of species n and the fit coefficients of the current
+// range (reloaded when T crosses into another range); reaction lane r keeps
+// its record and third-body efficiencies.
+template <int NS> struct WaveLane {
+    double Wi = 0.0, iWi = 0.0, Ri = 0.0;
+    int rc = -2; // range the coefficients hc belong to
+    double hc[9];
+    DevRxn R{};
+    double eff[NS];
+
+    __device__ __forceinline__ void load(const DevThermo *__restrict__ th, const DevMech *__restrict__ mc, int NR)
+    {
+        const int ln = threadIdx.x & 63;
+#pragma unroll
+        for (int m = 0; m < 9; m++) hc[m] = 0.0;
+        if (ln >= 1 && ln <= NS) Wi = th->Wi[ln - 1], iWi = th->_Wi[ln - 1], Ri = th->Ri[ln - 1];
+#pragma unroll
+        for (int it = 0; it < NS; it++) eff[it] = 0.0;
+        if (ln < NR) {
+            R = mc->rx[ln];
+#pragma unroll
+            for (int it = 0; it < NS; it++) eff[it] = mc->third[ln * NS + it];
+        }
+    }
+    // the fit coefficients of range r (the same on every lane: T is the cell's)
+    __device__ __forceinline__ void range(const DevThermo *__restrict__ th, int r)
+    {
+        if (r != rc) {
+            const int ln = threadIdx.x & 63;
+            if (ln >= 1 && ln <= NS) {
+                const double *c = fit_coeffs(th, r, ln - 1);
+#pragma unroll
+                for (int m = 0; m < 9; m++) hc[m] = c[m];
+            }
+            rc = r;
+        }
+    }
+};
+
+// evalcv (ProductionRates.h:459-479) at the distributed state: yv is
+// component ln, m_tmp = T; returns the component's q and d
+template <int NS, int THERMO>
+__device__ __forceinline__ void evalcv_wave(const DevThermo *__restrict__ th, const DevMech *__restrict__ mc,
+                                            WaveLane<NS> &W, WaveShared<NS> &sh, int NR, double yv, double m_tmp,
+                                            double &qv, double &dv, double m_dens, double m_p)
+{
+    const int ln = threadIdx.x & 63;
+    const int n = ln - 1;
+    const bool sp = ln >= 1 && ln <= NS;
+    // THERMO: the fit family (1 NASA-9, 0 JANAF) as a constant, so that each
+    // formula reads the coefficients at fixed positions (kept in registers)
+    constexpr int thermo = THERMO;
+    const double _dens = 1.0 / m_dens;
+    const TPre t = tpre<true>(m_tmp, thermo);
+    W.range(th, t.r);
+    double h1 = 0.0;
+    if (sp) {
+        // h_k and s_k with Ri = 1
+        h1 = h_fit(W.hc, thermo, t, 1.0);
+        if (t.TT < 200.0) h1 += cp200_k(th, 1.0, n) * (t.TT - 200.0);
+        double temp = 0.0;
+        temp += s_fit(W.hc, thermo, t, 1.0);
+        temp += (-h1 / m_tmp);
+        sh.S[n] = temp;
+        sh.S[MAXNS + n] = yv * W.iWi * m_dens * 1.0E-3;
+    }
+    const double logStandConc = dlog(m_p / (R0 * m_tmp));
+    // log T: the fit's log max(T, 200) when T >= 200 (the same call on the same value)
+    const double logT = m_tmp >= 200.0 ? t.lT : dlog(m_tmp);
+    const double logP = dlog(m_p * (1.0 / 101325.0));
+    __syncthreads();
+    double Kf = 0.0, Kck = 0.0;
+    if (ln < NR) {
+        double gka[NS], mca[NS];
+#pragma unroll
+        for (int k = 0; k < NS; k++) gka[k] = sh.S[k], mca[k] = sh.S[MAXNS + k];
+        RxnCtx c;
+        c.m_tmp = m_tmp, c.logT = logT, c.logP = logP, c.logStandConc = logStandConc;
+        c.rT = 1.0 / m_tmp;
+        c.t_ok = m_tmp >= 0x1p-500 && m_tmp <= 0x1p500;
+        rxn_rates<NS>(W.R, [&W](int it) { return W.eff[it]; }, c, gka, mca, Kf, Kck);
+        sh.K[ln] = Kf, sh.K[MAXNR + ln] = Kck;
+    }
+    __syncthreads();
+    // species n's sums over the reactions, in increasing order (rxn_accum)
+    double qi = 0.0, pi = 0.0;
+    if (sp) {
+        // a reaction adds to species n exactly when its nu_f or nu_b is non-zero
+        for (int r = 0; r < NR; r++) {
+            const unsigned b = sh.nu[64 * r + ln];
+            if (b) {
+                const int nf = (int)(b & 15u), nb = (int)(b >> 4);
+                const double kf = sh.K[r], kc = sh.K[MAXNR + r];
+                qi += nb * kf + nf * kc;
+                pi += nf * kf + nb * kc;
+            }
+        }
+        qi *= W.Wi * _dens;
+        pi *= W.Wi * _dens;
+        const double ei = (h1 - m_tmp) * W.Ri;
+        sh.V[4 * ln + 0] = qi * ei;
+        sh.V[4 * ln + 1] = pi * ei;
+        sh.V[4 * ln + 2] = yv * cp_fit(W.hc, thermo, t, W.Ri);
+        sh.V[4 * ln + 3] = yv * W.iWi;
+    }
+    __syncthreads();
+    const double q0 = lds_sub(sh.V + 0, 4, 1, NS + 1, 0.0);
+    const double d0 = lds_sub(sh.V + 1, 4, 1, NS + 1, 0.0);
+    // mix_cv: sum Y_n Cp_n, sum Y_n / W_n, both in species order
+    double cv = lds_sum(sh.V + 2, 4, 1, NS + 1, 0.0);
+    const double _W = lds_sum(sh.V + 3, 4, 1, NS + 1, 0.0);
+    __syncthreads(); // V is written again before the next read
+    cv -= Ru * _W;
+    const double _Cv = 1.0 / cv;
+    qv = ln == 0 ? q0 * _Cv : qi;
+    dv = ln == 0 ? d0 * _Cv : pi;
+}
+
+// get_Cope at the distributed state (mix_e's order)
+template <int NS, int THERMO>
+__device__ __forceinline__ double mix_e_wave(const DevThermo *__restrict__ th, WaveShared<NS> &sh, double yv, double T)
+{
+    const int ln = threadIdx.x & 63;
+    if (ln >= 1 && ln <= NS) {
+        const TPre t = THERMO ? tpre<true>(T, 1) : tpre<false>(T, 0);
+        const double ei = (h_k(th, t, 1.0, ln - 1) - T) * th->Ri[ln - 1];
+        sh.V[ln] = ei * yv;
+    }
+    __syncthreads();
+    const double e = lds_sum(sh.V, 1, 1, NS + 1, 0.0);
+    __syncthreads();
+    return e;
+}
+
+// T, held by lane 0, to every lane
+template <int NS> __device__ __forceinline__ double wave_T(WaveShared<NS> &sh, double yv)
+{
+    if ((threadIdx.x & 63) == 0) sh.V[0] = yv;
+    __syncthreads();
+    const double T = sh.V[0];
+    __syncthreads();
+    return T;
+}
+
+// ChemeODEQ2SolverKernel1 + Chemeq2 for interior cell tl of case cc on one
+// wave; every branch below is uniform across the wave
+template <int NS, int THERMO>
+__device__ __forceinline__ void chem_wave_cell(const DevThermo *__restrict__ th, const DevMech *__restrict__ mc,
+                                               const ChemCase &cc, long long tl, int record, int check,
+                                               WaveShared<NS> &sh)
+{
+    constexpr int NL = NS + 1;
+    const int ln = threadIdx.x & 63;
+    const bool comp = ln < NL;
+    const long long N = cc.g.N;
+    const long long id = chem_cell_id(cc.g, tl);
+    const double rho = cc.U[id];
+    const double rho1 = 1.0 / rho;
+    const double m_p = cc.p[id];
+    const int NR = mc->NR;
+    // the start values: T, then Getrhoyi's fractions (chem_y0)
+    double yv = 0.0;
+    if (ln == 0) yv = cc.T[id];
+    if (ln >= 1 && ln < NS) yv = cc.g.ghost ? cc.y[(ln - 1) * N + id] : cc.U[(4 + ln) * N + id] * rho1;
+    if (cc.g.ghost) {
+        if (ln == NS) yv = 0.0;
+    } else {
+        sh.V[ln] = yv;
+        __syncthreads();
+        const double rem = lds_sub(sh.V, 1, 1, NS, 1.0); // ((1 - Y_0) - Y_1) - ...
+        __syncthreads();
+        if (ln == NS) yv = rem;
+    }
+    const double e0 = mix_e_wave<NS, THERMO>(th, sh, yv, wave_T<NS>(sh, yv));
+    WaveLane<NS> W;
+    W.load(th, mc, NR);
+    for (int r = 0; r < NR; r++) {
+        unsigned v = 0;
+        if (ln >= 1 && ln <= NS) {
+            const unsigned long long pf = mc->rx[r].pf, pb = mc->rx[r].pb;
+            v = (unsigned)((pf >> (4 * (ln - 1))) & 15ull) | ((unsigned)((pb >> (4 * (ln - 1))) & 15ull) << 4);
+        }
+        sh.nu[64 * r + ln] = (unsigned char)v;
+    }
+    // Chemeq2 (ProductionRates.h:488-638), the steps of ChemLane
+    const double ymin = 1.0e-20, tfd = 1.0 + 1.0e-10, epscl = 1.0e4;
+    const double dtg = cc.dt[DT_CHEM];
+    double dt = 0.0, dtmin = 1.0e-7, epsmax = 1.0, ts = 0.0, tn = 0.0;
+    double qv = 0.0, dv = 0.0, ys = 0.0, qs = 0.0, ds = 0.0, rtau = 0.0;
+    int passes = 0, st = 0;
+    bool started = false;
+    for (;;) {
+        if (!started) {
+            if (ln >= 1 && comp) yv = dmax(yv, ymin);
+            sh.V[ln] = yv;
+            __syncthreads();
+            double sumy = lds_sum(sh.V, 1, 1, NL, 0.0);
+            __syncthreads();
+            sumy = 1.0 / sumy;
+            if (ln >= 1 && comp) yv *= sumy;
+        } else {
+            if (++passes > CHEM_MAXSUB) {
+                st = -1;
+                break;
+            }
+            if (comp) {
+                const double rt = dt * ds / ys;
+                rtau = rt;
+                const double alpha = alpha_of(rt);
+                const double scr = dt * (qs - ds) / (1.0 + alpha * rt);
+                yv = dmax(ys + scr, ymin);
+            }
+            tn = ts + dt;
+        }
+        evalcv_wave<NS, THERMO>(th, mc, W, sh, NR, yv, wave_T<NS>(sh, yv), qv, dv, rho, m_p);
+        if (!started) {
+            const double epsmin = 1.0e-4, sqreps = 0.05;
+            if (comp) {
+                const double ascr = fabs(qv);
+                const double sb = 0.1 * epsmin * ascr - dv;
+                const double scr2 = (sb >= 0.0 ? 1.0 : -1.0) * fabs(1.0 / yv);
+                sh.V[4 * ln + 0] = -fabs(ascr - dv) * scr2;
+                sh.V[4 * ln + 1] = scr2 * dv;
+            }
+            __syncthreads();
+            double scrtch = 1e-25;
+            for (int i = 0; i < NL; i++) scrtch = dmax(sh.V[4 * i], dmax(sh.V[4 * i + 1], scrtch));
+            __syncthreads();
+            dt = dmin(sqreps / scrtch, dtg);
+            dtmin *= dt;
+            ts = 0.0, tn = 0.0;
+            ys = yv, qs = qv, ds = dv;
+            started = true;
+            continue;
+        }
+        // corrector and the error norm (the maximum in component order)
+        if (comp) {
+            const double yp = yv;
+            const double rtaub = 0.5 * (rtau + dt * dv / yp);
+            const double alpha = alpha_of(rtaub);
+            const double ysi = ys;
+            const double qt = (1.0 - alpha) * qs + alpha * qv;
+            const double scr = (qt * dt - rtaub * ysi) / (1.0 + alpha * rtaub);
+            const double scr2 = dmax(ysi + scr, 0.0);
+            double scr1 = fabs(scr2 - yp);
+            yv = dmax(scr2, ymin);
+            double cand = 0.0, has = 0.0;
+            if ((0.25 * (ysi + yv)) > ymin) {
+                scr1 = scr1 / yv;
+                cand = 0.5 * (scr1 + dmin(fabs(qv - dv) / (qv + dv + 1.0e-30), scr1));
+                has = 1.0;
+            }
+            sh.V[4 * ln + 0] = cand;
+            sh.V[4 * ln + 1] = has;
+        }
+        __syncthreads();
+        double eps = 1e-10;
+        for (int i = 0; i < NL; i++)
+            if (sh.V[4 * i + 1] != 0.0) eps = dmax(sh.V[4 * i], eps);
+        __syncthreads();
+        eps = eps * epscl;
+        if (eps < epsmax) {
+            if (dtg <= (tn * tfd)) {
+                st = 1;
+                break;
+            }
+        } else {
+            tn = ts;
+        }
+        const double dto = dt;
+        double rteps = 0.5 * (eps + 1.0);
+        rteps = 0.5 * (rteps + eps / rteps);
+        rteps = 0.5 * (rteps + eps / rteps);
+        dt = dmin(dt * (1.0 / rteps + 0.005), tfd * (dtg - tn));
+        if (eps > epsmax) {
+            dt = dmin(dt, 0.34 * dto);
+            if (dt <= dtmin) epsmax *= 10.0;
+            continue;
+        }
+        epsmax = 1.0;
+        ts = tn;
+        ys = yv, qs = qv, ds = dv;
+    }
+    // write-back (ChemeODEQ2SolverKernel1's tail), the pass count, the checks
+    const double T1 = wave_T<NS>(sh, yv);
+    const double e1 = mix_e_wave<NS, THERMO>(th, sh, yv, T1);
+    bool bad = false;
+    if (ln >= 1 && ln < NS) {
+        const double v = yv * rho;
+        cc.U[(4 + ln) * N + id] = v;
+        bad = isnan(v) || isinf(v);
+    }
+    if (ln == 0) {
+        cc.T[id] = yv;
+        const double u4 = cc.U[4 * N + id] + (e1 - e0) * rho;
+        cc.U[4 * N + id] = u4;
+        if (record) cc.substeps[tl] = st < 0 ? -1 : passes;
+        bad = rho < 0 || isnan(rho) || isinf(rho) || isnan(u4) || isinf(u4);
+        for (int k = 1; k < 4; k++) bad = bad || isnan(cc.U[k * N + id]) || isinf(cc.U[k * N + id]);
+    }
+    const bool any_bad = __any(bad);
+    if (ln == 0) {
+        int bits = st < 0 ? EBIT_CHEM : 0;
+        if (st < 0) report_cell(cc.err, EBIT_CHEM, id);
+        if (check && any_bad) {
+            report_cell(cc.err, EBIT_NAN, id);
+            bits |= EBIT_NAN;
+        }
+        if (bits) report_pass(cc.err, bits, 4);
+    }
+}
+
+// one interior cell per one-wave block; THERMO = the context's fit family
+template <int NS, int THERMO>
+__global__ void __launch_bounds__(64) k_chem_wave(Geom g, const DevThermo *__restrict__ th,
+                                                  const DevMech *__restrict__ mc, Fields f, int record, int check)
+{
+    const long long tl = blockIdx.x;
+    if (tl >= (long long)g.X_inner * g.Y_inner * g.Z_inner) return;
+    if (step_halted(f.err, ~EBIT_CHEM)) return;
+    __shared__ WaveShared<NS> sh;
+    chem_wave_cell<NS, THERMO>(th, mc, one_case(g, f).c, tl, record, check, sh);
 }
 
 // zero the work-queue counter (a kernel, so the step graph holds no memset)

@@ -255,6 +255,13 @@ template <int NS, bool COP> struct Launch {
                     if (cap > 0)
                         hipLaunchKernelGGL((k_chem_coop<NS>), dim3(coop_blocks(cap)), dim3(64), 0, c->stream, c->g,
                                            c->d_th, c->d_mc, c->f, record, check, c->spill, cap);
+                } else if (c->d.chem_driver == XF_CHEM_WAVE) {
+                    if (c->d.thermo)
+                        hipLaunchKernelGGL((k_chem_wave<NS, 1>), dim3((unsigned)c->Ninner), dim3(64), 0, c->stream,
+                                           c->g, c->d_th, c->d_mc, c->f, record, check);
+                    else
+                        hipLaunchKernelGGL((k_chem_wave<NS, 0>), dim3((unsigned)c->Ninner), dim3(64), 0, c->stream,
+                                           c->g, c->d_th, c->d_mc, c->f, record, check);
                 } else {
                     hipLaunchKernelGGL((k_chem<NS, S>), dim3(nblk(c->Ninner)), dim3(256), 0, c->stream, c->g,
                                        c->d_th, c->d_mc, c->f, record, check);

@@ -43,21 +43,61 @@ __device__ __forceinline__ TPre tpre(double T0, int thermo)
     return t;
 }
 
-// Hia[n * 21 + m * 3 + r] and Hib[n * 6 + m * 3 + r], read from the regrouped copy
-#define XF_HA(m) th->Hc[(r * MAXNS + n) * 9 + (m)]
-#define XF_HB(m) th->Hc[(r * MAXNS + n) * 9 + 7 + (m)]
+// The fit formulas on the nine coefficients c of one fit range of one
+// species: Hia[n * 21 + m * 3 + r] (m = 0..6) then Hib[n * 6 + m * 3 + r]
+// (m = 0, 1), as the regrouped copy Hc[(r * MAXNS + n) * 9 + ...] holds them.
+// c comes from the table (cp_k, h_k, s_k) or from registers a wave loaded
+// once (the one-wave-per-cell chemistry): the same expressions either way.
 
 // HeatCapacity_NASA / HeatCapacity_JANAF (Thermo_device.h:10-40)
+__device__ __forceinline__ double cp_fit(const double *c, int thermo, const TPre &t, double Ri)
+{
+    const double T = t.T;
+    if (thermo) {
+        if (t.r < 0) return 0.0;
+        return Ri * ((c[0] * t.iT + c[1]) * t.iT + c[2] + (c[3] + (c[4] + (c[5] + c[6] * T) * T) * T) * T);
+    }
+    return Ri * (c[0] + (c[1] + (c[2] + (c[3] + c[4] * T) * T) * T) * T);
+}
+
+// get_Enthalpy_NASA / get_Enthalpy_JANAF (Thermo_device.h:62-102) without the
+// extrapolation below 200 K (h_k adds it)
+__device__ __forceinline__ double h_fit(const double *c, int thermo, const TPre &t, double Ri)
+{
+    const double T = t.T;
+    double hi = 0.0;
+    if (thermo) {
+        if (t.r >= 0)
+            hi = Ri * (-c[0] / T + c[1] * t.lT + (c[2] + (0.5 * c[3] + (c[4] * OT + (0.25 * c[5] + 0.2 * c[6] * T) * T) * T) * T) * T +
+                       c[7]);
+    } else {
+        hi = Ri * (T * (c[0] + T * (c[1] * 0.5 + T * (c[2] * OT + T * (c[3] * 0.25 + c[4] * T * 0.2)))) + c[5]);
+    }
+    return hi;
+}
+
+// get_Entropy_NASA / get_Entropy_JANAF (Thermo_device.h:132-161)
+__device__ __forceinline__ double s_fit(const double *c, int thermo, const TPre &t, double Ri)
+{
+    const double T = t.T;
+    if (thermo) {
+        if (t.r < 0) return 0.0;
+        return Ri * ((-0.5 * c[0] * t.iT - c[1]) * t.iT + c[2] * t.lT +
+                     (c[3] + (0.5 * c[4] + (c[5] * OT + c[6] * 0.25 * T) * T) * T) * T + c[8]);
+    }
+    return Ri * (c[0] * t.lT + (c[1] + (0.5 * c[2] + (c[3] * OT + c[4] * 0.25 * T) * T) * T) * T + c[6]);
+}
+
+// species n's coefficients of range t.r in the table (range 0 when t.r < 0,
+// where the NASA formulas return before reading them)
+__device__ __forceinline__ const double *fit_coeffs(const DevThermo *__restrict__ th, int r, int n)
+{
+    return th->Hc + ((r < 0 ? 0 : r) * MAXNS + n) * 9;
+}
+
 __device__ __forceinline__ double cp_k(const DevThermo *__restrict__ th, const TPre &t, double Ri, int n)
 {
-    const int r = t.r;
-    const double T = t.T;
-    if (th->thermo) {
-        if (r < 0) return 0.0;
-        return Ri * ((XF_HA(0) * t.iT + XF_HA(1)) * t.iT + XF_HA(2) +
-                     (XF_HA(3) + (XF_HA(4) + (XF_HA(5) + XF_HA(6) * T) * T) * T) * T);
-    }
-    return Ri * (XF_HA(0) + (XF_HA(1) + (XF_HA(2) + (XF_HA(3) + XF_HA(4) * T) * T) * T) * T);
+    return cp_fit(fit_coeffs(th, t.r, n), th->thermo, t, Ri);
 }
 
 // Cp of species n at exactly 200 K (used by the low-temperature extrapolation)
@@ -72,20 +112,9 @@ __device__ __forceinline__ double cp200_k(const DevThermo *__restrict__ th, doub
     return cp_k(th, t, Ri, n);
 }
 
-// get_Enthalpy_NASA / get_Enthalpy_JANAF (Thermo_device.h:62-102)
 __device__ __forceinline__ double h_k(const DevThermo *__restrict__ th, const TPre &t, double Ri, int n)
 {
-    const int r = t.r;
-    const double T = t.T;
-    double hi = 0.0;
-    if (th->thermo) {
-        if (r >= 0)
-            hi = Ri * (-XF_HA(0) / T + XF_HA(1) * t.lT +
-                       (XF_HA(2) + (0.5 * XF_HA(3) + (XF_HA(4) * OT + (0.25 * XF_HA(5) + 0.2 * XF_HA(6) * T) * T) * T) * T) * T +
-                       XF_HB(0));
-    } else {
-        hi = Ri * (T * (XF_HA(0) + T * (XF_HA(1) * 0.5 + T * (XF_HA(2) * OT + T * (XF_HA(3) * 0.25 + XF_HA(4) * T * 0.2)))) + XF_HA(5));
-    }
+    double hi = h_fit(fit_coeffs(th, t.r, n), th->thermo, t, Ri);
     if (t.TT < 200.0) {
         const double Cpi = cp200_k(th, Ri, n);
         hi += Cpi * (t.TT - 200.0);
@@ -93,21 +122,10 @@ __device__ __forceinline__ double h_k(const DevThermo *__restrict__ th, const TP
     return hi;
 }
 
-// get_Entropy_NASA / get_Entropy_JANAF (Thermo_device.h:132-161)
 __device__ __forceinline__ double s_k(const DevThermo *__restrict__ th, const TPre &t, double Ri, int n)
 {
-    const int r = t.r;
-    const double T = t.T;
-    if (th->thermo) {
-        if (r < 0) return 0.0;
-        return Ri * ((-0.5 * XF_HA(0) * t.iT - XF_HA(1)) * t.iT + XF_HA(2) * t.lT +
-                     (XF_HA(3) + (0.5 * XF_HA(4) + (XF_HA(5) * OT + XF_HA(6) * 0.25 * T) * T) * T) * T + XF_HB(1));
-    }
-    return Ri * (XF_HA(0) * t.lT + (XF_HA(1) + (0.5 * XF_HA(2) + (XF_HA(3) * OT + XF_HA(4) * 0.25 * T) * T) * T) * T + XF_HA(6));
+    return s_fit(fit_coeffs(th, t.r, n), th->thermo, t, Ri);
 }
-
-#undef XF_HA
-#undef XF_HB
 
 // get_CopR (Mixing_device.h:49-56)
 template <int NS>

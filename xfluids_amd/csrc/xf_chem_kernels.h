@@ -487,12 +487,14 @@ __device__ __forceinline__ double chem_spill_load(const ChemSpill<NS> &r, ChemLa
     return w[8];
 }
 
-// Chemeq2 on one cell.  Returns predictor passes, or -1 at the safety bound.
+// Chemeq2 on one cell, its saved state at lds (lane-strided, ChemLds).
+// Returns predictor passes, or -1 at the safety bound.
 template <int NS, int SPEC>
 __device__ int chemeq2_dev(const DevThermo *__restrict__ th, const DevMech *__restrict__ mc, double *y, double dtg,
-                           double rho, double m_p)
+                           double rho, double m_p, double *lds)
 {
-    ChemLane<NS, ChemRegs<NS + 1>, SPEC> L;
+    ChemLane<NS, ChemLds<NS + 1>, SPEC> L;
+    L.st.p = lds;
 #pragma unroll
     for (int i = 0; i < NS + 1; i++) L.y[i] = y[i];
     L.begin(dtg, rho, m_p);
@@ -544,23 +546,26 @@ __device__ __forceinline__ void chem_y0(const double *U, const double *ypl, int 
 template <int NS, int SPEC>
 __device__ __forceinline__ int chem_cell(const Geom &g, const DevThermo *__restrict__ th,
                                          const DevMech *__restrict__ mc, Fields &f, long long N, long long id,
-                                         double dtc)
+                                         double dtc, double *lds)
 {
-    constexpr int E = NS + 4;
-    double y[NS + 1];
-    y[0] = f.T[id];
+    // the cell's CHEMEQ2 state held directly (no copy of y in and out)
+    ChemLane<NS, ChemLds<NS + 1>, SPEC> L;
+    L.st.p = lds;
+    L.y[0] = f.T[id];
     const double rho = f.U[id]; // rho = U[0] (Getrhoyi)
     const double rho1 = 1.0 / rho;
-    chem_y0<NS>(f.U, f.y, g.ghost, N, id, rho1, y + 1);
-    const double e0 = mix_e<NS>(th, y + 1, y[0]);
-    const int passes = chemeq2_dev<NS, SPEC>(th, mc, y, dtc, rho, f.p[id]);
-    const double e1 = mix_e<NS>(th, y + 1, y[0]);
+    chem_y0<NS>(f.U, f.y, g.ghost, N, id, rho1, L.y + 1);
+    const double e0 = mix_e<NS>(th, L.y + 1, L.y[0]);
+    L.begin(dtc, rho, f.p[id]);
+    int st;
+    while ((st = L.advance(th, mc)) == 0) {
+    }
+    const double e1 = mix_e<NS>(th, L.y + 1, L.y[0]);
 #pragma unroll
-    for (int n = 0; n < NS - 1; n++) f.U[(5 + n) * N + id] = y[1 + n] * rho;
-    f.T[id] = y[0];
+    for (int n = 0; n < NS - 1; n++) f.U[(5 + n) * N + id] = L.y[1 + n] * rho;
+    f.T[id] = L.y[0];
     f.U[4 * N + id] += (e1 - e0) * rho;
-    (void)E;
-    return passes;
+    return st < 0 ? -1 : L.passes;
 }
 
 // EstimateNAN after the Reaction (flag 4, src/XFLUIDS.cpp:639;
@@ -578,12 +583,26 @@ __device__ __forceinline__ bool cell_bad(const double *U, long long N, long long
     return bad;
 }
 
-// one lane per interior cell; check = 1 (a time step's Reaction) also runs
-// the post-Reaction NaN check on the cell
+// two waves per SIMD for the 9-species state (<= 256 registers per lane)
+template <int NS> struct ChemOccupancy {
+    static constexpr int waves = NS <= 3 ? 3 : 2;
+};
+
+// LDS of one one-wave block: ys, qs, ds and rtau of its 64 lanes, 20 KiB at
+// NS = 9, so the 8 blocks of two waves per SIMD fill the CU's 160 KiB
+template <int NS> struct ChemQueueLds {
+    static constexpr int doubles = 4 * (NS + 1) * 64;
+};
+
+// one lane per interior cell, one-wave blocks; check = 1 (a time step's
+// Reaction) also runs the post-Reaction NaN check on the cell.  The saved
+// CHEMEQ2 state (ys, qs, ds, rtau) lives in LDS: in registers, with the rate
+// evaluation's, it needs ~380 per lane and the kernel ran one wave per SIMD.
 template <int NS, int SPEC>
-__global__ void __launch_bounds__(256) k_chem(Geom g, const DevThermo *__restrict__ th,
-                                              const DevMech *__restrict__ mc, Fields f, int record, int check)
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ChemOccupancy<NS>::waves)))
+k_chem(Geom g, const DevThermo *__restrict__ th, const DevMech *__restrict__ mc, Fields f, int record, int check)
 {
+    __shared__ double lds[ChemQueueLds<NS>::doubles];
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long total = (long long)g.X_inner * g.Y_inner * g.Z_inner;
     if (t >= total) return;
@@ -592,7 +611,7 @@ __global__ void __launch_bounds__(256) k_chem(Geom g, const DevThermo *__restric
     const int j = g.By + (int)((t / g.X_inner) % g.Y_inner);
     const int k = g.Bz + (int)(t / ((long long)g.X_inner * g.Y_inner));
     const long long id = (long long)g.Xmax * g.Ymax * k + (long long)g.Xmax * j + i;
-    const int passes = chem_cell<NS, SPEC>(g, th, mc, f, g.N, id, f.dt[DT_CHEM]);
+    const int passes = chem_cell<NS, SPEC>(g, th, mc, f, g.N, id, f.dt[DT_CHEM], lds + (threadIdx.x & 63));
     if (record) f.substeps[t] = passes;
     int bits = passes < 0 ? EBIT_CHEM : 0;
     if (passes < 0) report_cell(f.err, EBIT_CHEM, id);
@@ -790,16 +809,6 @@ struct ManyCases {
     __device__ __forceinline__ ChemCase get(int k) const { return cases[k]; }
 };
 
-// two waves per SIMD for the 9-species state (<= 256 registers per lane)
-template <int NS> struct ChemOccupancy {
-    static constexpr int waves = NS <= 3 ? 3 : 2;
-};
-
-// LDS of one queue block (one wave): ys, qs, ds and rtau of its 64 lanes,
-// 20 KiB at NS = 9, so the 8 blocks of two waves per SIMD fill the CU's 160 KiB
-template <int NS> struct ChemQueueLds {
-    static constexpr int doubles = 4 * (NS + 1) * 64;
-};
 
 __device__ __forceinline__ OneCase one_case(const Geom &g, const Fields &f)
 {
@@ -946,7 +955,8 @@ template <int NS> struct WaveLane {
 template <int NS, int THERMO>
 __device__ __forceinline__ void evalcv_wave(const DevThermo *__restrict__ th, const DevMech *__restrict__ mc,
                                             WaveLane<NS> &W, WaveShared<NS> &sh, int NR, double yv, double m_tmp,
-                                            double &qv, double &dv, double m_dens, double m_p)
+                                            double &qv, double &dv, double m_dens, double m_p, double _dens,
+                                            double logP)
 {
     const int ln = threadIdx.x & 63;
     const int n = ln - 1;
@@ -954,7 +964,6 @@ __device__ __forceinline__ void evalcv_wave(const DevThermo *__restrict__ th, co
     // THERMO: the fit family (1 NASA-9, 0 JANAF) as a constant, so that each
     // formula reads the coefficients at fixed positions (kept in registers)
     constexpr int thermo = THERMO;
-    const double _dens = 1.0 / m_dens;
     const TPre t = tpre<true>(m_tmp, thermo);
     W.range(th, t.r);
     double h1 = 0.0;
@@ -971,7 +980,6 @@ __device__ __forceinline__ void evalcv_wave(const DevThermo *__restrict__ th, co
     const double logStandConc = dlog(m_p / (R0 * m_tmp));
     // log T: the fit's log max(T, 200) when T >= 200 (the same call on the same value)
     const double logT = m_tmp >= 200.0 ? t.lT : dlog(m_tmp);
-    const double logP = dlog(m_p * (1.0 / 101325.0));
     __syncthreads();
     double Kf = 0.0, Kck = 0.0;
     if (ln < NR) {
@@ -980,7 +988,7 @@ __device__ __forceinline__ void evalcv_wave(const DevThermo *__restrict__ th, co
         for (int k = 0; k < NS; k++) gka[k] = sh.S[k], mca[k] = sh.S[MAXNS + k];
         RxnCtx c;
         c.m_tmp = m_tmp, c.logT = logT, c.logP = logP, c.logStandConc = logStandConc;
-        c.rT = 1.0 / m_tmp;
+        c.rT = m_tmp >= 200.0 ? t.iT : 1.0 / m_tmp; // the fit's 1 / max(T, 200) when T >= 200
         c.t_ok = m_tmp >= 0x1p-500 && m_tmp <= 0x1p500;
         rxn_rates<NS>(W.R, [&W](int it) { return W.eff[it]; }, c, gka, mca, Kf, Kck);
         sh.K[ln] = Kf, sh.K[MAXNR + ln] = Kck;
@@ -1062,6 +1070,9 @@ __device__ __forceinline__ void chem_wave_cell(const DevThermo *__restrict__ th,
     const double rho1 = 1.0 / rho;
     const double m_p = cc.p[id];
     const int NR = mc->NR;
+    // evalcv's 1 / rho and log(p / p_atm): constants of the cell
+    const double _dens = 1.0 / rho;
+    const double logP = dlog(m_p * (1.0 / 101325.0));
     // the start values: T, then Getrhoyi's fractions (chem_y0)
     double yv = 0.0;
     if (ln == 0) yv = cc.T[id];
@@ -1116,7 +1127,7 @@ __device__ __forceinline__ void chem_wave_cell(const DevThermo *__restrict__ th,
             }
             tn = ts + dt;
         }
-        evalcv_wave<NS, THERMO>(th, mc, W, sh, NR, yv, wave_T<NS>(sh, yv), qv, dv, rho, m_p);
+        evalcv_wave<NS, THERMO>(th, mc, W, sh, NR, yv, wave_T<NS>(sh, yv), qv, dv, rho, m_p, _dens, logP);
         if (!started) {
             const double epsmin = 1.0e-4, sqreps = 0.05;
             if (comp) {

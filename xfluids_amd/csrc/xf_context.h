@@ -263,7 +263,7 @@ template <int NS, bool COP> struct Launch {
                         hipLaunchKernelGGL((k_chem_wave<NS, 0>), dim3((unsigned)c->Ninner), dim3(64), 0, c->stream,
                                            c->g, c->d_th, c->d_mc, c->f, record, check);
                 } else {
-                    hipLaunchKernelGGL((k_chem<NS, S>), dim3(nblk(c->Ninner)), dim3(256), 0, c->stream, c->g,
+                    hipLaunchKernelGGL((k_chem<NS, S>), dim3(nblk(c->Ninner, 64)), dim3(64), 0, c->stream, c->g,
                                        c->d_th, c->d_mc, c->f, record, check);
                 }
             };

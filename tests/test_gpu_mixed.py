@@ -1,5 +1,6 @@
 """Mixed-precision form (FP32 smoothness indicators, weights and
-characteristic projections; FP64 everywhere else) against the FP64 oracle.
+characteristic projections, and FP32 exponentials and Wilke factors in the
+transport coefficients; FP64 everywhere else) against the FP64 oracle.
 
 Bar, after the parity cases' step counts: L1 relative error of density,
 pressure and temperature at most 1e-6, and the mass fractions no further

@@ -438,6 +438,9 @@ int xf_set_transport(xf_ctx *c, const double *fcv, const double *fct, const doub
             const double Wk = c->h_th.Wi[k], Wj = c->h_th.Wi[j];
             h.phiA[k * MAXNS + j] = std::sqrt(std::sqrt(Wj / Wk));
             h.phiB[k * MAXNS + j] = 1.0 / std::sqrt(1.0 + Wk / Wj);
+            // the mixed form's FP32 factors, phiB with the constant 0.5 / sqrt(2)
+            h.phiAf[k * MAXNS + j] = (float)h.phiA[k * MAXNS + j];
+            h.phiBcf[k * MAXNS + j] = (float)(h.phiB[k * MAXNS + j] * (0.5 / std::sqrt(2.0)));
         }
     XF_HIP(hipMemcpy(c->d_tr, &h, sizeof(DevTransport), hipMemcpyHostToDevice));
     c->transport_set = true;

@@ -333,8 +333,12 @@ template <int NS, bool COP> struct Launch {
         if constexpr (COP) {
             ProfScope ps(c, "visc_prep");
             const Geom &g = c->g;
-            hipLaunchKernelGGL((k_transport<NS>), dim3(nblk(c->N)), dim3(256), 0, c->stream, g, c->d_th, c->d_tr, Ug,
-                               c->f, c->d.visc_heat, c->d.visc_diffu);
+            if (c->d.precision == XF_MIXED)
+                hipLaunchKernelGGL((k_transport<NS, 1>), dim3(nblk(c->N)), dim3(256), 0, c->stream, g, c->d_th,
+                                   c->d_tr, Ug, c->f, c->d.visc_heat, c->d.visc_diffu);
+            else
+                hipLaunchKernelGGL((k_transport<NS, 0>), dim3(nblk(c->N)), dim3(256), 0, c->stream, g, c->d_th,
+                                   c->d_tr, Ug, c->f, c->d.visc_heat, c->d.visc_diffu);
             if (c->d.visc_diffu)
                 hipLaunchKernelGGL((k_yreduce<NS>), dim3(2 * NS), dim3(1024), 0, c->stream, c->f.ypart,
                                    (int)nblk(c->N), c->f.red);
@@ -349,10 +353,13 @@ template <int NS, bool COP> struct Launch {
     template <int DIR> static void visc_flux_d(xf_ctx *c, const double *Ug)
     {
         const Geom &g = c->g;
-        const long long n = (long long)(g.X_inner + (DIR == 0)) * (g.Y_inner + (DIR == 1)) * (g.Z_inner + (DIR == 2));
+        const long long plane = (long long)(g.X_inner + (DIR == 0)) * (g.Y_inner + (DIR == 1));
+        const long long nz = g.Z_inner + (DIR == 2);
+        // z: (chunks of 256 faces of a plane) x (face planes), as the kernel maps them
+        const unsigned blocks = DIR == 2 ? nblk(plane) * (unsigned)nz : nblk(plane * nz);
         static const char *names[3] = {"visc_x", "visc_y", "visc_z"};
         ProfScope ps(c, names[DIR]);
-        hipLaunchKernelGGL((k_visc_flux<NS, DIR>), dim3(nblk(n)), dim3(256), 0, c->stream, g, Ug, c->f,
+        hipLaunchKernelGGL((k_visc_flux<NS, DIR>), dim3(blocks), dim3(256), 0, c->stream, g, Ug, c->f,
                            c->d.visc_heat, c->d.visc_diffu, c->d.Yil_limiter, c->d.Dim_limiter);
     }
     static void visc_flux(xf_ctx *c, int dir, const double *Ug)

@@ -4,15 +4,18 @@
 // fluxes (src/solver_Reconstruction/FDM_Method/ConVenction_block.hpp:424-575).
 //
 // Kernels, in launch order for one GetLU:
-//   k_transport<NS>   every padded cell: u, v, w, Wilke mu and lambda, D_km;
+//   k_transport<NS,PREC> every padded cell: u, v, w, Wilke mu and lambda, D_km
+//                     (PREC 1: the mixed form's FP32 exponentials and factors);
 //                     per-block min/max of each interior mass fraction
 //   k_yreduce<NS>     one block: those partials into the reduction slots
 //   k_vel_deriv       fourth-order centre derivatives of u, v, w, with the
 //                     ghost-cell rules for the values each wall flux reads
 //   k_visc_flux<NS,DIR> per face of one direction; the sweep of that
 //                     direction subtracts the result from its face flux
-// All arithmetic is FP64 in the reference's operation order, with the same
-// exp/log as the oracle, so the plain form stays bit-identical.
+// All arithmetic of the plain form is FP64 in the reference's operation
+// order, with the same exp/log as the oracle, so it stays bit-identical; the
+// mixed form's transport coefficients take FP32 where the FP64 path spends
+// its instructions (transport_mixed).
 #pragma once
 #include "xf_fluid_kernels.h"
 
@@ -24,6 +27,7 @@ namespace xf {
 struct DevTransport {
     double fcv[MAXNS * 4], fct[MAXNS * 4], Dkj[MAXNS * MAXNS * 4];
     double phiA[MAXNS * MAXNS], phiB[MAXNS * MAXNS];
+    float phiAf[MAXNS * MAXNS], phiBcf[MAXNS * MAXNS]; // mixed form: phiA, phiB * 0.5 / sqrt(2)
 };
 
 // Vde component order (global_setup.h:122-133)
@@ -55,12 +59,87 @@ __device__ __forceinline__ double fit4(const double *a, double lT)
     return v;
 }
 
+// e^x to FP32 accuracy (about 1e-7 relative) for the mixed form: the argument
+// goes to base 2 in FP64, its integer part is applied exactly by the exponent
+// and the fraction in [-1/2, 1/2] through the hardware base-2 exponential
+__device__ __forceinline__ float exp_f32(double x)
+{
+    const double t = x * 0x1.71547652b82fep+0;
+    const double n = rint(t);
+    return ldexpf(__builtin_amdgcn_exp2f((float)(t - n)), (int)n);
+}
+
+// The mixed form's transport coefficients (the same formulas as the FP64
+// path below): the fits' cubics and the sums that carry the reference's
+// 1e-40 and 1e-20 guards stay FP64; the exponentials, the Wilke factors and
+// their reciprocals and square roots are FP32.  About a quarter of the FP64
+// path's instructions.
+template <int NS>
+__device__ __forceinline__ void transport_mixed(const DevThermo *__restrict__ th, const DevTransport *__restrict__ tr,
+                                                Fields &f, long long id, long long N, const double *X, double C_total,
+                                                double rho, double rho1, double p, double lT, int heat, int diffu)
+{
+    float mu[NS], rmu[NS], Xf[NS];
+#pragma unroll
+    for (int k = 0; k < NS; k++) {
+        mu[k] = exp_f32(fit4(tr->fcv + 4 * k, lT));
+        rmu[k] = __builtin_amdgcn_rcpf(mu[k]);
+        Xf[k] = (float)X[k];
+    }
+    float va = 0.0f, tca = 0.0f;
+#pragma unroll
+    for (int k = 0; k < NS; k++) {
+        float den = 0.0f;
+#pragma unroll
+        for (int i = 0; i < NS; i++) {
+            float phi = i == k ? tr->phiAf[k * MAXNS + i]
+                               : tr->phiAf[k * MAXNS + i] * __builtin_amdgcn_sqrtf(mu[k] * rmu[i]);
+            phi = (phi + 1.0f) * (phi + 1.0f) * tr->phiBcf[k * MAXNS + i];
+            den += Xf[i] * phi;
+        }
+        const float _den = __builtin_amdgcn_rcpf(den);
+        va += Xf[k] * mu[k] * _den;
+        if (heat) tca += Xf[k] * exp_f32(fit4(tr->fct + 4 * k, lT)) * _den;
+    }
+    f.mu[id] = (double)va;
+    if (heat) f.kap[id] = (double)tca;
+    if (diffu) {
+        double t2[NS];
+        const float rp = __builtin_amdgcn_rcpf((float)p);
+#pragma unroll
+        for (int k = 0; k < NS; k++) t2[k] = 1.0e-20;
+#pragma unroll
+        for (int k = 0; k < NS; k++) {
+#pragma unroll
+            for (int i = 0; i < k; i++) {
+                const double rDe = (double)__builtin_amdgcn_rcpf(exp_f32(fit4(tr->Dkj + 4 * (i * MAXNS + k), lT)) * rp);
+                t2[k] += (X[i] + 1.0e-40) * rDe;
+                t2[i] += (X[k] + 1.0e-40) * rDe;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NS; k++) {
+            double t1 = 0.0;
+#pragma unroll
+            for (int i = 0; i < NS; i++)
+                if (i != k) t1 += (X[i] + 1.0e-40) * th->Wi[i];
+            double D;
+            if (t1 <= 0.0)
+                D = (double)(exp_f32(fit4(tr->Dkj + 4 * (k * MAXNS + k), lT)) * rp);
+            else
+                D = t1 / t2[k] * rho1 * C_total;
+            D *= 1.0e-1;
+            f.Dkm[k * N + id] = smax(D, 1.0e-10);
+        }
+    }
+}
+
 // Gettransport_coeff_aver + Get_transport_coeff_aver (Visc_kernels.hpp:205-235,
 // Visc_device.h:99-177) with get_xi (Mixing_device.h:31-44) on every padded
 // cell; also GetStates' u, v, w (Update_device.hpp:36) and, with diffusion,
 // each block's min / max of the interior mass fractions
 // (ConVenction_block.hpp:462-490) into ypart[block][2 NS]
-template <int NS>
+template <int NS, int PREC>
 __global__ void __launch_bounds__(256) k_transport(Geom g, const DevThermo *__restrict__ th,
                                                    const DevTransport *__restrict__ tr, const double *__restrict__ Ug,
                                                    Fields f, int heat, int diffu)
@@ -129,20 +208,36 @@ __global__ void __launch_bounds__(256) k_transport(Geom g, const DevThermo *__re
     if (!live) return;
     const double T = f.T[id], p = f.p[id];
     const double lT = dlog(T);
+    if constexpr (PREC == 1) {
+        transport_mixed<NS>(th, tr, f, id, N, X, C_total, rho, rho1, p, lT, heat, diffu);
+        return;
+    }
     // Wilke mixture viscosity and conductivity (the conductivity uses the
     // viscosity's denominators, as Visc_device.h:121-134 does)
     double mu[NS], rmu[NS];
 #pragma unroll
     for (int k = 0; k < NS; k++) mu[k] = dexp(fit4(tr->fcv + 4 * k, lT)), rmu[k] = 1.0 / mu[k];
     double va = 0.0, tca = 0.0;
-    const double s2 = sqrt(2.0);
+    // the quotients by sqrt(2) take the corrected reciprocal of that constant
+    // (folded at compile time), which is the IEEE quotient bit for bit
+    const double s2 = sqrt(2.0), rs2 = 1.0 / s2;
 #pragma unroll
     for (int k = 0; k < NS; k++) {
         double den = 0.0;
 #pragma unroll
         for (int i = 0; i < NS; i++) {
-            double phi = tr->phiA[k * MAXNS + i] * sqrt(qdiv(mu[k], mu[i], rmu[i]));
-            phi = (phi + 1.0) * (phi + 1.0) * 0.5 / s2;
+            double phi;
+            if (i == k) {
+                // RN(mu_k / mu_k) = 1 wherever qdiv's correction holds (it
+                // returns exactly 1 for a = b), and sqrt(1) = 1: the product
+                // is phiA itself; elsewhere the IEEE quotient, as qdiv takes
+                const double am = fabs(mu[k]);
+                phi = tr->phiA[k * MAXNS + i];
+                if (__builtin_expect(!(am >= 0x1p-500 && am <= 0x1p500), 0)) phi = phi * sqrt(ieee_div(mu[k], mu[k]));
+            } else {
+                phi = tr->phiA[k * MAXNS + i] * sqrt(qdiv(mu[k], mu[i], rmu[i]));
+            }
+            phi = qdiv((phi + 1.0) * (phi + 1.0) * 0.5, s2, rs2);
             phi = phi * tr->phiB[k * MAXNS + i];
             den = den + X[i] * phi;
         }
@@ -316,17 +411,31 @@ __global__ void __launch_bounds__(256) k_visc_flux(Geom g, const double *__restr
     constexpr int NCOP = NS - 1;
     if (step_halted(f.err, ~0)) return;
     const int nx = g.X_inner + (DIR == 0), ny = g.Y_inner + (DIR == 1), nz = g.Z_inner + (DIR == 2);
-    // y faces: XCD-aware block order, so the stencil rows that neighbouring
-    // blocks share meet in one L2 (2.95 -> 2.13 ms at 256^3).  x needs no
-    // reorder, and z, whose stencil spans planes far beyond one L2, measured
-    // slower with it (2.95 -> 4.5 ms), so both keep the dispatch order.
+    // Block order.  x needs no reorder.  y faces: XCD-aware order, so the
+    // stencil rows that neighbouring blocks share meet in one L2 (2.95 ->
+    // 2.13 ms at 256^3).  z faces: the grid is (chunks of 256 faces of a
+    // plane) x (face planes), and each XCD takes a contiguous run of it with
+    // the plane index fastest.  The four blocks whose stencils read the same
+    // row of every input then run together on one XCD and share its L2; in
+    // plane-major order they ran a plane apart, and every row was fetched
+    // about four times.
     int bx = (int)blockIdx.x, by_, bz_;
-    if (DIR == 1) xcd_tile(bx, by_, bz_);
-    const long long t = (long long)bx * blockDim.x + threadIdx.x;
-    if (t >= (long long)nx * ny * nz) return;
-    const int i = g.Bx - (DIR == 0) + (int)(t % nx);
-    const int j = g.By - (DIR == 1) + (int)((t / nx) % ny);
-    const int k = g.Bz - (DIR == 2) + (int)(t / ((long long)nx * ny));
+    if (DIR != 0) xcd_tile(bx, by_, bz_);
+    int i, j, k;
+    if (DIR == 2) {
+        const long long P = (long long)nx * ny;
+        const long long tp = (long long)(bx / nz) * blockDim.x + threadIdx.x;
+        if (tp >= P) return;
+        i = g.Bx + (int)(tp % nx);
+        j = g.By + (int)(tp / nx);
+        k = g.Bz - 1 + bx % nz;
+    } else {
+        const long long t = (long long)bx * blockDim.x + threadIdx.x;
+        if (t >= (long long)nx * ny * nz) return;
+        i = g.Bx - (DIR == 0) + (int)(t % nx);
+        j = g.By - (DIR == 1) + (int)((t / nx) % ny);
+        k = g.Bz + (int)(t / ((long long)nx * ny));
+    }
     const long long N = g.N;
     const long long sy = g.Xmax, sz = (long long)g.Xmax * g.Ymax;
     const long long st = DIR == 0 ? 1 : (DIR == 1 ? sy : sz);

@@ -301,6 +301,13 @@ int xf_time_kernel(xf_ctx *ctx, const char *name, int reps, double *avg_ms);
    settings file (run.blockSize_x/y/z, src/read_ini/settings/read_json.cpp:112-114). */
 int xf_set_tile_shape(xf_ctx *ctx, int dir, int shape);
 int xf_get_tile_shape(const xf_ctx *ctx, int *shape3);
+/* Viscous wall fluxes (GetWallViscousFlux{X,Y,Z},
+   src/solver_Reconstruction/FDM_Method/ConVenction_block.hpp:506-575):
+   on = 1 (the default) computes each face's wall flux inside the sweep of
+   its direction; on = 0 runs a wall-flux launch before each sweep, which
+   writes the E flux planes the sweep then reads (the reference's
+   structure).  Both give the same bits.  XF_EINVAL without viscous terms. */
+int xf_set_visc_fusion(xf_ctx *ctx, int on);
 /* Autotuner: times every tile shape of every active sweep direction on the
    context's current state (reps back-to-back launches each, stage-1
    configuration) and keeps the fastest.  shape3[3] receives the choice,

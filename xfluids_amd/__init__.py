@@ -158,6 +158,7 @@ _SIGS = {
     "xf_copy_probe": (C.c_int, [C.c_size_t, C.c_int, _DP]),
     "xf_fma_probe": (C.c_int, [C.c_int, _DP]),
     "xf_set_tile_shape": (C.c_int, [_V, C.c_int, C.c_int]),
+    "xf_set_visc_fusion": (C.c_int, [_V, C.c_int]),
     "xf_get_tile_shape": (C.c_int, [_V, _IP]),
     "xf_autotune": (C.c_int, [_V, C.c_int, _IP, _DP]),
     "xf_ensemble_create": (C.c_int, [C.POINTER(_V), C.c_int, C.POINTER(_V)]),
@@ -466,6 +467,11 @@ class Context:
 
     def set_tile_shape(self, direction, shape):
         self._chk(lib().xf_set_tile_shape(self._h, direction, shape), "xf_set_tile_shape")
+
+    def set_visc_fusion(self, on=True):
+        """Viscous wall fluxes inside the sweeps (True) or as launches of
+        their own before each sweep (False); the same bits either way."""
+        self._chk(lib().xf_set_visc_fusion(self._h, 1 if on else 0), "xf_set_visc_fusion")
 
     def tile_shape(self):
         out = np.zeros(3, np.int32)

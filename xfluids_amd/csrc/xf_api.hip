@@ -180,13 +180,14 @@ void lu_async(xf_ctx *c, const double *Ug, int rk_stage, const double *Ua, const
     if (c->g.DimY) dirs[nd++] = 1;
     if (c->g.DimZ) dirs[nd++] = 2;
     // viscous terms (ConVenction_block.hpp:424-575): transport coefficients
-    // and centre gradients once, then each direction's wall flux right before
-    // its sweep, which subtracts it from the face flux
+    // and centre gradients once; each sweep subtracts its direction's wall
+    // flux from the face flux, computing it itself, or (visc_fused 0) from
+    // the planes a wall-flux launch right before it writes
     if (c->d.visc) c->ops.visc_prep(c, Ug);
     for (int q = 0; q < nd; q++) {
         int mode = (q > 0) ? 1 : 0;
         if (rk_stage > 0 && q == nd - 1) mode |= 2;
-        if (c->d.visc) c->ops.visc_flux(c, dirs[q], Ug);
+        if (c->d.visc && !c->g.visc_fused) c->ops.visc_flux(c, dirs[q], Ug);
         c->ops.sweep(c, dirs[q], Ug, mode, rk_stage, Ua, Ub, Uo);
     }
 }
@@ -298,6 +299,11 @@ int xf_create(const xf_case_desc *desc, xf_ctx **out)
     g.positivity = d.positivity ? 1 : 0;
     g.visc = d.visc ? 1 : 0;
     g.ghost = d.ghost ? 1 : 0;
+    g.visc_heat = d.visc_heat ? 1 : 0;
+    g.visc_diffu = d.visc_diffu ? 1 : 0;
+    g.visc_fused = 1;
+    g.Yil_limiter = d.Yil_limiter;
+    g.Dim_limiter = d.Dim_limiter;
     g.N = c->N;
     auto fail = [&](int code) {
         xf_destroy(c);
@@ -940,6 +946,18 @@ int xf_set_tile_shape(xf_ctx *c, int dir, int shape)
     if (c->shape[dir] != shape) {
         c->shape[dir] = shape;
         // the captured step graph holds the old kernels
+        if (c->gexec) (void)hipGraphExecDestroy(c->gexec), c->gexec = nullptr;
+        if (c->graph) (void)hipGraphDestroy(c->graph), c->graph = nullptr;
+    }
+    return XF_OK;
+}
+
+int xf_set_visc_fusion(xf_ctx *c, int on)
+{
+    if (!c || !c->d.visc || (on != 0 && on != 1)) return XF_EINVAL;
+    if (c->g.visc_fused != on) {
+        c->g.visc_fused = on;
+        // the captured step graph holds the old launches and arguments
         if (c->gexec) (void)hipGraphExecDestroy(c->gexec), c->gexec = nullptr;
         if (c->graph) (void)hipGraphDestroy(c->graph), c->graph = nullptr;
     }

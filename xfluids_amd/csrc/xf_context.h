@@ -359,8 +359,7 @@ template <int NS, bool COP> struct Launch {
         const unsigned blocks = DIR == 2 ? nblk(plane) * (unsigned)nz : nblk(plane * nz);
         static const char *names[3] = {"visc_x", "visc_y", "visc_z"};
         ProfScope ps(c, names[DIR]);
-        hipLaunchKernelGGL((k_visc_flux<NS, DIR>), dim3(blocks), dim3(256), 0, c->stream, g, Ug, c->f,
-                           c->d.visc_heat, c->d.visc_diffu, c->d.Yil_limiter, c->d.Dim_limiter);
+        hipLaunchKernelGGL((k_visc_flux<NS, DIR>), dim3(blocks), dim3(256), 0, c->stream, g, Ug, c->f);
     }
     static void visc_flux(xf_ctx *c, int dir, const double *Ug)
     {

@@ -58,8 +58,13 @@ struct Geom {
     long long N; // padded cells
     double dx, dy, dz; // cell sizes (WENO-CU6 epsilon = 1e-8 dl^2)
     int positivity;    // positivity-preserving flux limiter on
-    int visc;          // viscous wall fluxes (Fv) subtracted from the face fluxes
+    int visc;          // viscous wall fluxes subtracted from the face fluxes
     int ghost;         // GhostSpecies mixture: mass fractions from the y planes
+    // viscous terms: heat conduction and species diffusion switches, the
+    // diffusion limiters, and whether the sweeps compute each face's wall
+    // flux themselves (1) or read it from the Fv planes (0)
+    int visc_heat, visc_diffu, visc_fused;
+    double Yil_limiter, Dim_limiter;
 };
 
 // Device pointers of one case

@@ -869,10 +869,25 @@ __device__ __forceinline__ void sweep_tile(const Geom &g, const DevThermo *__res
             cell_flux(li_r, id_r, UR, FR);
             positivity_face<E, NCOP>(flux, UL, UR, FL, FR, lambda_0, lambda);
         }
-        // viscous wall flux of this face (after the limiter, as GetLU orders them)
+        // viscous wall flux of this face (after the limiter, as GetLU orders
+        // them): the same function k_visc_flux runs, computed here while the
+        // other waves of the tile still run their characteristic fluxes, or
+        // (visc_fused 0) read from the planes k_visc_flux wrote
         if (g.visc) {
+            double Fv[E];
+            if constexpr (COP) {
+                if (g.visc_fused) {
+                    visc_face<NS, DIR>(g, Ug, f, id_l, Fv);
+                } else {
 #pragma unroll
-            for (int n = 0; n < E; n++) flux[n] = flux[n] - f.Fv[n * N + id_l];
+                    for (int n = 0; n < E; n++) Fv[n] = f.Fv[n * N + id_l];
+                }
+            } else {
+#pragma unroll
+                for (int n = 0; n < E; n++) Fv[n] = f.Fv[n * N + id_l];
+            }
+#pragma unroll
+            for (int n = 0; n < E; n++) flux[n] = flux[n] - Fv[n];
         }
     }
 

@@ -400,46 +400,20 @@ static __global__ void __launch_bounds__(256) k_vel_deriv(Geom g, Fields f, int 
 }
 
 // GetWallViscousFlux{X,Y,Z} with MARCO_PREVISCFLUX / MARCO_VISCFLUX
-// (Fourth_Order/Visc_Order_kernels.hpp:73-342, Fourth_Order/Flux_discrete.h:4-93)
-// on the faces right of cells i in [B - 1, B + inner - 1] along DIR (interior
-// on the other axes).  Writes the flux the sweep subtracts: Fv[n * N + id].
+// (Fourth_Order/Visc_Order_kernels.hpp:73-342, Fourth_Order/Flux_discrete.h:4-93):
+// the viscous flux F[E] of the face right of cell id along DIR, from the
+// transport coefficients, centre gradients and limits of this GetLU
 template <int NS, int DIR>
-__global__ void __launch_bounds__(256) k_visc_flux(Geom g, const double *__restrict__ Ug, Fields f, int heat,
-                                                   int diffu, double Yil_limiter, double Dim_limiter)
+__device__ __forceinline__ void visc_face(const Geom &g, const double *__restrict__ Ug, const Fields &f, long long id,
+                                          double *F)
 {
     constexpr int E = NS + 4;
-    constexpr int NCOP = NS - 1;
-    if (step_halted(f.err, ~0)) return;
-    const int nx = g.X_inner + (DIR == 0), ny = g.Y_inner + (DIR == 1), nz = g.Z_inner + (DIR == 2);
-    // Block order.  x needs no reorder.  y faces: XCD-aware order, so the
-    // stencil rows that neighbouring blocks share meet in one L2 (2.95 ->
-    // 2.13 ms at 256^3).  z faces: the grid is (chunks of 256 faces of a
-    // plane) x (face planes), and each XCD takes a contiguous run of it with
-    // the plane index fastest.  The four blocks whose stencils read the same
-    // row of every input then run together on one XCD and share its L2; in
-    // plane-major order they ran a plane apart, and every row was fetched
-    // about four times.
-    int bx = (int)blockIdx.x, by_, bz_;
-    if (DIR != 0) xcd_tile(bx, by_, bz_);
-    int i, j, k;
-    if (DIR == 2) {
-        const long long P = (long long)nx * ny;
-        const long long tp = (long long)(bx / nz) * blockDim.x + threadIdx.x;
-        if (tp >= P) return;
-        i = g.Bx + (int)(tp % nx);
-        j = g.By + (int)(tp / nx);
-        k = g.Bz - 1 + bx % nz;
-    } else {
-        const long long t = (long long)bx * blockDim.x + threadIdx.x;
-        if (t >= (long long)nx * ny * nz) return;
-        i = g.Bx - (DIR == 0) + (int)(t % nx);
-        j = g.By - (DIR == 1) + (int)((t / nx) % ny);
-        k = g.Bz + (int)(t / ((long long)nx * ny));
-    }
+    const int heat = g.visc_heat, diffu = g.visc_diffu;
+    const double Yil_limiter = g.Yil_limiter, Dim_limiter = g.Dim_limiter;
     const long long N = g.N;
     const long long sy = g.Xmax, sz = (long long)g.Xmax * g.Ymax;
     const long long st = DIR == 0 ? 1 : (DIR == 1 ? sy : sz);
-    const long long id = sz * k + sy * j + i, m1 = id - st, p1 = id + st, p2 = id + 2 * st;
+    const long long m1 = id - st, p1 = id + st, p2 = id + 2 * st;
     const double DX = g.DimX, DY = g.DimY, DZ = g.DimZ;
     const double _dl = DIR == 0 ? g._dx : (DIR == 1 ? g._dy : g._dz);
     const double OT = 1.0 / 3.0, SX = 1.0 / 16.0, TF = 1.0 / 24.0;
@@ -486,7 +460,6 @@ __global__ void __launch_bounds__(256) k_visc_flux(Geom g, const double *__restr
         v_hlf = (9.0 * (v[p1] + v[id]) - (v[p2] + v[m1])) * SX * DY;
         w_hlf = (9.0 * (w[p1] + w[id]) - (w[p2] + w[m1])) * SX;
     }
-    double F[E];
     F[0] = 0.0;
     F[1] = f_x;
     F[2] = f_y;
@@ -537,8 +510,48 @@ __global__ void __launch_bounds__(256) k_visc_flux(Geom g, const double *__restr
 #pragma unroll
         for (int q = 5; q < E; q++) F[q] = 0.0;
     }
+}
+
+// GetWallViscousFlux{X,Y,Z} as a kernel of its own, on the faces right of
+// cells i in [B - 1, B + inner - 1] along DIR (interior on the other axes):
+// writes the flux the sweep subtracts, Fv[n * N + id].  The sweeps compute
+// the same faces themselves unless Geom::visc_fused is 0.
+template <int NS, int DIR>
+__global__ void __launch_bounds__(256) k_visc_flux(Geom g, const double *__restrict__ Ug, Fields f)
+{
+    constexpr int E = NS + 4;
+    if (step_halted(f.err, ~0)) return;
+    const int nx = g.X_inner + (DIR == 0), ny = g.Y_inner + (DIR == 1), nz = g.Z_inner + (DIR == 2);
+    // Block order.  x needs no reorder.  y faces: XCD-aware order, so the
+    // stencil rows that neighbouring blocks share meet in one L2 (2.95 ->
+    // 2.13 ms at 256^3).  z faces: the grid is (chunks of 256 faces of a
+    // plane) x (face planes), and each XCD takes a contiguous run of it with
+    // the plane index fastest.  The four blocks whose stencils read the same
+    // row of every input then run together on one XCD and share its L2; in
+    // plane-major order they ran a plane apart, and every row was fetched
+    // about four times.
+    int bx = (int)blockIdx.x, by_, bz_;
+    if (DIR != 0) xcd_tile(bx, by_, bz_);
+    int i, j, k;
+    if (DIR == 2) {
+        const long long P = (long long)nx * ny;
+        const long long tp = (long long)(bx / nz) * blockDim.x + threadIdx.x;
+        if (tp >= P) return;
+        i = g.Bx + (int)(tp % nx);
+        j = g.By + (int)(tp / nx);
+        k = g.Bz - 1 + bx % nz;
+    } else {
+        const long long t = (long long)bx * blockDim.x + threadIdx.x;
+        if (t >= (long long)nx * ny * nz) return;
+        i = g.Bx - (DIR == 0) + (int)(t % nx);
+        j = g.By - (DIR == 1) + (int)((t / nx) % ny);
+        k = g.Bz + (int)(t / ((long long)nx * ny));
+    }
+    const long long id = (long long)g.Xmax * g.Ymax * k + (long long)g.Xmax * j + i;
+    double F[E];
+    visc_face<NS, DIR>(g, Ug, f, id, F);
 #pragma unroll
-    for (int q = 0; q < E; q++) f.Fv[q * N + id] = F[q];
+    for (int q = 0; q < E; q++) f.Fv[q * g.N + id] = F[q];
 }
 
 } // namespace xf

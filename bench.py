@@ -44,6 +44,12 @@ PEAK_FP64_TFLOPS = 78.6     # MI355X FP64 vector (and matrix) peak
 #   characteristic face flux, inviscid part (the sweep kernel's work;
 #   `--case mixing-layer --nz 8 --inviscid`), per face in x, y and z
 FACE_FLOPS_E13 = 10486
+#   viscous wall flux per face (GetWallViscousFlux, computed inside the
+#   sweeps): the face count of the viscous case less the inviscid one
+#   (`--case mixing-layer --nz 8` against `... --inviscid`: 11 053 - 10 486;
+#   the viscous solve leaves a residual of 7 operations in ~10^7 per grid,
+#   from the centre-derivative range, which no column of the solve fits)
+FACE_VISC_FLOPS_E13 = 567
 #   chemistry per interior cell as a + b * passes (`--case mixing-layer
 #   --chem`): a = start of CHEMEQ2 plus the two mixture energies, b = one
 #   rate evaluation plus one predictor and corrector; exp and log apart
@@ -54,10 +60,11 @@ CHEM_PASS_EXP, CHEM_PASS_LOG = 38, 542
 # profiles/pmc-instmix-mixing-layer-256cubed.md) over the counted ones: the
 # per-cell chemistry at 4.60 passes executes 24 279 per cell against
 # 9113 + 4.60 * 10 901 counted (it evaluates each species' Gibbs energy once
-# per pass, not once per reaction); the plain y sweep executes 4867 per face
-# against 10 486 (it skips the unused stencil ends and the zero terms)
+# per pass, not once per reaction); the plain z sweep with its wall flux
+# executes 5364 per face against 10 486 + 567 (it skips the unused stencil
+# ends and the zero terms)
 CHEM_EXECUTED_RATIO = 24279.0 / (9113.141 + 4.60 * 10900.994)
-SWEEP_E13_EXECUTED_RATIO = 4867.0 / 10486.0
+SWEEP_E13_EXECUTED_RATIO = 5364.0 / (10486.0 + 567.0)
 # Compulsory HBM bytes per cell-update of a full reactive step, E = 13,
 # Ns = 9: fluid 8 * (8E + 6) plus two Strang half-steps 2 * 8 * ((E+2) + (Ns+1))
 ALG_BYTES_E13 = 8 * (8 * 13 + 6) + 2 * 8 * ((13 + 2) + (9 + 1))
@@ -238,10 +245,14 @@ def headline_roofline(d, form):
              "sweep_z": d.X_inner * d.Y_inner * (d.Z_inner + 1)}.get(name)
     avg = ks[name]["ms_per_launch"]
     if faces:
-        achieved = faces * FACE_FLOPS_E13 / (avg * 1e-3) / 1e12
+        # the sweep computes the characteristic face flux and, inside the
+        # same launch, the face's viscous wall flux
+        per_face = FACE_FLOPS_E13 + FACE_VISC_FLOPS_E13
+        achieved = faces * per_face / (avg * 1e-3) / 1e12
         return {"kernel": name, "bound": "fp64-valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
                 "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": tr.get(name),
-                "avg_launch_ms": avg, "alg_flops_per_launch": faces * FACE_FLOPS_E13,
+                "avg_launch_ms": avg, "alg_flops_per_launch": faces * per_face,
+                "flops_per_face": [FACE_FLOPS_E13, FACE_VISC_FLOPS_E13],
                 "executed_over_counted": SWEEP_E13_EXECUTED_RATIO,
                 "frac_executed": achieved * SWEEP_E13_EXECUTED_RATIO / PEAK_FP64_TFLOPS}
     bytes_ = d.Xmax * d.Ymax * d.Zmax * 8 * (d.E + 10)

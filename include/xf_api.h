@@ -304,9 +304,12 @@ int xf_get_tile_shape(const xf_ctx *ctx, int *shape3);
 /* Viscous wall fluxes (GetWallViscousFlux{X,Y,Z},
    src/solver_Reconstruction/FDM_Method/ConVenction_block.hpp:506-575):
    on = 1 (the default) computes each face's wall flux inside the sweep of
-   its direction; on = 0 runs a wall-flux launch before each sweep, which
-   writes the E flux planes the sweep then reads (the reference's
-   structure).  Both give the same bits.  XF_EINVAL without viscous terms. */
+   its direction, for mixtures of four or more species (with fewer, the
+   sweep runs more waves per SIMD and keeps its registers for them, and the
+   wall flux stays a launch of its own); on = 0 runs a wall-flux launch
+   before each sweep, which writes the E flux planes the sweep then reads
+   (the reference's structure).  Both give the same bits.  XF_EINVAL
+   without viscous terms. */
 int xf_set_visc_fusion(xf_ctx *ctx, int on);
 /* Autotuner: times every tile shape of every active sweep direction on the
    context's current state (reps back-to-back launches each, stage-1

@@ -187,7 +187,7 @@ void lu_async(xf_ctx *c, const double *Ug, int rk_stage, const double *Ua, const
     for (int q = 0; q < nd; q++) {
         int mode = (q > 0) ? 1 : 0;
         if (rk_stage > 0 && q == nd - 1) mode |= 2;
-        if (c->d.visc && !c->g.visc_fused) c->ops.visc_flux(c, dirs[q], Ug);
+        if (c->d.visc && !(c->g.visc_fused && c->ops.visc_in_sweep)) c->ops.visc_flux(c, dirs[q], Ug);
         c->ops.sweep(c, dirs[q], Ug, mode, rk_stage, Ua, Ub, Uo);
     }
 }

@@ -49,6 +49,8 @@ struct Ops {
                          int fill, int where, int sel);
     void (*sweep_multi)(const xf_ctx *c0, hipStream_t, const EnsCase *, int ncases, long long blocks, int dir, int shape,
                         int mode, int stage, int sel_g, int sel_a, int sel_b, int sel_o);
+    // the sweeps compute the wall flux themselves when Geom::visc_fused is set
+    bool visc_in_sweep;
 };
 
 // dispatch tables, one translation unit each
@@ -429,8 +431,8 @@ template <int NS, bool COP> struct Launch {
     }
     static Ops ops()
     {
-        return Ops{&states, &sweep, &chem, &prepare, &chem_multi, &spill_bytes, &multi_blocks, &visc_prep,
-                   &visc_flux, &states_multi, &sweep_multi};
+        return Ops{&states,    &sweep,     &chem,         &prepare,     &chem_multi,         &spill_bytes,
+                   &multi_blocks, &visc_prep, &visc_flux, &states_multi, &sweep_multi, visc_in_sweep<NS>()};
     }
 };
 

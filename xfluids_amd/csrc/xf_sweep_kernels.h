@@ -491,6 +491,14 @@ template <int NS, int PREC> struct SweepOccupancy {
     static constexpr int waves = NS == 1 ? 4 : (NS <= 3 ? (PREC == XF_MIXED ? 4 : 3) : 2);
 };
 
+// Whether the sweeps compute the viscous wall flux of their faces (when
+// Geom::visc_fused asks for it): with NS >= 4, where every form of the sweep
+// runs two waves per SIMD and has the registers.  With fewer species the
+// sweep runs three or four waves per SIMD, the wall flux's loads would cost
+// it registers (and spills) in inviscid runs too, and the wall flux stays a
+// launch of its own.
+template <int NS> constexpr bool visc_in_sweep() { return NS >= 4; }
+
 // One tile of TS faces x TW pencils of one direction: (tx, ty, tz) is the
 // tile, in units of tiles along x, the pencil rows and the third axis.
 template <int NS, bool COP, int DIR, int SCHEME, int PREC, int SHAPE>
@@ -875,7 +883,7 @@ __device__ __forceinline__ void sweep_tile(const Geom &g, const DevThermo *__res
         // (visc_fused 0) read from the planes k_visc_flux wrote
         if (g.visc) {
             double Fv[E];
-            if constexpr (COP) {
+            if constexpr (COP && visc_in_sweep<NS>()) {
                 if (g.visc_fused) {
                     visc_face<NS, DIR>(g, Ug, f, id_l, Fv);
                 } else {

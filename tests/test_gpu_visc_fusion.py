@@ -56,3 +56,21 @@ def test_visc_fusion_switch_needs_viscous_terms():
     with pytest.raises(xf.XfError):
         ctx.set_visc_fusion(False)
     ctx.close()
+
+
+def test_switch_between_graph_replays():
+    """Switching mid-run rebuilds the captured step: a fused step, then an
+    unfused one, equals two unfused steps."""
+    announce("visc fusion switch between graph replays")
+    s, U, T, _ = make("mixing_layer_3d")
+    ctx = s.context()
+    ctx.upload(U, T)
+    ctx.initialize()
+    ctx.steps(1, graph=True)
+    ctx.set_visc_fusion(False)
+    ctx.steps(1, graph=True)
+    Ua, Ta = ctx.download(0)
+    ctx.close()
+    Ub, Tb = _run(s, U, T, 2, False)
+    assert np.array_equal(Ua.view(np.uint64), Ub.view(np.uint64))
+    assert np.array_equal(Ta.view(np.uint64), Tb.view(np.uint64))

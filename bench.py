@@ -86,8 +86,8 @@ NX, NY = 1024, 512
 HEADLINE_N = 256
 STARTUP = 10
 PROFILES = os.path.join(ROOT, "profiles")
-TRAFFIC_FILES = [os.path.join(PROFILES, f) for f in ("pmc-traffic-mixing-layer-256cubed.md",
-                                                     "pmc-traffic-shock-bubble-1024x512.md")]
+TRAFFIC_MIXING_LAYER = os.path.join(PROFILES, "pmc-traffic-mixing-layer-256cubed.md")
+TRAFFIC_SHOCK_BUBBLE = os.path.join(PROFILES, "pmc-traffic-shock-bubble-1024x512.md")
 
 
 def progress(msg):
@@ -95,17 +95,17 @@ def progress(msg):
     print("[bench] %s" % msg, file=sys.stderr, flush=True)
 
 
-def measured_traffic():
-    """HBM bytes per launch by kernel key from the committed rocprofv3 counter
-    summaries (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor)."""
+def measured_traffic(fn):
+    """HBM bytes per launch by kernel key from one workload's committed
+    rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE, scaled by the
+    copy-kernel factor)."""
     out = {}
-    for fn in TRAFFIC_FILES:
-        if os.path.exists(fn):
-            for ln in open(fn):
-                ln = ln.strip()
-                if ln.startswith("traffic[") and "=" in ln:
-                    k, v = ln.split("=", 1)
-                    out[k.strip()[8:-1]] = float(v.split()[0])
+    if os.path.exists(fn):
+        for ln in open(fn):
+            ln = ln.strip()
+            if ln.startswith("traffic[") and "=" in ln:
+                k, v = ln.split("=", 1)
+                out[k.strip()[8:-1]] = float(v.split()[0])
     return out
 
 
@@ -227,7 +227,7 @@ def headline_roofline(d, form):
     """Roofline of the kernel with the largest share of the plain step."""
     ks = form["kernels"]
     name = max(ks, key=lambda k: ks[k]["ms_per_step"])
-    tr = measured_traffic()
+    tr = measured_traffic(TRAFFIC_MIXING_LAYER)
     if name == "chem":
         cs = form["chem_sample"]
         achieved = cs["achieved_tflops"]
@@ -322,7 +322,8 @@ def shock_bubble(steps, warmup, kernel_reps, accuracy=True):
     ach = faces * FACE_FLOPS / (kt[name] * 1e-3) / 1e12
     out["roofline"] = {"kernel": name, "bound": "fp64-valu", "achieved": ach, "peak": PEAK_FP64_TFLOPS,
                        "frac": ach / PEAK_FP64_TFLOPS, "avg_launch_ms": kt[name], "timed_launches": kernel_reps,
-                       "alg_flops_per_launch": faces * FACE_FLOPS, "traffic": measured_traffic().get(name)}
+                       "alg_flops_per_launch": faces * FACE_FLOPS,
+                       "traffic": measured_traffic(TRAFFIC_SHOCK_BUBBLE).get(name)}
     out["step_roofline"] = step_roofline(d, out["forms"]["plain"]["value"])
     return out
 
@@ -573,6 +574,7 @@ def main():
     ap.add_argument("--ensemble-sides", default="128,256,512,1024")
     ap.add_argument("--c2-steps", type=int, default=500)
     ap.add_argument("--c3-only", action="store_true", help="only the shock tube window (its own JSON line)")
+    ap.add_argument("--c2-only", action="store_true", help="only the shock-bubble leg (its own JSON line)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
@@ -582,6 +584,10 @@ def main():
         return
     if args.c3_only:
         print(json.dumps({"shock_tube": shock_tube(8192, args.c3_window_us * 1e-6, args.c3_steps, 10)}))
+        return
+    if args.c2_only:
+        print(json.dumps({"shock_bubble": shock_bubble(args.c2_steps, 10, args.kernel_reps,
+                                                       accuracy=not args.no_accuracy)}))
         return
     s, ml = mixing_layer(args.n, args.startup, args.warmup, args.steps)
     d = s.desc

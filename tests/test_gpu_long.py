@@ -20,7 +20,12 @@ from parity import announce, compare, field_errors, make, oracle_for
 
 pytestmark = pytest.mark.gpu
 
-LONG = {"vortex": 200, "shock_bubble_z_janaf": 500, "shock_bubble_js_nasa": 500}
+# the inviscid cases at their workloads' step counts; two viscous cases whose
+# sweeps compute the wall flux themselves (four or more species), longer
+# than their parity runs: Inert-SBI in 3-D (NS = 5, WENO-CU6, limiter) and
+# the reacting 9-species mixing layer
+LONG = {"vortex": 200, "shock_bubble_z_janaf": 500, "shock_bubble_js_nasa": 500, "inert_sbi_3d": 60,
+        "mixing_layer_3d": 20}
 TOL_L1 = 1e-6
 TOL_Y = 1e-6
 

@@ -514,8 +514,9 @@ __device__ __forceinline__ void visc_face(const Geom &g, const double *__restric
 
 // GetWallViscousFlux{X,Y,Z} as a kernel of its own, on the faces right of
 // cells i in [B - 1, B + inner - 1] along DIR (interior on the other axes):
-// writes the flux the sweep subtracts, Fv[n * N + id].  The sweeps compute
-// the same faces themselves unless Geom::visc_fused is 0.
+// writes the flux the sweep subtracts, Fv[n * N + id].  With four or more
+// species the sweeps compute the same faces themselves unless
+// Geom::visc_fused is 0 (visc_in_sweep).
 template <int NS, int DIR>
 __global__ void __launch_bounds__(256) k_visc_flux(Geom g, const double *__restrict__ Ug, Fields f)
 {

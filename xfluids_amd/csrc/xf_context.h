@@ -348,8 +348,16 @@ template <int NS, bool COP> struct Launch {
             const int j0 = g.DimY ? g.By - 2 : 0, ny = g.DimY ? g.Y_inner + 4 : 1;
             const int k0 = g.DimZ ? g.Bz - 2 : 0, nz = g.DimZ ? g.Z_inner + 4 : 1;
             const int *bc = c->d.BC;
-            hipLaunchKernelGGL(k_vel_deriv, dim3(nblk((long long)nx * ny * nz)), dim3(256), 0, c->stream, g, c->f, i0,
-                               nx, j0, ny, k0, nz, bc[0], bc[1], bc[2], bc[3], bc[4], bc[5]);
+            // the mixed form stores the gradients (and the transport
+            // coefficients) as FP32 planes
+            if (c->d.precision == XF_MIXED)
+                hipLaunchKernelGGL((k_vel_deriv<XF_MIXED>), dim3(nblk((long long)nx * ny * nz)), dim3(256), 0,
+                                   c->stream, g, c->f, i0, nx, j0, ny, k0, nz, bc[0], bc[1], bc[2], bc[3], bc[4],
+                                   bc[5]);
+            else
+                hipLaunchKernelGGL((k_vel_deriv<XF_PLAIN_FP64>), dim3(nblk((long long)nx * ny * nz)), dim3(256), 0,
+                                   c->stream, g, c->f, i0, nx, j0, ny, k0, nz, bc[0], bc[1], bc[2], bc[3], bc[4],
+                                   bc[5]);
         }
     }
     template <int DIR> static void visc_flux_d(xf_ctx *c, const double *Ug)
@@ -361,7 +369,11 @@ template <int NS, bool COP> struct Launch {
         const unsigned blocks = DIR == 2 ? nblk(plane) * (unsigned)nz : nblk(plane * nz);
         static const char *names[3] = {"visc_x", "visc_y", "visc_z"};
         ProfScope ps(c, names[DIR]);
-        hipLaunchKernelGGL((k_visc_flux<NS, DIR>), dim3(blocks), dim3(256), 0, c->stream, g, Ug, c->f);
+        if (c->d.precision == XF_MIXED)
+            hipLaunchKernelGGL((k_visc_flux<NS, DIR, XF_MIXED>), dim3(blocks), dim3(256), 0, c->stream, g, Ug, c->f);
+        else
+            hipLaunchKernelGGL((k_visc_flux<NS, DIR, XF_PLAIN_FP64>), dim3(blocks), dim3(256), 0, c->stream, g, Ug,
+                               c->f);
     }
     static void visc_flux(xf_ctx *c, int dir, const double *Ug)
     {

@@ -885,7 +885,7 @@ __device__ __forceinline__ void sweep_tile(const Geom &g, const DevThermo *__res
             double Fv[E];
             if constexpr (COP && visc_in_sweep<NS>()) {
                 if (g.visc_fused) {
-                    visc_face<NS, DIR>(g, Ug, f, id_l, Fv);
+                    visc_face<NS, DIR, MIX ? XF_MIXED : XF_PLAIN_FP64>(g, Ug, f, id_l, Fv);
                 } else {
 #pragma unroll
                     for (int n = 0; n < E; n++) Fv[n] = f.Fv[n * N + id_l];

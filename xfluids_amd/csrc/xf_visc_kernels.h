@@ -8,16 +8,20 @@
 //                     (PREC 1: the mixed form's FP32 exponentials and factors);
 //                     per-block min/max of each interior mass fraction
 //   k_yreduce<NS>     one block: those partials into the reduction slots
-//   k_vel_deriv       fourth-order centre derivatives of u, v, w, with the
+//   k_vel_deriv<PREC> fourth-order centre derivatives of u, v, w, with the
 //                     ghost-cell rules for the values each wall flux reads
-//   k_visc_flux<NS,DIR> per face of one direction; the sweep of that
-//                     direction subtracts the result from its face flux
+//   visc_face<NS,DIR,PREC> per face of one direction, inside the sweep of
+//                     that direction (NS >= 4), or k_visc_flux<NS,DIR,PREC>
+//                     writing the flux planes the sweep subtracts
 // All arithmetic of the plain form is FP64 in the reference's operation
 // order, with the same exp/log as the oracle, so it stays bit-identical; the
 // mixed form's transport coefficients take FP32 where the FP64 path spends
-// its instructions (transport_mixed).
+// its instructions (transport_mixed), and the mixed form stores the transport
+// coefficients and gradients as FP32 planes that the wall flux reads back
+// into its FP64 arithmetic.
 #pragma once
 #include "xf_fluid_kernels.h"
+#include <type_traits>
 
 namespace xf {
 
@@ -57,6 +61,25 @@ __device__ __forceinline__ double fit4(const double *a, double lT)
 #pragma unroll
     for (int i = 2; i >= 0; i--) v = v * lT + a[i];
     return v;
+}
+
+// The viscous work planes (mu, kap, D_km, the centre gradients) hold FP64
+// values, or in the mixed form FP32 ones in the first half of each plane's
+// allocation.  A reader sees either as doubles, so the wall flux's
+// arithmetic is FP64 in both forms.
+template <class T> struct VPlane {
+    const T *p;
+    __device__ __forceinline__ double operator[](long long i) const { return (double)p[i]; }
+    __device__ __forceinline__ VPlane at(long long off) const { return VPlane{p + off}; }
+};
+template <int PREC> using vtype = typename std::conditional<PREC == XF_MIXED, float, double>::type;
+template <int PREC> __device__ __forceinline__ VPlane<vtype<PREC>> vplane(const double *p)
+{
+    return VPlane<vtype<PREC>>{reinterpret_cast<const vtype<PREC> *>(p)};
+}
+template <int PREC> __device__ __forceinline__ vtype<PREC> *vplane_w(double *p)
+{
+    return reinterpret_cast<vtype<PREC> *>(p);
 }
 
 // e^x to FP32 accuracy (about 1e-7 relative) for the mixed form: the argument
@@ -101,8 +124,8 @@ __device__ __forceinline__ void transport_mixed(const DevThermo *__restrict__ th
         va += Xf[k] * mu[k] * _den;
         if (heat) tca += Xf[k] * exp_f32(fit4(tr->fct + 4 * k, lT)) * _den;
     }
-    f.mu[id] = (double)va;
-    if (heat) f.kap[id] = (double)tca;
+    vplane_w<XF_MIXED>(f.mu)[id] = va;
+    if (heat) vplane_w<XF_MIXED>(f.kap)[id] = tca;
     if (diffu) {
         double t2[NS];
         const float rp = __builtin_amdgcn_rcpf((float)p);
@@ -129,7 +152,7 @@ __device__ __forceinline__ void transport_mixed(const DevThermo *__restrict__ th
             else
                 D = t1 / t2[k] * rho1 * C_total;
             D *= 1.0e-1;
-            f.Dkm[k * N + id] = smax(D, 1.0e-10);
+            vplane_w<XF_MIXED>(f.Dkm)[k * N + id] = (float)smax(D, 1.0e-10);
         }
     }
 }
@@ -339,7 +362,8 @@ __device__ __forceinline__ void centre_deriv(const Geom &g, const Fields &f, lon
 // operations.  Ghost cells in two directions at once (edges and corners) are
 // read by no wall flux.  The z no-slip wall uses the mirror offset - k of the
 // x and y kernels (the reference's k - offset indexes outside the array).
-static __global__ void __launch_bounds__(256) k_vel_deriv(Geom g, Fields f, int i0, int nx, int j0, int ny, int k0,
+template <int PREC>
+__global__ void __launch_bounds__(256) k_vel_deriv(Geom g, Fields f, int i0, int nx, int j0, int ny, int k0,
                                                           int nz, int bc0, int bc1, int bc2, int bc3, int bc4, int bc5)
 {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -396,14 +420,14 @@ static __global__ void __launch_bounds__(256) k_vel_deriv(Geom g, Fields f, int 
         for (int q = 0; q < 4; q++) D[comp[d][q]] = fac < 0.0 ? -T9[comp[d][q]] : T9[comp[d][q]];
     }
 #pragma unroll
-    for (int q = 0; q < 9; q++) f.Vde[q * N + id] = D[q];
+    for (int q = 0; q < 9; q++) vplane_w<PREC>(f.Vde)[q * N + id] = (vtype<PREC>)D[q];
 }
 
 // GetWallViscousFlux{X,Y,Z} with MARCO_PREVISCFLUX / MARCO_VISCFLUX
 // (Fourth_Order/Visc_Order_kernels.hpp:73-342, Fourth_Order/Flux_discrete.h:4-93):
 // the viscous flux F[E] of the face right of cell id along DIR, from the
 // transport coefficients, centre gradients and limits of this GetLU
-template <int NS, int DIR>
+template <int NS, int DIR, int PREC>
 __device__ __forceinline__ void visc_face(const Geom &g, const double *__restrict__ Ug, const Fields &f, long long id,
                                           double *F)
 {
@@ -417,17 +441,18 @@ __device__ __forceinline__ void visc_face(const Geom &g, const double *__restric
     const double DX = g.DimX, DY = g.DimY, DZ = g.DimZ;
     const double _dl = DIR == 0 ? g._dx : (DIR == 1 ? g._dy : g._dz);
     const double OT = 1.0 / 3.0, SX = 1.0 / 16.0, TF = 1.0 / 24.0;
-    const double *u = f.u, *v = f.v, *w = f.w, *va = f.mu, *Vd = f.Vde;
-    auto half = [&](const double *a) { return (9.0 * (a[p1] + a[id]) - (a[p2] + a[m1])) * SX; };
-    auto grad = [&](const double *a) { return (27.0 * (a[p1] - a[id]) - (a[p2] - a[m1])) * _dl * TF; };
-    auto pair2 = [&](const double *a, const double *b) {
+    const double *u = f.u, *v = f.v, *w = f.w;
+    const auto va = vplane<PREC>(f.mu), Vd = vplane<PREC>(f.Vde);
+    auto half = [&](const auto &a) { return (9.0 * (a[p1] + a[id]) - (a[p2] + a[m1])) * SX; };
+    auto grad = [&](const auto &a) { return (27.0 * (a[p1] - a[id]) - (a[p2] - a[m1])) * _dl * TF; };
+    auto pair2 = [&](const auto &a, const auto &b) {
         return 9.0 * (a[p1] + a[id]) - (a[p2] + a[m1]) + 9.0 * (b[p1] + b[id]) - (b[p2] + b[m1]);
     };
     const double mue = half(va);
     const double lamada = -2.0 * OT * mue;
     double f_x, f_y, f_z, u_hlf, v_hlf, w_hlf;
     if (DIR == 0) {
-        const double *Dvcy = Vd + DVCY * N, *Dwcz = Vd + DWCZ * N, *Ducy = Vd + DUCY * N, *Ducz = Vd + DUCZ * N;
+        const auto Dvcy = Vd.at(DVCY * N), Dwcz = Vd.at(DWCZ * N), Ducy = Vd.at(DUCY * N), Ducz = Vd.at(DUCZ * N);
         f_x = (2.0 * mue + lamada) * (27.0 * (u[p1] - u[id]) - (u[p2] - u[m1])) * _dl * TF;
         f_x += lamada * pair2(Dvcy, Dwcz) * SX;
         f_y = mue * (27.0 * (v[p1] - v[id]) - (v[p2] - v[m1])) * _dl * TF * DY;
@@ -438,7 +463,7 @@ __device__ __forceinline__ void visc_face(const Geom &g, const double *__restric
         v_hlf = (9.0 * (v[p1] + v[id]) - (v[p2] + v[m1])) * SX * DY;
         w_hlf = (9.0 * (w[p1] + w[id]) - (w[p2] + w[m1])) * SX * DZ;
     } else if (DIR == 1) {
-        const double *Dvcx = Vd + DVCX * N, *Dvcz = Vd + DVCZ * N, *Ducx = Vd + DUCX * N, *Dwcz = Vd + DWCZ * N;
+        const auto Dvcx = Vd.at(DVCX * N), Dvcz = Vd.at(DVCZ * N), Ducx = Vd.at(DUCX * N), Dwcz = Vd.at(DWCZ * N);
         f_x = mue * (27.0 * (u[p1] - u[id]) - (u[p2] - u[m1])) * _dl * TF * DX;
         f_x += mue * (9.0 * (Dvcx[p1] + Dvcx[id]) - (Dvcx[p2] + Dvcx[m1])) * SX * DX;
         f_y = (2.0 * mue + lamada) * (27.0 * (v[p1] - v[id]) - (v[p2] - v[m1])) * _dl * TF;
@@ -449,7 +474,7 @@ __device__ __forceinline__ void visc_face(const Geom &g, const double *__restric
         v_hlf = (9.0 * (v[p1] + v[id]) - (v[p2] + v[m1])) * SX;
         w_hlf = (9.0 * (w[p1] + w[id]) - (w[p2] + w[m1])) * SX * DZ;
     } else {
-        const double *Dwcx = Vd + DWCX * N, *Dwcy = Vd + DWCY * N, *Ducx = Vd + DUCX * N, *Dvcy = Vd + DVCY * N;
+        const auto Dwcx = Vd.at(DWCX * N), Dwcy = Vd.at(DWCY * N), Ducx = Vd.at(DUCX * N), Dvcy = Vd.at(DVCY * N);
         f_x = mue * (27.0 * (u[p1] - u[id]) - (u[p2] - u[m1])) * _dl * TF * DX;
         f_x += mue * (9.0 * (Dwcx[p1] + Dwcx[id]) - (Dwcx[p2] + Dwcx[m1])) * SX * DX;
         f_y = mue * (27.0 * (v[p1] - v[id]) - (v[p2] - v[m1])) * _dl * TF * DY;
@@ -466,7 +491,7 @@ __device__ __forceinline__ void visc_face(const Geom &g, const double *__restric
     F[3] = f_z;
     F[4] = f_x * u_hlf + f_y * v_hlf + f_z * w_hlf;
     if (heat) {
-        double kk = half(f.kap);
+        double kk = half(vplane<PREC>(f.kap));
         kk *= grad(f.T);
         F[4] += kk;
     }
@@ -493,7 +518,8 @@ __device__ __forceinline__ void visc_face(const Geom &g, const double *__restric
             ymax -= ymin;
             ymax *= Yil_limiter;
             dmax *= Dim_limiter * ymax;
-            const double *hl = f.hi + l * N, *Dl = f.Dkm + l * N;
+            const double *hl = f.hi + l * N;
+            const auto Dl = vplane<PREC>(f.Dkm).at(l * N);
             const double hi_wall = half(hl);
             const double Dim_wall = half(Dl);
             const double Yil_wall =
@@ -517,7 +543,7 @@ __device__ __forceinline__ void visc_face(const Geom &g, const double *__restric
 // writes the flux the sweep subtracts, Fv[n * N + id].  With four or more
 // species the sweeps compute the same faces themselves unless
 // Geom::visc_fused is 0 (visc_in_sweep).
-template <int NS, int DIR>
+template <int NS, int DIR, int PREC>
 __global__ void __launch_bounds__(256) k_visc_flux(Geom g, const double *__restrict__ Ug, Fields f)
 {
     constexpr int E = NS + 4;
@@ -550,7 +576,7 @@ __global__ void __launch_bounds__(256) k_visc_flux(Geom g, const double *__restr
     }
     const long long id = (long long)g.Xmax * g.Ymax * k + (long long)g.Xmax * j + i;
     double F[E];
-    visc_face<NS, DIR>(g, Ug, f, id, F);
+    visc_face<NS, DIR, PREC>(g, Ug, f, id, F);
 #pragma unroll
     for (int q = 0; q < E; q++) f.Fv[q * g.N + id] = F[q];
 }

@@ -11,7 +11,11 @@ from parity import announce, make
 
 pytestmark = pytest.mark.gpu
 
-VISC_CASES = ["mixing_layer_3d", "shock_bubble_visc", "shock_bubble_cu6_pp_visc", "inert_sbi_3d"]
+# in-sweep wall flux: the 9-, 10- and 11-species cases (3-D, 3-D, 2-D) and
+# Inert-SBI (NS = 5, 3-D); the 3-species cases keep the separate launch in
+# both settings
+VISC_CASES = ["mixing_layer_3d", "mixing_layer_21", "rsbi", "inert_sbi_3d", "shock_bubble_visc",
+              "shock_bubble_cu6_pp_visc"]
 
 
 def _run(s, U, T, nsteps, fused, precision=xf.XF_PLAIN_FP64, mode="graph"):

@@ -900,13 +900,33 @@ __device__ __forceinline__ void sweep_tile(const Geom &g, const DevThermo *__res
     }
 
     // ---- flux divergence on the TS-1 cells of the tile ----------------------
+    // the lane's cell, and what the divergence reads of it from memory (LU
+    // of the earlier directions; U and the stage input of the RK update),
+    // loaded before the barriers: the loads are in flight while the tile's
+    // other waves finish their faces.  Each cell is read and written only by
+    // its own lane, and no tile stages from the arrays written here.
+    const bool cell_ok = fidx >= 1 && okp && sl <= B + inner - 1;
+    double LUi[E], Ai[E], Bi[E];
+#pragma unroll
+    for (int n = 0; n < E; n++) LUi[n] = 0.0, Ai[n] = 0.0, Bi[n] = 0.0;
+    if (cell_ok) {
+        const long long id = pbase + (long long)sl * stride;
+        if (mode & 1) {
+#pragma unroll
+            for (int n = 0; n < E; n++) LUi[n] = f.LU[n * N + id];
+        }
+        if (mode & 2) {
+#pragma unroll
+            for (int n = 0; n < E; n++) Ai[n] = Ua[n * N + id], Bi[n] = Ub[n * N + id];
+        }
+    }
     __syncthreads();
     double *sFw = sU; // reuse the staging area (E * NC >= E * 256)
     const int fw = DIR == 0 ? tcol * TS + fidx : fidx * TW + tcol;
 #pragma unroll
     for (int n = 0; n < E; n++) sFw[n * 256 + fw] = flux[n];
     __syncthreads();
-    if (fidx >= 1 && okp && sl <= B + inner - 1) {
+    if (cell_ok) {
         const long long id = pbase + (long long)sl * stride;
         const int fwm = DIR == 0 ? tcol * TS + fidx - 1 : (fidx - 1) * TW + tcol;
         const double dl = DIR == 0 ? g._dx : (DIR == 1 ? g._dy : g._dz);
@@ -915,7 +935,7 @@ __device__ __forceinline__ void sweep_tile(const Geom &g, const DevThermo *__res
 #pragma unroll
         for (int n = 0; n < E; n++) {
             const double d = (sFw[n * 256 + fwm] - sFw[n * 256 + fw]) * dl;
-            const double LUt = (mode & 1) ? f.LU[n * N + id] + d : 0.0 + d;
+            const double LUt = (mode & 1) ? LUi[n] + d : 0.0 + d;
             if (mode & 2) {
                 // fused SSP-RK3 stage: Ua = U, Ub = stage input (stage 2, 3),
                 // Uo = output.  Before the update, EstimateNAN (flags 1-3,
@@ -923,7 +943,7 @@ __device__ __forceinline__ void sweep_tile(const Geom &g, const DevThermo *__res
                 // the stage's UI: U1 at stages 1 and 2 (at stage 1 still the
                 // previous step's second-stage result, which is what Ub holds
                 // then), U at stage 3
-                const double a0 = Ua[n * N + id], b0 = Ub[n * N + id];
+                const double a0 = Ai[n], b0 = Bi[n];
                 const double ui = (stage == 3) ? a0 : b0;
                 bad = bad || isnan(LUt) || isinf(LUt) || isnan(ui) || isinf(ui) || (n == 0 && ui < 0);
                 double a = a0, b = (stage == 1) ? 0.0 : b0;

@@ -604,6 +604,24 @@ __device__ __forceinline__ void sweep_tile(const Geom &g, const DevThermo *__res
             sR1[li] = 0.0;
         }
     }
+    // the lane's face, and the thermal values of its two cells that the
+    // multi-species sound speed reads from memory, loaded before the staging
+    // barrier so that they arrive while the tile's other waves stage
+    bool okp;
+    const long long pbase = pencil_base(tcol, okp);
+    const int sl = c0 + fidx - 1;
+    const bool face_ok = okp && (sl <= B + inner - 1);
+    constexpr int NT = COP ? NS : 1;
+    double hl[NT], hr[NT], Tlr[2], gcvlr[2], Rlr[2], Cplr[2];
+    if (COP && face_ok) {
+        const long long id_l = pbase + (long long)sl * stride, id_r = id_l + stride;
+#pragma unroll
+        for (int k = 0; k < NT; k++) hl[k] = f.hi[k * N + id_l], hr[k] = f.hi[k * N + id_r];
+        Tlr[0] = f.T[id_l], Tlr[1] = f.T[id_r];
+        gcvlr[0] = f.gcv[id_l], gcvlr[1] = f.gcv[id_r];
+        Rlr[0] = f.R[id_l], Rlr[1] = f.R[id_r];
+        Cplr[0] = f.Cp[id_l], Cplr[1] = f.Cp[id_r];
+    }
     __syncthreads();
 
     if (tx == 0 && ty == 0 && tz == 0 && tid < 3) {
@@ -612,10 +630,6 @@ __device__ __forceinline__ void sweep_tile(const Geom &g, const DevThermo *__res
     }
 
     // ---- one characteristic face flux per lane ------------------------------
-    bool okp;
-    const long long pbase = pencil_base(tcol, okp);
-    const int sl = c0 + fidx - 1;
-    const bool face_ok = okp && (sl <= B + inner - 1);
     double flux[E];
 #pragma unroll
     for (int n = 0; n < E; n++) flux[n] = 0.0;
@@ -642,14 +656,11 @@ __device__ __forceinline__ void sweep_tile(const Geom &g, const DevThermo *__res
         rf._P = (pl + D * pr) * D1;
         rf._rho = sqrt(rr * rl);
         if (COP) {
-            double yl[COP ? NS : 1], yr[COP ? NS : 1], hl[COP ? NS : 1], hr[COP ? NS : 1];
-            cell_y<COP ? NS : 1>(g, f, id_l, r1l, [&](int k) { return sU[(5 + k) * NC + li_l]; }, yl);
-            cell_y<COP ? NS : 1>(g, f, id_r, r1r, [&](int k) { return sU[(5 + k) * NC + li_r]; }, yr);
-#pragma unroll
-            for (int k = 0; k < (COP ? NS : 1); k++) hl[k] = f.hi[k * N + id_l], hr[k] = f.hi[k * N + id_r];
-            cop_c2<COP ? NS : 1>(th, rf, D, D1, yl, yr, hl, hr, f.T[id_l], f.T[id_r], f.gcv[id_l], f.gcv[id_r],
-                                 f.R[id_l], f.R[id_r], f.Cp[id_l], f.Cp[id_r], rl, rr, ul, vl, wl, ur, vr, wr, pl, pr,
-                                 Hl, Hr);
+            double yl[NT], yr[NT];
+            cell_y<NT>(g, f, id_l, r1l, [&](int k) { return sU[(5 + k) * NC + li_l]; }, yl);
+            cell_y<NT>(g, f, id_r, r1r, [&](int k) { return sU[(5 + k) * NC + li_r]; }, yr);
+            cop_c2<NT>(th, rf, D, D1, yl, yr, hl, hr, Tlr[0], Tlr[1], gcvlr[0], gcvlr[1], Rlr[0], Rlr[1], Cplr[0],
+                       Cplr[1], rl, rr, ul, vl, wl, ur, vr, wr, pl, pr, Hl, Hr);
         } else {
             rf._yi[0] = 1.0;
             rf.c2 = ncop_gamma * rf._P / rf._rho;

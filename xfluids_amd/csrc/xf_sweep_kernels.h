@@ -896,7 +896,11 @@ __device__ __forceinline__ void sweep_tile(const Geom &g, const DevThermo *__res
             double Fv[E];
             if constexpr (COP && visc_in_sweep<NS>()) {
                 if (g.visc_fused) {
-                    visc_face<NS, DIR, MIX ? XF_MIXED : XF_PLAIN_FP64>(g, Ug, f, id_l, Fv);
+                    // the stencil cells' U and 1 / rho from the staged tile
+                    // (the face's left cell sits at fidx + 2)
+                    visc_face<NS, DIR, MIX ? XF_MIXED : XF_PLAIN_FP64>(
+                        g, f, id_l, [&](int c, int n) { return sU[n * NC + lds_ix<DIR, TS, TW>(fidx + 1 + c, tcol)]; },
+                        [&](int c) { return sR1[lds_ix<DIR, TS, TW>(fidx + 1 + c, tcol)]; }, Fv);
                 } else {
 #pragma unroll
                     for (int n = 0; n < E; n++) Fv[n] = f.Fv[n * N + id_l];

@@ -427,9 +427,11 @@ __global__ void __launch_bounds__(256) k_vel_deriv(Geom g, Fields f, int i0, int
 // (Fourth_Order/Visc_Order_kernels.hpp:73-342, Fourth_Order/Flux_discrete.h:4-93):
 // the viscous flux F[E] of the face right of cell id along DIR, from the
 // transport coefficients, centre gradients and limits of this GetLU
-template <int NS, int DIR, int PREC>
-__device__ __forceinline__ void visc_face(const Geom &g, const double *__restrict__ Ug, const Fields &f, long long id,
-                                          double *F)
+// uat(c, n) is U_n and r1at(c) is 1 / U_0 (as GetStates forms it) of the
+// stencil cell c = 0..3 (i - 1, i, i + 1, i + 2 along DIR): from memory, or
+// from a sweep's staged tile.
+template <int NS, int DIR, int PREC, class UAt, class R1At>
+__device__ __forceinline__ void visc_face(const Geom &g, const Fields &f, long long id, UAt uat, R1At r1at, double *F)
 {
     constexpr int E = NS + 4;
     const int heat = g.visc_heat, diffu = g.visc_diffu;
@@ -502,10 +504,8 @@ __device__ __forceinline__ void visc_face(const Geom &g, const double *__restric
         double rw[4];
 #pragma unroll
         for (int c = 0; c < 4; c++) {
-            const double r = Ug[cid[c]];
-            rw[c] = r;
-            const double r1 = 1.0 / r;
-            cell_y<NS>(g, f, cid[c], r1, [&](int ii) { return Ug[(5 + ii) * N + cid[c]]; }, Y[c]);
+            rw[c] = uat(c, 0);
+            cell_y<NS>(g, f, cid[c], r1at(c), [&](int ii) { return uat(c, 5 + ii); }, Y[c]);
         }
         const double rho_wall = (9.0 * (rw[2] + rw[1]) - (rw[3] + rw[0])) * SX;
         double CorrectTerm = 0.0, Dim_Yil = 1.0e-20, Yi_wall[NS];
@@ -575,8 +575,11 @@ __global__ void __launch_bounds__(256) k_visc_flux(Geom g, const double *__restr
         k = g.Bz + (int)(t / ((long long)nx * ny));
     }
     const long long id = (long long)g.Xmax * g.Ymax * k + (long long)g.Xmax * j + i;
+    const long long st = DIR == 0 ? 1 : (DIR == 1 ? (long long)g.Xmax : (long long)g.Xmax * g.Ymax);
     double F[E];
-    visc_face<NS, DIR, PREC>(g, Ug, f, id, F);
+    visc_face<NS, DIR, PREC>(
+        g, f, id, [&](int c, int n) { return Ug[n * g.N + id + (c - 1) * st]; },
+        [&](int c) { return 1.0 / Ug[id + (c - 1) * st]; }, F);
 #pragma unroll
     for (int q = 0; q < E; q++) f.Fv[q * g.N + id] = F[q];
 }

@@ -429,7 +429,8 @@ __global__ void __launch_bounds__(256) k_vel_deriv(Geom g, Fields f, int i0, int
 // transport coefficients, centre gradients and limits of this GetLU
 // uat(c, n) is U_n and r1at(c) is 1 / U_0 (as GetStates forms it) of the
 // stencil cell c = 0..3 (i - 1, i, i + 1, i + 2 along DIR): from memory, or
-// from a sweep's staged tile.
+// from a sweep's staged tile.  The velocities are formed from them as
+// GetStates (and k_transport) form u, v, w.
 template <int NS, int DIR, int PREC, class UAt, class R1At>
 __device__ __forceinline__ void visc_face(const Geom &g, const Fields &f, long long id, UAt uat, R1At r1at, double *F)
 {
@@ -443,7 +444,13 @@ __device__ __forceinline__ void visc_face(const Geom &g, const Fields &f, long l
     const double DX = g.DimX, DY = g.DimY, DZ = g.DimZ;
     const double _dl = DIR == 0 ? g._dx : (DIR == 1 ? g._dy : g._dz);
     const double OT = 1.0 / 3.0, SX = 1.0 / 16.0, TF = 1.0 / 24.0;
-    const double *u = f.u, *v = f.v, *w = f.w;
+    // u, v, w of the four stencil cells (GetStates' U_{1..3} / rho)
+    double uu[4], vv[4], ww[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const double r1 = r1at(c);
+        uu[c] = uat(c, 1) * r1, vv[c] = uat(c, 2) * r1, ww[c] = uat(c, 3) * r1;
+    }
     const auto va = vplane<PREC>(f.mu), Vd = vplane<PREC>(f.Vde);
     auto half = [&](const auto &a) { return (9.0 * (a[p1] + a[id]) - (a[p2] + a[m1])) * SX; };
     auto grad = [&](const auto &a) { return (27.0 * (a[p1] - a[id]) - (a[p2] - a[m1])) * _dl * TF; };
@@ -455,37 +462,37 @@ __device__ __forceinline__ void visc_face(const Geom &g, const Fields &f, long l
     double f_x, f_y, f_z, u_hlf, v_hlf, w_hlf;
     if (DIR == 0) {
         const auto Dvcy = Vd.at(DVCY * N), Dwcz = Vd.at(DWCZ * N), Ducy = Vd.at(DUCY * N), Ducz = Vd.at(DUCZ * N);
-        f_x = (2.0 * mue + lamada) * (27.0 * (u[p1] - u[id]) - (u[p2] - u[m1])) * _dl * TF;
+        f_x = (2.0 * mue + lamada) * (27.0 * (uu[2] - uu[1]) - (uu[3] - uu[0])) * _dl * TF;
         f_x += lamada * pair2(Dvcy, Dwcz) * SX;
-        f_y = mue * (27.0 * (v[p1] - v[id]) - (v[p2] - v[m1])) * _dl * TF * DY;
+        f_y = mue * (27.0 * (vv[2] - vv[1]) - (vv[3] - vv[0])) * _dl * TF * DY;
         f_y += mue * (9.0 * (Ducy[p1] + Ducy[id]) - (Ducy[p2] + Ducy[m1])) * SX * DY;
-        f_z = mue * (27.0 * (w[p1] - w[id]) - (w[p2] - w[m1])) * _dl * TF * DZ;
+        f_z = mue * (27.0 * (ww[2] - ww[1]) - (ww[3] - ww[0])) * _dl * TF * DZ;
         f_z += mue * (9.0 * (Ducz[p1] + Ducz[id]) - (Ducz[p2] + Ducz[m1])) * SX * DZ;
-        u_hlf = (9.0 * (u[p1] + u[id]) - (u[p2] + u[m1])) * SX;
-        v_hlf = (9.0 * (v[p1] + v[id]) - (v[p2] + v[m1])) * SX * DY;
-        w_hlf = (9.0 * (w[p1] + w[id]) - (w[p2] + w[m1])) * SX * DZ;
+        u_hlf = (9.0 * (uu[2] + uu[1]) - (uu[3] + uu[0])) * SX;
+        v_hlf = (9.0 * (vv[2] + vv[1]) - (vv[3] + vv[0])) * SX * DY;
+        w_hlf = (9.0 * (ww[2] + ww[1]) - (ww[3] + ww[0])) * SX * DZ;
     } else if (DIR == 1) {
         const auto Dvcx = Vd.at(DVCX * N), Dvcz = Vd.at(DVCZ * N), Ducx = Vd.at(DUCX * N), Dwcz = Vd.at(DWCZ * N);
-        f_x = mue * (27.0 * (u[p1] - u[id]) - (u[p2] - u[m1])) * _dl * TF * DX;
+        f_x = mue * (27.0 * (uu[2] - uu[1]) - (uu[3] - uu[0])) * _dl * TF * DX;
         f_x += mue * (9.0 * (Dvcx[p1] + Dvcx[id]) - (Dvcx[p2] + Dvcx[m1])) * SX * DX;
-        f_y = (2.0 * mue + lamada) * (27.0 * (v[p1] - v[id]) - (v[p2] - v[m1])) * _dl * TF;
+        f_y = (2.0 * mue + lamada) * (27.0 * (vv[2] - vv[1]) - (vv[3] - vv[0])) * _dl * TF;
         f_y += lamada * pair2(Ducx, Dwcz) * SX;
-        f_z = mue * (27.0 * (w[p1] - w[id]) - (w[p2] - w[m1])) * _dl * TF * DZ;
+        f_z = mue * (27.0 * (ww[2] - ww[1]) - (ww[3] - ww[0])) * _dl * TF * DZ;
         f_z += mue * (9.0 * (Dvcz[p1] + Dvcz[id]) - (Dvcz[p2] + Dvcz[m1])) * SX * DZ;
-        u_hlf = (9.0 * (u[p1] + u[id]) - (u[p2] + u[m1])) * SX * DX;
-        v_hlf = (9.0 * (v[p1] + v[id]) - (v[p2] + v[m1])) * SX;
-        w_hlf = (9.0 * (w[p1] + w[id]) - (w[p2] + w[m1])) * SX * DZ;
+        u_hlf = (9.0 * (uu[2] + uu[1]) - (uu[3] + uu[0])) * SX * DX;
+        v_hlf = (9.0 * (vv[2] + vv[1]) - (vv[3] + vv[0])) * SX;
+        w_hlf = (9.0 * (ww[2] + ww[1]) - (ww[3] + ww[0])) * SX * DZ;
     } else {
         const auto Dwcx = Vd.at(DWCX * N), Dwcy = Vd.at(DWCY * N), Ducx = Vd.at(DUCX * N), Dvcy = Vd.at(DVCY * N);
-        f_x = mue * (27.0 * (u[p1] - u[id]) - (u[p2] - u[m1])) * _dl * TF * DX;
+        f_x = mue * (27.0 * (uu[2] - uu[1]) - (uu[3] - uu[0])) * _dl * TF * DX;
         f_x += mue * (9.0 * (Dwcx[p1] + Dwcx[id]) - (Dwcx[p2] + Dwcx[m1])) * SX * DX;
-        f_y = mue * (27.0 * (v[p1] - v[id]) - (v[p2] - v[m1])) * _dl * TF * DY;
+        f_y = mue * (27.0 * (vv[2] - vv[1]) - (vv[3] - vv[0])) * _dl * TF * DY;
         f_y += mue * (9.0 * (Dwcy[p1] + Dwcy[id]) - (Dwcy[p2] + Dwcy[m1])) * SX * DY;
-        f_z = (2.0 * mue + lamada) * (27.0 * (w[p1] - w[id]) - (w[p2] - w[m1])) * _dl * TF;
+        f_z = (2.0 * mue + lamada) * (27.0 * (ww[2] - ww[1]) - (ww[3] - ww[0])) * _dl * TF;
         f_z += lamada * pair2(Ducx, Dvcy) * SX;
-        u_hlf = (9.0 * (u[p1] + u[id]) - (u[p2] + u[m1])) * SX * DX;
-        v_hlf = (9.0 * (v[p1] + v[id]) - (v[p2] + v[m1])) * SX * DY;
-        w_hlf = (9.0 * (w[p1] + w[id]) - (w[p2] + w[m1])) * SX;
+        u_hlf = (9.0 * (uu[2] + uu[1]) - (uu[3] + uu[0])) * SX * DX;
+        v_hlf = (9.0 * (vv[2] + vv[1]) - (vv[3] + vv[0])) * SX * DY;
+        w_hlf = (9.0 * (ww[2] + ww[1]) - (ww[3] + ww[0])) * SX;
     }
     F[0] = 0.0;
     F[1] = f_x;

@@ -61,10 +61,10 @@ CHEM_PASS_EXP, CHEM_PASS_LOG = 38, 542
 # per-cell chemistry at 4.60 passes executes 24 279 per cell against
 # 9113 + 4.60 * 10 901 counted (it evaluates each species' Gibbs energy once
 # per pass, not once per reaction); the plain z sweep with its wall flux
-# executes 5364 per face against 10 486 + 567 (it skips the unused stencil
+# executes 5322 per face against 10 486 + 567 (it skips the unused stencil
 # ends and the zero terms)
 CHEM_EXECUTED_RATIO = 24279.0 / (9113.141 + 4.60 * 10900.994)
-SWEEP_E13_EXECUTED_RATIO = 5364.0 / (10486.0 + 567.0)
+SWEEP_E13_EXECUTED_RATIO = 5322.0 / (10486.0 + 567.0)
 # Compulsory HBM bytes per cell-update of a full reactive step, E = 13,
 # Ns = 9: fluid 8 * (8E + 6) plus two Strang half-steps 2 * 8 * ((E+2) + (Ns+1))
 ALG_BYTES_E13 = 8 * (8 * 13 + 6) + 2 * 8 * ((13 + 2) + (9 + 1))

@@ -550,22 +550,46 @@ __device__ __forceinline__ void sweep_tile(const Geom &g, const DevThermo *__res
     };
 
     // ---- stage U and the physical flux F of TS+5 cells per pencil ----------
-    for (int q = tid; q < NC; q += 256) {
-        const int pc = DIR == 0 ? q / (TS + 5) : q % TW;
-        const int ls = DIR == 0 ? q % (TS + 5) : q / TW;
-        bool ok;
-        const long long base = pencil_base(pc, ok);
-        const int s = c0 - 3 + ls;
-        const int li = lds_ix<DIR, TS, TW>(ls, pc);
-        if (ok && s < Lmax) {
-            const long long id = base + (long long)s * stride;
-            double U[E];
+    // A lane stages at most NIT cells; every cell's loads are issued before
+    // any of them is used, so the second cell's loads overlap the first's
+    constexpr int NIT = (NC + 255) / 256;
+    double Us[NIT][E], ps[NIT];
+    long long ids[NIT];
+    int lis[NIT];
+    bool oks[NIT];
 #pragma unroll
-            for (int n = 0; n < E; n++) U[n] = Ug[n * N + id];
+    for (int it = 0; it < NIT; it++) {
+        const int q = tid + 256 * it;
+        oks[it] = false;
+        lis[it] = 0;
+        ids[it] = 0;
+        if (q < NC) {
+            const int pc = DIR == 0 ? q / (TS + 5) : q % TW;
+            const int ls = DIR == 0 ? q % (TS + 5) : q / TW;
+            bool ok;
+            const long long base = pencil_base(pc, ok);
+            const int s = c0 - 3 + ls;
+            lis[it] = lds_ix<DIR, TS, TW>(ls, pc);
+            if (ok && s < Lmax) {
+                oks[it] = true;
+                ids[it] = base + (long long)s * stride;
+#pragma unroll
+                for (int n = 0; n < E; n++) Us[it][n] = Ug[n * N + ids[it]];
+                ps[it] = f.p[ids[it]];
+            }
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < NIT; it++) {
+        if (tid + 256 * it >= NC) continue;
+        const int li = lis[it];
+        if (oks[it]) {
+            const long long id = ids[it];
+            const double *U = Us[it];
             // velocities exactly as GetStates: u = U1 * (1 / rho)
             const double rho1 = 1.0 / U[0];
             const double vel[3] = {U[1] * rho1, U[2] * rho1, U[3] * rho1};
-            const double p = f.p[id];
+            const double p = ps[it];
             const double Um = U[1 + DIR];
             sP[li] = p;
             sR1[li] = rho1;

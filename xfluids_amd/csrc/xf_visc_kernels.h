@@ -173,6 +173,8 @@ __global__ void __launch_bounds__(256) k_transport(Geom g, const DevThermo *__re
     const long long N = g.N;
     const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = id < N;
+    // T and p, loaded ahead of the block's reduction barrier
+    const double T = live ? f.T[id] : 0.0, p = live ? f.p[id] : 0.0;
     // mass fractions (Getrhoyi), velocities (GetStates) and mole fractions
     // (get_xi); the mass fractions feed the min / max and are then dropped
     double X[NS], C_total = 0.0, rho = 1.0, rho1 = 1.0;
@@ -229,7 +231,6 @@ __global__ void __launch_bounds__(256) k_transport(Geom g, const DevThermo *__re
         }
     }
     if (!live) return;
-    const double T = f.T[id], p = f.p[id];
     const double lT = dlog(T);
     if constexpr (PREC == 1) {
         transport_mixed<NS>(th, tr, f, id, N, X, C_total, rho, rho1, p, lT, heat, diffu);

@@ -124,3 +124,77 @@ def field_errors(s, Uo, To, Ug, Tg):
         out["Y_range_oracle"] = [float(Yo.min()), float(Yo.max())]
         out["Y_range_gpu"] = [float(Yg.min()), float(Yg.max())]
     return out
+
+
+# ---- whole-cell shifts along periodic axes -----------------------------------
+def grid_shape(d, E=None):
+    shp = (d.Zmax, d.Ymax, d.Xmax)
+    return shp + (E,) if E else shp
+
+
+def periodic_axes(d, which):
+    """(array axis, ghost width, interior cells) of the named axes ("x", "y", "z")."""
+    spec = {"x": (2, d.Bwidth_X, d.X_inner), "y": (1, d.Bwidth_Y, d.Y_inner), "z": (0, d.Bwidth_Z, d.Z_inner)}
+    return [spec[w] for w in which]
+
+
+def _axis_slice(ax, lo, hi, ndim):
+    s = [slice(None)] * ndim
+    s[ax] = slice(lo, hi)
+    return tuple(s)
+
+
+def periodic_fill(A, axes):
+    """Ghost layers of a padded (Zmax, Ymax, Xmax[, E]) array along each
+    periodic axis, copied from the opposite interior edge (in place)."""
+    for ax, B, n in axes:
+        A[_axis_slice(ax, 0, B, A.ndim)] = A[_axis_slice(ax, n, n + B, A.ndim)]
+        A[_axis_slice(ax, B + n, 2 * B + n, A.ndim)] = A[_axis_slice(ax, B, 2 * B, A.ndim)]
+    return A
+
+
+def roll_interior(A, axes, shifts):
+    """A copy with the interior rolled by whole cells along each periodic axis
+    (all rows of the other axes, ghosts included), ghosts refilled."""
+    A = A.copy()
+    for (ax, B, n), s in zip(axes, shifts):
+        sl = _axis_slice(ax, B, B + n, A.ndim)
+        A[sl] = np.roll(A[sl], s, axis=ax)
+    return periodic_fill(A, axes)
+
+
+def interior(A, d):
+    return A[(slice(d.Bwidth_Z, d.Bwidth_Z + d.Z_inner), slice(d.Bwidth_Y, d.Bwidth_Y + d.Y_inner),
+              slice(d.Bwidth_X, d.Bwidth_X + d.X_inner))]
+
+
+def shifted_pair(s, U0, T0, which, shifts):
+    """The starting state with its ghosts filled periodically along `which`,
+    and the same state rolled by `shifts` cells: (Ua, Ta), (Ub, Tb), flat."""
+    d = s.desc
+    axes = periodic_axes(d, which)
+    Ua = periodic_fill(np.asarray(U0).reshape(grid_shape(d, d.E)).copy(), axes)
+    Ta = periodic_fill(np.asarray(T0).reshape(grid_shape(d)).copy(), axes)
+    Ub, Tb = roll_interior(Ua, axes, shifts), roll_interior(Ta, axes, shifts)
+    return (Ua.ravel(), Ta.ravel()), (Ub.ravel(), Tb.ravel())
+
+
+def unshift_compare(s, which, shifts, Ra, Rb, Rb0, E=1):
+    """Interior of run a's result rolled like b's start against run b's
+    result, for a field of E components (U: E = d.E; T: 1): (values that
+    differ, max |diff| / max |value| per component -- the momenta against the
+    largest momentum component, the partial densities against the density --,
+    whether b's field moved away from its start Rb0)."""
+    d = s.desc
+    axes = periodic_axes(d, which)
+    shp = grid_shape(d, E if E > 1 else None)
+    A = interior(roll_interior(np.asarray(Ra).reshape(shp), axes, shifts), d).reshape(-1, E)
+    B = interior(np.asarray(Rb).reshape(shp), d).reshape(-1, E)
+    B0 = interior(np.asarray(Rb0).reshape(shp), d).reshape(-1, E)
+    neq = int(np.count_nonzero(A.view(np.uint64) != B.view(np.uint64)))
+    scale = np.abs(B).max(axis=0)
+    if E >= 5:  # the momenta against the largest of them, partial densities against rho
+        scale[1:4] = scale[1:4].max()
+        scale[5:] = scale[0]
+    rel = np.abs(A - B).max(axis=0) / np.maximum(scale, 1e-300)
+    return neq, rel, not np.array_equal(B, B0)

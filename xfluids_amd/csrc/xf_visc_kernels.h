@@ -157,15 +157,26 @@ __device__ __forceinline__ void transport_mixed(const DevThermo *__restrict__ th
     }
 }
 
+// Waves per SIMD k_transport is compiled for (1: no bound).  Unbounded, the
+// FP64 form at nine species takes 194 registers and runs two waves per SIMD,
+// with the FP64 divisions, square roots and exponentials waiting on each
+// other; held to three waves (168 registers, 96 bytes per lane spilled) it
+// takes 15 % less time, and four waves (256 bytes spilled) gain nothing more.
+// At ten and eleven species a three-wave bound spills 272 and 432 bytes, and
+// the mixed form already runs five waves.
+template <int NS, int PREC> struct TransportOccupancy {
+    static constexpr int waves = (PREC == 0 && NS == 9) ? 3 : 1;
+};
+
 // Gettransport_coeff_aver + Get_transport_coeff_aver (Visc_kernels.hpp:205-235,
 // Visc_device.h:99-177) with get_xi (Mixing_device.h:31-44) on every padded
 // cell; also GetStates' u, v, w (Update_device.hpp:36) and, with diffusion,
 // each block's min / max of the interior mass fractions
 // (ConVenction_block.hpp:462-490) into ypart[block][2 NS]
 template <int NS, int PREC>
-__global__ void __launch_bounds__(256) k_transport(Geom g, const DevThermo *__restrict__ th,
-                                                   const DevTransport *__restrict__ tr, const double *__restrict__ Ug,
-                                                   Fields f, int heat, int diffu)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TransportOccupancy<NS, PREC>::waves)))
+k_transport(Geom g, const DevThermo *__restrict__ th, const DevTransport *__restrict__ tr,
+            const double *__restrict__ Ug, Fields f, int heat, int diffu)
 {
     constexpr int E = NS + 4;
     constexpr int NCOP = NS - 1;
